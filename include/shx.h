@@ -1,0 +1,143 @@
+/*
+ * shx.h — C ABI of the MI355X (gfx950) nonsubsampled shearlet transform.
+ *
+ * Drop-in boundary for the 2D NSST hot path of ShearCuda.  Every entry point
+ * takes plain device pointers (FP32, row-major L x L planes) and sizes, plus a
+ * HIP stream (hipStream_t passed as void*; NULL = the default stream), and
+ * returns SHX_OK or a negative status; shx_last_error() then holds a message.
+ * No entry point exits the process.
+ *
+ * Coefficient layout ("cells", as in the MATLAB cell array the reference
+ * returns, shearcuda/src_cuda/shear_trans_cuda.cpp:204-216): a host array of
+ * S+1 device pointers.  Cell 0 is the lowpass (one L x L plane); cell j
+ * (1..S, coarse to fine) holds D_j contiguous L x L planes, direction k at
+ * offset k*L*L.  Array element [r][c] is MATLAB element (r+1, c+1).
+ *
+ * Thread-safety: a dictionary owns its workspaces; use one dictionary from one
+ * stream at a time.  Distinct dictionaries are independent.
+ */
+#ifndef SHX_H
+#define SHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHX_OK 0
+#define SHX_ERR_ARG (-1)        /* invalid argument (size, count, pointer)   */
+#define SHX_ERR_HIP (-2)        /* HIP runtime error                        */
+#define SHX_ERR_ALLOC (-3)      /* device allocation failed                 */
+#define SHX_ERR_UNSUPPORTED (-4)/* unsupported size or option               */
+
+#define SHX_SHIFT_NONE 0   /* fft2(w, L, L): shearlet_toolbox_1/nsst_dec2.m:50 */
+#define SHX_SHIFT_CENTER 1 /* padded filter rolled by -n1/2: the MEX GPU path,
+                              prepare_shearing_filters_cuda.cpp:254            */
+
+typedef struct shx_dict shx_dict;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* shx_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int shx_version(void);
+
+/* Shearlet dictionary: normalised FP32 spectra of the directional filters of
+ * every scale (computed in FP64 on the host), FFT twiddles and workspaces.
+ * Replaces prepare_shearing_filters_cuda(w_s, L)
+ * (shearcuda/src_cuda/prepare_shearing_filters_cuda.cpp:61-249).
+ *   L        image side, power of two in [32, 1024]
+ *   nscales  number of directional scales S (1..8)
+ *   ndir[j]  directions of scale j (power of two, 1..32)
+ *   dsize[j] spatial filter side n1 of scale j (<= L)
+ *   w_s[j]   host FP64 array ndir[j] x dsize[j] x dsize[j] (row-major)
+ *   shift    SHX_SHIFT_NONE or SHX_SHIFT_CENTER */
+int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s,
+                                 int shift, shx_dict** out);
+/* Frees a dictionary (clear_shearing_filters_cuda.cpp:51-96). NULL is a no-op. */
+int shx_clear_shearing_filters(shx_dict* dict);
+/* Dictionary shape: L, S and ndir[0..S-1] (ndir may be NULL). */
+int shx_dict_info(const shx_dict* dict, int* L, int* nscales, int* ndir);
+/* Device bytes held by the dictionary (spectra + workspaces). */
+size_t shx_dict_bytes(const shx_dict* dict);
+
+/* Forward NSST: atrousdec + per-scale FFT x filter bank -> bands.
+ * Replaces shear_trans_cuda(x, 'maxflat', shear, Ct)
+ * (shearcuda/src_cuda/shear_trans_cuda.cpp:56-217, convolution :220-287). */
+int shx_shear_trans(shx_dict* dict, const float* x, float* const* coeff, void* stream);
+
+/* Inverse NSST.  Replaces inverse_shear_cuda(Ct, 'maxflat', shear)
+ * (shearcuda/src_cuda/inverse_shear_cuda.cpp:57-219, applyfilters :222-286). */
+int shx_inverse_shear(shx_dict* dict, const float* const* coeff, float* x, void* stream);
+
+/* Inverse NSST of hard-thresholded bands without modifying them: band b
+ * (b = 0 lowpass, then scale-major, direction-minor) is read as
+ * c * (|c| > th[b]).  th has 1 + sum(ndir) entries (host memory).  The
+ * threshold is fused into the reconstruction's first read of each band. */
+int shx_inverse_shear_thresh(shx_dict* dict, const float* const* coeff, const float* th, float* x, void* stream);
+
+/* Plain denoise: shear_trans, nsst_HT thresholds T*sigma*s (T = Tscalars[0]
+ * lowpass, [1] scales 1..S-1, [2] finest; shearlet_toolbox_1/nsst_HT.m:32-47)
+ * fused into the inverse.  scalars: 1 + sum(ndir) per-band noise scalars
+ * (lowpass first).  coeff: caller-owned band storage (receives the
+ * unthresholded coefficients). */
+int shx_denoise(shx_dict* dict, const float* x, float* const* coeff, double sigma, const double* Tscalars,
+                const double* scalars, float* out, void* stream);
+
+/* Measurement hook: runs shx_denoise `reps` times with a HIP event after
+ * every kernel launch on `stream` and returns each launch's mean duration
+ * (stage_ms, milliseconds), the bytes it must move at minimum (stage_bytes:
+ * each FP32 input and output it touches, once) and its name (names: max_stages
+ * strings of name_len chars).  *nstages receives the number of launches.
+ * No counterpart in the reference (its GpuTimes singleton,
+ * shearcuda/src_cuda/GpuTimes.cpp:59-127, timed host calls with device
+ * syncs). */
+int shx_denoise_profile(shx_dict* dict, const float* x, float* const* coeff, double sigma, const double* Tscalars,
+                        const double* scalars, float* out, int reps, int max_stages, double* stage_ms,
+                        double* stage_bytes, char* names, int name_len, int* nstages, void* stream);
+
+/* In-place hard threshold of coefficient cells, thresh_cuda(Ct, lambda,
+ * option, E, sc, opt) (shearcuda/src_cuda/thresh_cuda.cpp:57-217,
+ * shearcuda/thresh.m:28-55).
+ *   cells[i] holds cell_planes[i] planes of plane_elems[i] floats.
+ *   option 1 (wavelets): only cells[1], th = lambda.
+ *   option 2 (shearlets): th(s,w) = sc[s] * lambda * E[s + w*E_rows]
+ *       (E column-major E_rows x E_cols doubles, E_rows = ncells).
+ *   opt 1: *maxi = max over thresholded bands of max|c|/E (else 0). */
+int shx_thresh(float* const* cells, int ncells, const int* cell_planes, const int64_t* plane_elems, double lambda,
+               int option, const double* E, int E_rows, int E_cols, const double* sc, int n_sc, int opt,
+               double* maxi, void* stream);
+
+/* A trous maxflat pyramid on an L x L image (L power of two >= 32):
+ * y[0] lowpass, y[1..nlevels] coarse -> fine; each an L x L device plane.
+ * Replaces atrousdec / atrousrec (shearcuda/src_cuda/atrousdec.cpp:41-131,
+ * atrousrec.cpp:42-123; demo_denoise/shearcuda/shearcuda.cpp:401-546).
+ * work: 2 planes of scratch (L*L floats each) or NULL (allocated per call). */
+int shx_atrousdec(const float* x, int L, int nlevels, float* const* y, float* work, void* stream);
+int shx_atrousrec(const float* const* y, int L, int nlevels, float* x, float* work, void* stream);
+
+/* re_weight_band_cuda(x, coeff): atrousrec(coeff(j) .* atrousdec(x)) with
+ * numel(coeff)-1 levels (shearcuda/src_cuda/re_weight_band_cuda.cpp:59-183).
+ * work: ncoeff + 2 planes or NULL (allocated per call). */
+int shx_re_weight_band(const float* x, int L, const double* coeff, int ncoeff, float* y, float* work, void* stream);
+
+/* Sum |x|^p over n elements, result (p-th root not taken) in *out (host).
+ * norm_cuda's reduction (shearcuda/src_cuda/norm_cuda.cpp:172-215). */
+int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream);
+
+/* In-place soft threshold sign(v)*max(|v|-t, 0) (softth_cuda.cpp:49-112). */
+int shx_softth(float* x, int64_t n, double t, void* stream);
+
+/* Largest |h - sum_k v_k w_k^T| of the separable factors the pyramid kernels
+ * use for the four maxflat filters (FP64; runs on the host, no GPU needed). */
+double shx_maxflat_factor_error(void);
+/* Copy maxflat filter which (0 h0, 1 h1, 2 g0, 3 g1) into out (row-major,
+ * len*len doubles, len returned); out may be NULL to query len. */
+int shx_maxflat_filter(int which, double* out, int* len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHX_H */

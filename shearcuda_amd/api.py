@@ -1,0 +1,330 @@
+"""Host-side mirror of ShearCuda's MATLAB entry points on GPU tensors.
+
+Each function keeps the reference's name, argument meaning and error text and
+calls the C ABI (include/shx.h); device memory and streams come from PyTorch
+(tensors on ``cuda``), which is plumbing only.  Inputs must already be FP32
+tensors on the GPU: unlike the reference MEX files, which forward CPU arrays to
+the MATLAB implementation (e.g. shearcuda/src_cuda/shear_trans_cuda.cpp:94-98),
+there is no CPU path here and CPU inputs raise TypeError.
+
+Arrays are row-major: element [r, c] is MATLAB element (r+1, c+1).  Scale j
+of a coefficient list is a (D_j, L, L) tensor (MATLAB's L x L x D_j cell
+element); cell 0 is the (L, L) lowpass.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr_array
+from .filters import DEFAULT_DCOMP, DEFAULT_DSIZE, DEFAULT_TSCALARS, dictionary_filters
+
+SHIFT_NONE = 0
+SHIFT_CENTER = 1
+_SHIFTS = {"none": SHIFT_NONE, "center": SHIFT_CENTER}
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _require_gpu(x, name="x"):
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
+        raise TypeError("%s must be a FP32 tensor on the GPU (no CPU path)" % name)
+    if x.dtype != torch.float32:
+        raise TypeError("%s: Signal should be GPUsingle (float32), got %s" % (name, x.dtype))
+    if not x.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+
+
+class ShearDict:
+    """A prepared shearlet dictionary (the ``shear`` struct of
+    prepare_shearing_filters_cuda.cpp:234-248): FP32 spectra on the GPU plus
+    workspaces.  Free it with clear_shearing_filters_cuda() or close()."""
+
+    def __init__(self, handle, L, ndir, dsize, shift, w_s, device):
+        self._h = handle
+        self.L = L
+        self.ndir = list(ndir)
+        self.dsize = list(dsize)
+        self.shift = shift
+        self.w_s = w_s
+        self.device = device
+        self.atrousfilter = "maxflat"
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("Invalid shearlet dictionary (already cleared)")
+        return self._h
+
+    @property
+    def nscales(self):
+        return len(self.ndir)
+
+    @property
+    def nbands(self):
+        return 1 + sum(self.ndir)
+
+    def nbytes(self):
+        return int(lib.shx_dict_bytes(self.handle))
+
+    def close(self):
+        if self._h is not None:
+            lib.shx_clear_shearing_filters(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def alloc_coeffs(self):
+        """Empty coefficient cells for this dictionary (Ct of shear_trans_cuda)."""
+        L = self.L
+        dev = self.device
+        return [torch.empty((L, L), device=dev, dtype=torch.float32)] + [
+            torch.empty((d, L, L), device=dev, dtype=torch.float32) for d in self.ndir]
+
+
+def prepare_shearing_filters_cuda(w_s, L, shift="none"):
+    """shear = prepare_shearing_filters_cuda(w_s, L)
+    (shearcuda/src_cuda/prepare_shearing_filters_cuda.cpp:61-249).
+
+    w_s: list over scales of (D_j, n1_j, n1_j) FP64 arrays.
+    shift: 'none' (toolbox convention, nsst_dec2.m:50) or 'center' (the MEX GPU
+    path's circular n1/2 shift, prepare_shearing_filters_cuda.cpp:254)."""
+    if shift not in _SHIFTS:
+        raise ValueError("shift must be 'none' or 'center'")
+    w_s = [np.ascontiguousarray(np.asarray(w, dtype=np.float64)) for w in w_s]
+    for w in w_s:
+        if w.ndim != 3 or w.shape[1] != w.shape[2]:
+            raise ValueError("each w_s{j} must be (D, n1, n1)")
+    S = len(w_s)
+    ndir = (ctypes.c_int * S)(*[w.shape[0] for w in w_s])
+    dsize = (ctypes.c_int * S)(*[w.shape[1] for w in w_s])
+    ptrs = (ctypes.POINTER(ctypes.c_double) * S)(
+        *[w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for w in w_s])
+    handle = ctypes.c_void_p()
+    torch.cuda.init()
+    check(lib.shx_prepare_shearing_filters(int(L), S, ndir, dsize, ptrs, _SHIFTS[shift], ctypes.byref(handle)))
+    return ShearDict(handle, int(L), list(ndir), list(dsize), shift, w_s, torch.device("cuda", torch.cuda.current_device()))
+
+
+def shearing_filters_Myer(m, num, L, dataType="GPUsingle", shift="none"):
+    """shear = shearing_filters_Myer(dsize, dcomp, L, 'GPUsingle')
+    (shearcuda/shearing_filters_Myer.m:29-81, GPU branch)."""
+    if dataType not in ("GPUsingle",):
+        raise ValueError("only 'GPUsingle' dictionaries exist on this path")
+    return prepare_shearing_filters_cuda(dictionary_filters(num, m), L, shift)
+
+
+def clear_shearing_filters_cuda(shear):
+    """clear_shearing_filters_cuda(shear) (clear_shearing_filters_cuda.cpp:51-96)."""
+    shear.close()
+
+
+def _check_pfilt(pfilt):
+    if pfilt != "maxflat":
+        raise ValueError("only the 'maxflat' pyramid filters are supported on this path")
+
+
+def shear_trans_cuda(x, pfilt, shear, Ct=None):
+    """Ct = shear_trans_cuda(x, 'maxflat', shear [, Ct])
+    (shearcuda/src_cuda/shear_trans_cuda.cpp:56-217)."""
+    _check_pfilt(pfilt)
+    _require_gpu(x)
+    L = shear.L
+    if x.dim() != 2:
+        raise ValueError("Input data should be 2-dimensional")
+    if x.shape[0] != x.shape[1]:
+        raise ValueError("Input data should be a square")
+    if x.shape[0] != L:
+        raise ValueError("Input data size does not match dictionary size")
+    if Ct is None:
+        Ct = shear.alloc_coeffs()
+    elif len(Ct) != shear.nscales + 1:
+        raise ValueError("Incorrect number of cell elements")
+    for i, c in enumerate(Ct):
+        _require_gpu(c, "Ct{%d}" % (i + 1))
+    check(lib.shx_shear_trans(shear.handle, ctypes.c_void_p(x.data_ptr()),
+                              ptr_array([c.data_ptr() for c in Ct]), _stream()))
+    return Ct
+
+
+def inverse_shear_cuda(Ct, pfilt, shear, out=None):
+    """x = inverse_shear_cuda(Ct, 'maxflat', shear)
+    (shearcuda/src_cuda/inverse_shear_cuda.cpp:57-219)."""
+    _check_pfilt(pfilt)
+    if len(Ct) != shear.nscales + 1:
+        raise ValueError("Incorrect number of cell elements")
+    for i, c in enumerate(Ct):
+        _require_gpu(c, "Ct{%d}" % (i + 1))
+    if out is None:
+        out = torch.empty((shear.L, shear.L), device=Ct[0].device, dtype=torch.float32)
+    check(lib.shx_inverse_shear(shear.handle, ptr_array([c.data_ptr() for c in Ct]),
+                                ctypes.c_void_p(out.data_ptr()), _stream()))
+    return out
+
+
+def inverse_shear_thresh_cuda(Ct, th, shear, out=None):
+    """Inverse NSST of hard-thresholded bands (thresholds fused into the
+    reconstruction's first read; Ct is not modified).  th: 1 + sum(D_j)
+    thresholds, lowpass first."""
+    th = np.ascontiguousarray(np.asarray(th, dtype=np.float32))
+    if th.size != shear.nbands:
+        raise ValueError("need %d thresholds" % shear.nbands)
+    if out is None:
+        out = torch.empty((shear.L, shear.L), device=Ct[0].device, dtype=torch.float32)
+    check(lib.shx_inverse_shear_thresh(shear.handle, ptr_array([c.data_ptr() for c in Ct]),
+                                       th.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       ctypes.c_void_p(out.data_ptr()), _stream()))
+    return out
+
+
+def band_thresholds(shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS):
+    """nsst_HT thresholds T*sigma*s per band (shearlet_toolbox_1/nsst_HT.m:32-47),
+    lowpass first.  scalars: [s_low, array(D_1), ..., array(D_S)]."""
+    S = shear.nscales
+    th = [Tscalars[0] * sigma * float(scalars[0])]
+    for j in range(S):
+        T = Tscalars[1] if j < S - 1 else Tscalars[2]
+        th.extend(T * sigma * np.asarray(scalars[j + 1], dtype=np.float64))
+    return np.array(th, dtype=np.float64)
+
+
+def denoise_cuda(x, shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS, Ct=None, out=None):
+    """Plain NSST hard-threshold denoise: shear_trans + nsst_HT + inverse with
+    the threshold fused into the inverse.  Returns (x_hat, Ct)."""
+    _require_gpu(x)
+    if Ct is None:
+        Ct = shear.alloc_coeffs()
+    if out is None:
+        out = torch.empty_like(x)
+    sc = np.concatenate([[float(scalars[0])]] + [np.asarray(s, dtype=np.float64) for s in scalars[1:]])
+    sc = np.ascontiguousarray(sc, dtype=np.float64)
+    T = np.ascontiguousarray(np.asarray(Tscalars, dtype=np.float64))
+    check(lib.shx_denoise(shear.handle, ctypes.c_void_p(x.data_ptr()), ptr_array([c.data_ptr() for c in Ct]),
+                          float(sigma), T.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                          sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.c_void_p(out.data_ptr()),
+                          _stream()))
+    return out, Ct
+
+
+def thresh_cuda(Ct, lam, option, E, sc, opt):
+    """maxi = thresh_cuda(Ct, lambda, option, E, sc, opt), in place
+    (shearcuda/src_cuda/thresh_cuda.cpp:57-217)."""
+    for i, c in enumerate(Ct):
+        _require_gpu(c, "Ct{%d}" % (i + 1))
+    E = np.asarray(E, dtype=np.float64)
+    if E.ndim == 1:
+        E = E[:, None]
+    sc = np.ascontiguousarray(np.asarray(sc, dtype=np.float64).ravel())
+    Ecm = np.asfortranarray(E)
+    planes = (ctypes.c_int * len(Ct))(*[c.shape[0] if c.dim() == 3 else 1 for c in Ct])
+    elems = (ctypes.c_int64 * len(Ct))(*[c.shape[-1] * c.shape[-2] for c in Ct])
+    maxi = ctypes.c_double(0.0)
+    Eflat = np.ascontiguousarray(Ecm.ravel(order="F"))
+    check(lib.shx_thresh(ptr_array([c.data_ptr() for c in Ct]), len(Ct), planes, elems, float(lam), int(option),
+                         Eflat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), E.shape[0], E.shape[1],
+                         sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), sc.size, int(opt), ctypes.byref(maxi),
+                         _stream()))
+    return maxi.value
+
+
+def atrousdec_cuda(x, pfilt, nlevels):
+    """y = atrousdec(x, 'maxflat', nlevels) on the GPU
+    (shearcuda/src_cuda/atrousdec.cpp:41-131): [lowpass, coarse ... fine]."""
+    _check_pfilt(pfilt)
+    _require_gpu(x)
+    L = x.shape[0]
+    y = [torch.empty_like(x) for _ in range(nlevels + 1)]
+    check(lib.shx_atrousdec(ctypes.c_void_p(x.data_ptr()), L, int(nlevels), ptr_array([t.data_ptr() for t in y]),
+                            None, _stream()))
+    return y
+
+
+def atrousrec_cuda(y, pfilt):
+    """x = atrousrec(y, 'maxflat') on the GPU (shearcuda/src_cuda/atrousrec.cpp:42-123)."""
+    _check_pfilt(pfilt)
+    for t in y:
+        _require_gpu(t, "y")
+    out = torch.empty_like(y[0])
+    check(lib.shx_atrousrec(ptr_array([t.data_ptr() for t in y]), y[0].shape[0], len(y) - 1,
+                            ctypes.c_void_p(out.data_ptr()), None, _stream()))
+    return out
+
+
+def re_weight_band_cuda(x, coeff, shear=None):
+    """y = re_weight_band_cuda(x, coeff [, shear])
+    (shearcuda/src_cuda/re_weight_band_cuda.cpp:59-183); always 'maxflat'."""
+    _require_gpu(x)
+    if x.dim() != 2 or x.shape[0] != x.shape[1]:
+        raise ValueError("Input data should be a square")
+    coeff = np.ascontiguousarray(np.asarray(coeff, dtype=np.float64).ravel())
+    if coeff.size <= 1:
+        raise ValueError("There should be at least one band")
+    out = torch.empty_like(x)
+    check(lib.shx_re_weight_band(ctypes.c_void_p(x.data_ptr()), x.shape[0],
+                                 coeff.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), coeff.size,
+                                 ctypes.c_void_p(out.data_ptr()), None, _stream()))
+    return out
+
+
+def norm_cuda(x, p, idx=None):
+    """norm_cuda(x, p, idx): (sum |x(:,:,idx)|^p)^(1/p)
+    (shearcuda/src_cuda/norm_cuda.cpp:57-215); idx is 1-based like MATLAB."""
+    _require_gpu(x)
+    t = x if idx is None else x[idx - 1]
+    t = t.contiguous()
+    s = ctypes.c_double(0.0)
+    check(lib.shx_norm_pow(ctypes.c_void_p(t.data_ptr()), t.numel(), float(p), ctypes.byref(s), _stream()))
+    return s.value ** (1.0 / p)
+
+
+def softth_cuda(x, t):
+    """softth_cuda(x, t): in-place sign(x).*max(|x|-t,0) (softth_cuda.cpp:49-112)."""
+    _require_gpu(x)
+    check(lib.shx_softth(ctypes.c_void_p(x.data_ptr()), x.numel(), float(t), _stream()))
+    return x
+
+
+def com_norm(pfilt, n, shear):
+    """E = com_norm(pfilt, n, shear): exact band norms from the transform of a
+    scaled impulse (shearcuda/com_norm.m:25-55).  Returns (S+1, maxD) FP64."""
+    _check_pfilt(pfilt)
+    L = int(n[0]) if np.ndim(n) else int(n)
+    X = np.zeros((L, L), dtype=np.float32)
+    X[L // 2, L // 2] = L   # fftshift(real(ifft2(ones))) * sqrt(L*L)
+    C = shear_trans_cuda(torch.from_numpy(X).to(shear.device), pfilt, shear)
+    E = np.zeros((len(C), max(shear.ndir)))
+    for s, c in enumerate(C):
+        c = c if c.dim() == 3 else c[None]
+        for w in range(c.shape[0]):
+            E[s, w] = norm_cuda(c, 2, w + 1) / np.sqrt(c.shape[-1] * c.shape[-2])
+    return E
+
+
+def nsst_scalars_cuda(shear, noise):
+    """Per-band MAD/0.6745 noise scalars of the transform of a unit white-noise
+    image (shearlet_toolbox_1/nsst_scalars.m:28-45); the median runs on the
+    host, as demo_denoise/shearcuda/shearcuda.cpp:241-268 does.
+    Returns [s_low, array(D_1), ..., array(D_S)]."""
+    C = shear_trans_cuda(noise, "maxflat", shear)
+
+    def mad(v):
+        v = v.ravel().astype(np.float64)
+        return np.median(np.abs(v - np.median(v))) / 0.6745
+
+    out = [mad(C[0].cpu().numpy())]
+    for c in C[1:]:
+        a = c.cpu().numpy()
+        out.append(np.array([mad(a[k]) for k in range(a.shape[0])]))
+    return out
+
+
+def make_dictionary(L, dcomp=DEFAULT_DCOMP, dsize=DEFAULT_DSIZE, shift="none"):
+    """Shortcut: spatial Meyer filters + prepare_shearing_filters_cuda."""
+    return prepare_shearing_filters_cuda(dictionary_filters(dcomp, dsize), L, shift)

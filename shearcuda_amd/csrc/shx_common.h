@@ -1,0 +1,38 @@
+// Shared device/host helpers for the shx HIP library (gfx950, FP32).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#ifndef SHX_MAX_SCALES
+#define SHX_MAX_SCALES 8          // directional scales per dictionary
+#endif
+#define SHX_MAX_DIRS 32           // directions per scale (2^5)
+#define SHX_MAX_BANDS (SHX_MAX_SCALES * SHX_MAX_DIRS)
+
+namespace shx {
+
+// ---- complex helpers (float2 = re, im) -------------------------------------
+__host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__host__ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__host__ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__host__ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by +i (dir>0) or -i (dir<0)
+template <int DIR>
+__host__ __device__ __forceinline__ float2 cmul_i(float2 a) {
+  return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// Hard threshold used everywhere on the path: keep iff |v| > th (strict).
+__device__ __forceinline__ float hard_keep(float v, float th) { return fabsf(v) > th ? v : 0.0f; }
+
+__host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+
+}  // namespace shx
