@@ -51,48 +51,39 @@ def cpu_baseline(L, sigma, budget_s):
                 n, L, L, el)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle work (0 = skip)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the NSST path has no multi-GPU form here); "
-              "run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
 
+
+def plain_setup(L, sigma, seed=0):
     import numpy as np
     import torch
 
     from shearcuda_amd import api
-    from shearcuda_amd._lib import check, lib, ptr_array
     from shearcuda_amd.synth import noisy_ellipses, white_noise
-
-    L, sigma = args.size, 20.0
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
     shear = api.make_dictionary(L)
-    _, x = noisy_ellipses(L, sigma=sigma, seed=0)
-    xg = torch.from_numpy(x.astype(np.float32)).to(dev)
-    sc = api.nsst_scalars_cuda(shear, torch.from_numpy(white_noise(L).astype(np.float32)).to(dev))
+    _, x = noisy_ellipses(L, sigma=sigma, seed=seed)
+    xg = torch.from_numpy(x.astype(np.float32)).cuda()
+    sc = api.nsst_scalars_cuda(shear, torch.from_numpy(white_noise(L).astype(np.float32)).cuda())
+    return shear, xg, sc
+
+
+def profile_plain(shear, xg, sc, sigma, reps):
+    """Per-kernel mean durations of one plain denoise step (HIP events)."""
+    import numpy as np
+    import torch
+
+    from shearcuda_amd._lib import check, lib, ptr_array
     Ct = shear.alloc_coeffs()
     out = torch.empty_like(xg)
-
-    for _ in range(args.warmup):
-        api.denoise_cuda(xg, shear, sigma, sc, Ct=Ct, out=out)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        api.denoise_cuda(xg, shear, sigma, sc, Ct=Ct, out=out)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    ms = el * 1e3 / args.steps
-    value = L * L * args.steps / el / 1e6
-
-    # per-kernel durations with HIP events on the launch stream
     MAXS, NL = 64, 32
     stage_ms = (ctypes.c_double * MAXS)()
     stage_b = (ctypes.c_double * MAXS)()
@@ -100,7 +91,6 @@ def main():
     nst = ctypes.c_int()
     scv = np.concatenate([[float(sc[0])]] + [np.asarray(s, dtype=np.float64) for s in sc[1:]])
     T = np.array([0.0, 3.0, 4.0])
-    reps = max(10, args.steps)
     check(lib.shx_denoise_profile(shear.handle, ctypes.c_void_p(xg.data_ptr()), ptr_array([c.data_ptr() for c in Ct]),
                                   sigma, T.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                   scv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.c_void_p(out.data_ptr()),
@@ -111,6 +101,86 @@ def main():
         nm = names.raw[i * NL:(i + 1) * NL].split(b"\0", 1)[0].decode()
         stages.append({"name": nm, "ms": stage_ms[i], "bytes": stage_b[i],
                        "GBs": stage_b[i] / (stage_ms[i] * 1e-3) / 1e9 if stage_ms[i] > 0 else 0.0})
+    return stages
+
+
+def bench_plain(L, sigma, steps, warmup):
+    from shearcuda_amd import api
+    shear, xg, sc = plain_setup(L, sigma)
+    g, Ct, out = api.denoise_graph(xg, shear, sigma, sc)
+    el = _timed(g.launch, steps, warmup)
+    stages = profile_plain(shear, xg, sc, sigma, max(20, steps))
+    g.close()
+    shear.close()
+    return el * 1e3 / steps, stages
+
+
+# mixed-size batch: 48 x 256^2, 32 x 512^2, 16 x 1024^2, sigma ~ U[5, 40]
+BATCH_MIX = [(256, 48), (512, 32), (1024, 16)]
+
+
+def bench_batched(steps, warmup, mix=BATCH_MIX):
+    """One graph launch denoises every image of the mix (per-image sigma)."""
+    import numpy as np
+    import torch
+
+    from shearcuda_amd import api
+    from shearcuda_amd.synth import noisy_ellipses, white_noise
+    rng = np.random.default_rng(2024)
+    plans, npx, seed = [], 0, 100
+    for L, n in mix:
+        shear = api.make_dictionary(L)
+        sc = api.nsst_scalars_cuda(shear, torch.from_numpy(white_noise(L).astype(np.float32)).cuda())
+        sig = rng.uniform(5.0, 40.0, size=n)
+        xs = np.stack([noisy_ellipses(L, sigma=float(sig[i]), seed=seed + i)[1] for i in range(n)]).astype(np.float32)
+        seed += n
+        b = api.Batch(shear, n)
+        plans.append(dict(L=L, n=n, shear=shear, sc=sc, sig=sig, b=b, x=torch.from_numpy(xs).cuda(),
+                          c=b.alloc_coeffs(), th=api.batch_thresholds(shear, sig, sc),
+                          out=torch.empty((n, L, L), device="cuda")))
+        npx += n * L * L
+    g = api.capture(lambda: [p["b"].denoise(p["x"], p["c"], p["th"], p["out"]) for p in plans])
+    el = _timed(g.launch, steps, warmup)
+    batched = npx * steps / el / 1e6
+    # the same images one at a time through the plain form (one graph each)
+    singles = []
+    for p in plans:
+        for i in range(p["n"]):
+            gi, _, _ = api.denoise_graph(p["x"][i], p["shear"], float(p["sig"][i]), p["sc"])
+            singles.append(gi)
+    el1 = _timed(lambda: [gi.launch() for gi in singles], max(2, steps // 2), 1)
+    plain = npx * max(2, steps // 2) / el1 / 1e6
+    for gi in singles:
+        gi.close()
+    g.close()
+    for p in plans:
+        p["b"].close()
+        p["shear"].close()
+    return {"images": sum(n for _, n in mix), "pixels": npx, "mix": [[L, n] for L, n in mix],
+            "ms_per_launch": round(el * 1e3 / steps, 4), "Mpixel_s": round(batched, 2),
+            "plain_image_by_image_Mpixel_s": round(plain, 2), "speedup_vs_plain": round(batched / plain, 3),
+            "path_roofline_frac": round(400.0 * npx / (el / steps) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle work (0 = skip)")
+    ap.add_argument("--no-batched", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the NSST path has no multi-GPU form here); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.set_device(0)
+
+    L, sigma = args.size, 20.0
+    ms, stages = bench_plain(L, sigma, args.steps, args.warmup)
+    value = L * L / (ms * 1e-3) / 1e6
     dom = max(stages, key=lambda s: s["ms"])
     roof = {"kernel": dom["name"], "bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
@@ -123,15 +193,21 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plain 2D NSST hard-threshold denoise (dec+thresh+rec), one %dx%d FP32 image per step, "
                                "maxflat 4 levels, directions [8,8,16,16], Meyer sizes [32,32,16,16], T=[0,3,4], "
-                               "sigma=20" % (L, L), "L": L, "images_per_step": 1, "form": "plain"},
+                               "sigma=20, one hipGraph launch per step" % (L, L),
+                   "L": L, "images_per_step": 1, "form": "plain"},
         "roofline": roof,
         "path_roofline": {"bytes_per_step": path_bytes, "achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
         "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()} for s in stages],
     }
+    if L == 512:
+        ms1k, _ = bench_plain(1024, sigma, max(20, args.steps // 4), 5)
+        res["plain_1024"] = {"ms_per_image": round(ms1k, 5), "Mpixel_s": round(1024 * 1024 / (ms1k * 1e-3) / 1e6, 2),
+                             "path_roofline_frac": round(400.0 * 1024 * 1024 / (ms1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if not args.no_batched:
+        res["batched"] = bench_batched(max(5, args.steps // 20), 2)
     if args.cpu_budget > 0:
         res["cpu_baseline"] = cpu_baseline(L, sigma, args.cpu_budget)
-    shear.close()
     print(json.dumps(res), flush=True)
 
 

@@ -37,6 +37,8 @@ extern "C" {
                               prepare_shearing_filters_cuda.cpp:254            */
 
 typedef struct shx_dict shx_dict;
+typedef struct shx_batch shx_batch;
+typedef struct shx_graph shx_graph;
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* shx_last_error(void);
@@ -96,6 +98,39 @@ int shx_denoise(shx_dict* dict, const float* x, float* const* coeff, double sigm
 int shx_denoise_profile(shx_dict* dict, const float* x, float* const* coeff, double sigma, const double* Tscalars,
                         const double* scalars, float* out, int reps, int max_stages, double* stage_ms,
                         double* stage_bytes, char* names, int name_len, int* nstages, void* stream);
+
+/* ---- batched form ----------------------------------------------------------
+ * A batch plan runs nimg images of the dictionary's size L through every
+ * kernel of the path at once (the image index is one more grid dimension), so
+ * small images fill the GPU.  Buffers are contiguous device arrays:
+ *   x, out : nimg x L x L
+ *   coeff  : nimg x (1 + sum ndir) x L x L   (per image: lowpass, then the
+ *            bands scale-major, direction-minor)
+ *   th_dev : nimg x (1 + sum ndir) FP32 thresholds in device memory
+ *            (per image: lowpass first), keep iff |c| > th.
+ * No counterpart in the reference (its MEX entry points take one image). */
+int shx_batch_create(shx_dict* dict, int nimg, shx_batch** out);
+int shx_batch_destroy(shx_batch* batch);
+size_t shx_batch_bytes(const shx_batch* batch);
+int shx_batch_shear_trans(shx_batch* batch, const float* x, float* coeff, void* stream);
+/* th_dev may be NULL (plain inverse). */
+int shx_batch_inverse_shear(shx_batch* batch, const float* coeff, const float* th_dev, float* out, void* stream);
+int shx_batch_denoise(shx_batch* batch, const float* x, float* coeff, const float* th_dev, float* out,
+                      void* stream);
+
+/* ---- HIP graphs --------------------------------------------------------------
+ * Any sequence of shx calls that neither allocates nor synchronises
+ * (shx_shear_trans, shx_inverse_shear*, shx_denoise, shx_batch_*) can be
+ * recorded between shx_capture_begin/end on a non-default stream and replayed
+ * with one shx_graph_launch; e.g. several batch plans of different sizes in
+ * one launch.  Replays reuse the recorded pointers and thresholds. */
+int shx_capture_begin(void* stream);
+int shx_capture_end(void* stream, shx_graph** out);
+int shx_graph_launch(shx_graph* graph, void* stream);
+int shx_graph_destroy(shx_graph* graph);
+/* Records shx_denoise on an internal stream into a graph. */
+int shx_denoise_graph(shx_dict* dict, const float* x, float* const* coeff, double sigma, const double* Tscalars,
+                      const double* scalars, float* out, shx_graph** out_graph);
 
 /* In-place hard threshold of coefficient cells, thresh_cuda(Ct, lambda,
  * option, E, sc, opt) (shearcuda/src_cuda/thresh_cuda.cpp:57-217,

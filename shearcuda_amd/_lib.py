@@ -39,6 +39,18 @@ _SIGNATURES = [
     ("shx_denoise_profile", c_int, [c_void_p, c_void_p, P(c_void_p), c_double, P(c_double), P(c_double), c_void_p,
                                     c_int, c_int, P(c_double), P(c_double), ctypes.c_char_p, c_int, P(c_int),
                                     c_void_p]),
+    ("shx_batch_create", c_int, [c_void_p, c_int, P(c_void_p)]),
+    ("shx_batch_destroy", c_int, [c_void_p]),
+    ("shx_batch_bytes", c_size_t, [c_void_p]),
+    ("shx_batch_shear_trans", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx_batch_inverse_shear", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx_batch_denoise", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx_capture_begin", c_int, [c_void_p]),
+    ("shx_capture_end", c_int, [c_void_p, P(c_void_p)]),
+    ("shx_graph_launch", c_int, [c_void_p, c_void_p]),
+    ("shx_graph_destroy", c_int, [c_void_p]),
+    ("shx_denoise_graph", c_int, [c_void_p, c_void_p, P(c_void_p), c_double, P(c_double), P(c_double), c_void_p,
+                                  P(c_void_p)]),
     ("shx_thresh", c_int, [P(c_void_p), c_int, P(c_int), P(c_int64), c_double, c_int, P(c_double), c_int, c_int,
                            P(c_double), c_int, c_int, P(c_double), c_void_p]),
     ("shx_atrousdec", c_int, [c_void_p, c_int, c_int, P(c_void_p), c_void_p, c_void_p]),

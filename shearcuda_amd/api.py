@@ -328,3 +328,125 @@ def nsst_scalars_cuda(shear, noise):
 def make_dictionary(L, dcomp=DEFAULT_DCOMP, dsize=DEFAULT_DSIZE, shift="none"):
     """Shortcut: spatial Meyer filters + prepare_shearing_filters_cuda."""
     return prepare_shearing_filters_cuda(dictionary_filters(dcomp, dsize), L, shift)
+
+
+# ---- HIP graphs ----------------------------------------------------------------
+
+class Graph:
+    """A recorded sequence of shx launches (hipGraph); launch() replays it on
+    the current stream with the recorded pointers and thresholds."""
+
+    def __init__(self, handle, keep=()):
+        self._h = handle
+        self._keep = keep  # tensors the graph reads or writes
+
+    def launch(self):
+        check(lib.shx_graph_launch(self._h, _stream()))
+
+    def close(self):
+        if self._h is not None:
+            lib.shx_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def capture(fn, keep=()):
+    """Record the shx calls made by fn() into a Graph (on a side stream)."""
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        check(lib.shx_capture_begin(ctypes.c_void_p(s.cuda_stream)))
+        err = None
+        try:
+            fn()
+        except Exception as e:  # finish the capture before re-raising
+            err = e
+        h = ctypes.c_void_p()
+        st = lib.shx_capture_end(ctypes.c_void_p(s.cuda_stream), ctypes.byref(h))
+    torch.cuda.current_stream().wait_stream(s)
+    if err is not None:
+        if st == 0:
+            lib.shx_graph_destroy(h)
+        raise err
+    check(st)
+    return Graph(h, keep)
+
+
+def denoise_graph(x, shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS, Ct=None, out=None):
+    """Graph of one plain denoise step (replay with .launch())."""
+    if Ct is None:
+        Ct = shear.alloc_coeffs()
+    if out is None:
+        out = torch.empty_like(x)
+    g = capture(lambda: denoise_cuda(x, shear, sigma, scalars, Tscalars, Ct=Ct, out=out), keep=(x, Ct, out))
+    return g, Ct, out
+
+
+# ---- batched form -----------------------------------------------------------------
+
+class Batch:
+    """nimg images of the dictionary's size through every kernel at once.
+    Buffers: x/out (nimg, L, L); coefficients (nimg, 1 + sum D, L, L) with the
+    lowpass first; thresholds (nimg, 1 + sum D) FP32 on the device."""
+
+    def __init__(self, shear, nimg):
+        self.shear = shear
+        self.nimg = int(nimg)
+        h = ctypes.c_void_p()
+        check(lib.shx_batch_create(shear.handle, self.nimg, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h is not None:
+            lib.shx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def alloc_coeffs(self):
+        L = self.shear.L
+        return torch.empty((self.nimg, self.shear.nbands, L, L), device=self.shear.device, dtype=torch.float32)
+
+    def _chk(self, *ts):
+        for t in ts:
+            _require_gpu(t)
+        if ts[0].shape[0] != self.nimg:
+            raise ValueError("batch holds %d images, got %d" % (self.nimg, ts[0].shape[0]))
+
+    def shear_trans(self, x, coeff):
+        self._chk(x, coeff)
+        check(lib.shx_batch_shear_trans(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(coeff.data_ptr()),
+                                        _stream()))
+        return coeff
+
+    def inverse_shear(self, coeff, out, th=None):
+        self._chk(coeff, out)
+        thp = ctypes.c_void_p(th.data_ptr()) if th is not None else None
+        check(lib.shx_batch_inverse_shear(self._h, ctypes.c_void_p(coeff.data_ptr()), thp,
+                                          ctypes.c_void_p(out.data_ptr()), _stream()))
+        return out
+
+    def denoise(self, x, coeff, th, out):
+        self._chk(x, coeff, out)
+        _require_gpu(th, "th")
+        if tuple(th.shape) != (self.nimg, self.shear.nbands):
+            raise ValueError("thresholds must be (%d, %d)" % (self.nimg, self.shear.nbands))
+        check(lib.shx_batch_denoise(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(coeff.data_ptr()),
+                                    ctypes.c_void_p(th.data_ptr()), ctypes.c_void_p(out.data_ptr()), _stream()))
+        return out
+
+
+def batch_thresholds(shear, sigmas, scalars, Tscalars=DEFAULT_TSCALARS):
+    """(nimg, 1 + sum D) FP32 device table of nsst_HT thresholds, one row per
+    image noise level."""
+    rows = [band_thresholds(shear, float(s), scalars, Tscalars) for s in np.atleast_1d(sigmas)]
+    return torch.from_numpy(np.array(rows, dtype=np.float32)).to(shear.device)

@@ -17,7 +17,6 @@
 #include "spectral.h"
 
 using shx::BandMap;
-using shx::BandThresh;
 using shx::PlaneSet;
 
 // ---------------------------------------------------------------------------
@@ -44,6 +43,16 @@ int fail(int code, const char* fmt, ...) {
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 }  // namespace
 
+// Per-image intermediate buffers of the NSST path.
+struct Work {
+  float* planes = nullptr;  // per image: S pyramid bands (ypyr) + 2 temporaries
+  float2* spec = nullptr;   // per image: S packed spectra (Y1 / rec input of the row pass)
+  float2* band = nullptr;   // per image: nbands packed spectra (T / U)
+  float2* q = nullptr;      // per image: ngroups packed partial reconstructions
+  long long s_planes = 0, s_spec = 0, s_band = 0, s_q = 0;  // floats per image
+  size_t bytes = 0;
+};
+
 struct shx_dict {
   int L = 0, S = 0, nbands = 0, shift = 0, device = -1;
   int ndir[SHX_MAX_SCALES] = {};
@@ -52,14 +61,21 @@ struct shx_dict {
   float2* Hmain = nullptr;
   float2* Hbeta = nullptr;
   float2* tw = nullptr;
-  float* planes = nullptr;  // S band planes (ypyr) + 2 pyramid temporaries
-  float2* spec = nullptr;   // S packed spectra
-  float2* band = nullptr;   // nbands packed spectra
+  Work w;  // one image
   size_t bytes = 0;
   size_t plane() const { return (size_t)L * L; }
-  float* ypyr(int j) const { return planes + (size_t)(j - 1) * plane(); }  // j = 1..S
-  float* tmpA() const { return planes + (size_t)S * plane(); }
-  float* tmpB() const { return planes + (size_t)(S + 1) * plane(); }
+};
+
+struct shx_batch {
+  shx_dict* d = nullptr;
+  int nimg = 0;
+  Work w;
+};
+
+struct shx_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t own = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -115,16 +131,17 @@ hipError_t pyr_dec(const float* x, int L, int nlev, float* const* y, float* tA, 
   const float* in = x;
   for (int lev = 0; lev < nlev; ++lev) {
     float* low = (lev == nlev - 1) ? y[0] : ((lev & 1) ? tB : tA);
-    hipError_t e = shx::launch_atrous_dec(in, low, y[nlev - lev], L, 1 << lev, shx::dec_filters(), s);
+    hipError_t e = shx::launch_atrous_dec(in, low, y[nlev - lev], shx::ImgStrides{0, 0, 0}, 1, L, 1 << lev,
+                                          shx::dec_filters(), s);
     if (e != hipSuccess) return e;
     in = low;
   }
   return hipSuccess;
 }
 
-// w: optional per-band weights (nlev+1), th_low: threshold of y[0] (< 0: none)
-hipError_t pyr_rec(const float* const* y, int L, int nlev, float* out, float* tA, float* tB, float th_low,
-                   const double* w, hipStream_t s) {
+// w: optional per-band weights (nlev+1)
+hipError_t pyr_rec(const float* const* y, int L, int nlev, float* out, float* tA, float* tB, const double* w,
+                   hipStream_t s) {
   const float* xl = y[0];
   for (int lev = nlev - 1; lev >= 0; --lev) {
     const int j = nlev - lev;
@@ -132,8 +149,8 @@ hipError_t pyr_rec(const float* const* y, int L, int nlev, float* out, float* tA
     const bool first = (lev == nlev - 1);
     const float sx = (first && w) ? (float)w[0] : 1.0f;
     const float sy = w ? (float)w[j] : 1.0f;
-    hipError_t e = shx::launch_atrous_rec(xl, y[j], o, L, 1 << lev, shx::rec_filters(), first ? th_low : -1.0f, sx,
-                                          sy, s);
+    hipError_t e = shx::launch_atrous_rec(xl, y[j], o, shx::ImgStrides{0, 0, 0}, 1, L, 1 << lev, shx::rec_filters(),
+                                          shx::LowThresh{0, 0.f, nullptr, 0}, sx, sy, s);
     if (e != hipSuccess) return e;
     xl = o;
   }
@@ -148,12 +165,57 @@ int check_dict(const shx_dict* d) {
   return SHX_OK;
 }
 
-PlaneSet band_cells(const shx_dict* d, float* const* coeff) {
-  PlaneSet p{};
-  p.ncells = d->S;
-  for (int j = 0; j <= d->S; ++j) p.off[j] = d->map.first[j];
-  for (int j = 0; j < d->S; ++j) p.cell[j] = coeff[j + 1];
-  return p;
+int alloc_work(const shx_dict* d, int nimg, Work& w) {
+  const long long pl = (long long)d->L * d->L;
+  w.s_planes = pl * (d->S + 2);
+  w.s_spec = pl * d->S;
+  w.s_band = pl * d->nbands;
+  w.s_q = pl * d->map.ngroups;
+  const size_t n = (size_t)nimg;
+  hipError_t e;
+  if ((e = hipMalloc(&w.planes, sizeof(float) * w.s_planes * n)) != hipSuccess ||
+      (e = hipMalloc(&w.spec, sizeof(float) * w.s_spec * n)) != hipSuccess ||
+      (e = hipMalloc(&w.band, sizeof(float) * w.s_band * n)) != hipSuccess ||
+      (e = hipMalloc(&w.q, sizeof(float) * w.s_q * n)) != hipSuccess)
+    return fail(SHX_ERR_ALLOC, "workspace allocation (%d images of %d^2) failed: %s", nimg, d->L, hipGetErrorString(e));
+  w.bytes = sizeof(float) * n * (w.s_planes + w.s_spec + w.s_band + w.s_q);
+  return SHX_OK;
+}
+
+void free_work(Work& w) {
+  hipFree(w.planes);
+  hipFree(w.spec);
+  hipFree(w.band);
+  hipFree(w.q);
+  w = Work{};
+}
+
+// Coefficients of nimg images: lowpass plane (+ per-image stride) and bands.
+struct Coeffs {
+  float* low;
+  long long low_stride;
+  PlaneSet bands;
+};
+
+Coeffs cell_coeffs(const shx_dict* d, float* const* coeff) {
+  Coeffs c{};
+  c.low = coeff[0];
+  c.low_stride = 0;
+  c.bands.ncells = d->S;
+  for (int j = 0; j <= d->S; ++j) c.bands.off[j] = d->map.first[j];
+  for (int j = 0; j < d->S; ++j) c.bands.cell[j] = coeff[j + 1];
+  c.bands.img_stride = 0;
+  return c;
+}
+
+// contiguous stacks: image i at base + i*(1+nbands)*L*L, lowpass first
+Coeffs stack_coeffs(const shx_dict* d, float* base) {
+  const long long pl = (long long)d->L * d->L, st = pl * (1 + d->nbands);
+  Coeffs c{};
+  c.low = base;
+  c.low_stride = st;
+  c.bands = shx::single_cell(base + pl, d->nbands, st);
+  return c;
 }
 
 // One kernel launch of a path, with the bytes it must move at minimum
@@ -164,58 +226,145 @@ struct Stage {
   std::function<hipError_t()> run;
 };
 
-void stages_dec(shx_dict* d, const float* x, float* const* coeff, hipStream_t s, std::vector<Stage>& st) {
+// Thresholds of a reconstruction: none, host values (one image, kernel
+// arguments) or a device table [nimg][1 + nbands] (lowpass first).
+struct Thr {
+  int mode = 0;
+  const float* host = nullptr;
+  const float* dev = nullptr;
+};
+
+void stages_pyr_dec(shx_dict* d, const Work& w, int nimg, const float* x, long long xs, const Coeffs& c,
+                    hipStream_t s, std::vector<Stage>& st) {
   const int L = d->L, S = d->S;
-  const double P = (double)L * L * 4.0;  // bytes per real plane
+  const long long pl = (long long)L * L;
+  const double P = 4.0 * pl * nimg;
   const float* in = x;
+  long long ins = xs;
   for (int lev = 0; lev < S; ++lev) {
-    float* low = (lev == S - 1) ? coeff[0] : ((lev & 1) ? d->tmpB() : d->tmpA());
-    float* high = d->ypyr(S - lev);
+    const bool last = lev == S - 1;
+    float* low = last ? c.low : w.planes + (size_t)(S + (lev & 1)) * pl;
+    const long long lows = last ? c.low_stride : w.s_planes;
+    float* high = w.planes + (size_t)(S - lev - 1) * pl;
     const int m = 1 << lev;
+    shx::ImgStrides is{ins, lows, w.s_planes};
     st.push_back({"atrous_dec_m" + std::to_string(m), 3 * P,
-                  [=] { return shx::launch_atrous_dec(in, low, high, L, m, shx::dec_filters(), s); }});
+                  [=] { return shx::launch_atrous_dec(in, low, high, is, nimg, L, m, shx::dec_filters(), s); }});
     in = low;
+    ins = lows;
   }
-  PlaneSet yset = shx::single_cell(d->planes, S);
-  PlaneSet specset = shx::single_cell(reinterpret_cast<float*>(d->spec), S);
-  PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(d->band), d->nbands);
-  PlaneSet out = band_cells(d, coeff);
-  const int nb = d->nbands;
-  st.push_back({"rows_r2c_scales", 2 * S * P, [=] { return shx::launch_rows_r2c(L, yset, specset, S, nullptr, d->tw, s); }});
-  st.push_back({"cols_dec", (S + 2.0 * nb) * P,
-                [=] { return shx::launch_cols_dec(L, d->spec, d->band, d->Hmain, d->Hbeta, d->map, d->tw, s); }});
-  st.push_back({"rows_c2r_bands", 2.0 * nb * P, [=] { return shx::launch_rows_c2r(L, bandset, out, nb, d->tw, s); }});
 }
 
-void stages_rec(shx_dict* d, const float* const* coeff, const float* th, float* x, hipStream_t s,
-                std::vector<Stage>& st) {
+void stages_spec_dec(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::vector<Stage>& st) {
   const int L = d->L, S = d->S, nb = d->nbands;
-  const double P = (double)L * L * 4.0;
-  PlaneSet in = band_cells(d, const_cast<float* const*>(coeff));
-  PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(d->band), nb);
-  PlaneSet specset = shx::single_cell(reinterpret_cast<float*>(d->spec), S);
-  PlaneSet yset = shx::single_cell(d->planes, S);
-  const bool use_th = th != nullptr;
-  BandThresh bt{};
-  if (use_th)
-    for (int b = 0; b < nb; ++b) bt.v[b] = th[1 + b];
-  st.push_back({"rows_r2c_bands", 2.0 * nb * P,
-                [=] { return shx::launch_rows_r2c(L, in, bandset, nb, use_th ? &bt : nullptr, d->tw, s); }});
-  st.push_back({"cols_rec", (2.0 * nb + S) * P,
-                [=] { return shx::launch_cols_rec(L, d->band, d->spec, d->Hmain, d->Hbeta, d->map, d->tw, s); }});
-  st.push_back({"rows_c2r_scales", 2 * S * P, [=] { return shx::launch_rows_c2r(L, specset, yset, S, d->tw, s); }});
-  const float th_low = use_th ? th[0] : -1.0f;
-  const float* xl = coeff[0];
+  const double P = 4.0 * L * L * nimg;
+  PlaneSet yset = shx::single_cell(w.planes, S, w.s_planes);
+  PlaneSet specset = shx::single_cell(reinterpret_cast<float*>(w.spec), S, w.s_spec);
+  shx::ThreshSpec none{};
+  st.push_back({"rows_r2c_scales", 2 * S * P,
+                [=] { return shx::launch_rows_r2c(L, yset, specset, S, nimg, none, d->tw, s); }});
+  st.push_back({"cols_dec", (S + 2.0 * nb) * P, [=] {
+                  return shx::launch_cols_dec(L, w.spec, w.s_spec / 2, w.band, w.s_band / 2, d->Hmain, d->Hbeta,
+                                              d->map, nimg, d->tw, s);
+                }});
+}
+
+shx::ThreshSpec band_thresh(const shx_dict* d, const Thr& t) {
+  shx::ThreshSpec th{};
+  th.mode = t.mode;
+  if (t.mode == 1)
+    for (int b = 0; b < d->nbands; ++b) th.arg[b] = t.host[1 + b];
+  if (t.mode == 2) {
+    th.dev = t.dev;
+    th.dev_stride = 1 + d->nbands;
+    th.dev_off = 1;
+  }
+  return th;
+}
+
+void stages_spec_rec_tail(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::vector<Stage>& st) {
+  const int L = d->L, S = d->S, nb = d->nbands, ng = d->map.ngroups;
+  const double P = 4.0 * L * L * nimg;
+  st.push_back({"cols_rec", (2.0 * nb + ng) * P, [=] {
+                  return shx::launch_cols_rec(L, w.band, w.s_band / 2, w.q, w.s_q / 2, d->Hmain, d->Hbeta, d->map,
+                                              nimg, d->tw, s);
+                }});
+  PlaneSet qset = shx::single_cell(reinterpret_cast<float*>(w.q), ng, w.s_q);
+  PlaneSet yset = shx::single_cell(w.planes, S, w.s_planes);
+  std::vector<int> sf(d->map.group_first, d->map.group_first + S + 1);
+  st.push_back({"rows_c2r_scales", (ng + S) * P,
+                [=] { return shx::launch_rows_c2r(L, qset, yset, S, nimg, sf.data(), d->tw, s); }});
+}
+
+void stages_pyr_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const Thr& t, float* x, long long xs,
+                    hipStream_t s, std::vector<Stage>& st) {
+  const int L = d->L, S = d->S;
+  const long long pl = (long long)L * L;
+  const double P = 4.0 * pl * nimg;
+  shx::LowThresh lt{0, 0.f, nullptr, 0};
+  if (t.mode == 1) lt = shx::LowThresh{1, t.host[0], nullptr, 0};
+  if (t.mode == 2) lt = shx::LowThresh{2, 0.f, t.dev, 1 + d->nbands};
+  const float* xl = c.low;
+  long long xls = c.low_stride;
   for (int lev = S - 1; lev >= 0; --lev) {
     const int j = S - lev, m = 1 << lev;
-    float* o = (lev == 0) ? x : ((lev & 1) ? d->tmpA() : d->tmpB());
-    const float* yh = d->ypyr(j);
-    const float t = (lev == S - 1) ? th_low : -1.0f;
+    const bool last = lev == 0;
+    float* o = last ? x : w.planes + (size_t)(S + (lev & 1)) * pl;
+    const long long os = last ? xs : w.s_planes;
+    const float* yh = w.planes + (size_t)(j - 1) * pl;
+    const shx::LowThresh tl = (lev == S - 1) ? lt : shx::LowThresh{0, 0.f, nullptr, 0};
+    shx::ImgStrides is{xls, w.s_planes, os};
     st.push_back({"atrous_rec_m" + std::to_string(m), 3 * P, [=] {
-                    return shx::launch_atrous_rec(xl, yh, o, L, m, shx::rec_filters(), t, 1.0f, 1.0f, s);
+                    return shx::launch_atrous_rec(xl, yh, o, is, nimg, L, m, shx::rec_filters(), tl, 1.0f, 1.0f, s);
                   }});
     xl = o;
+    xls = os;
   }
+}
+
+// Forward transform: pyramid, spectral pass, band rows written to c.bands.
+void stages_dec(shx_dict* d, const Work& w, int nimg, const float* x, long long xs, const Coeffs& c, hipStream_t s,
+                std::vector<Stage>& st) {
+  stages_pyr_dec(d, w, nimg, x, xs, c, s, st);
+  stages_spec_dec(d, w, nimg, s, st);
+  const int L = d->L, nb = d->nbands;
+  const double P = 4.0 * L * L * nimg;
+  PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
+  PlaneSet out = c.bands;
+  st.push_back({"rows_c2r_bands", 2.0 * nb * P,
+                [=] { return shx::launch_rows_c2r(L, bandset, out, nb, nimg, nullptr, d->tw, s); }});
+}
+
+// Inverse transform of (optionally thresholded) coefficients.
+void stages_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const Thr& t, float* x, long long xs,
+                hipStream_t s, std::vector<Stage>& st) {
+  const int L = d->L, nb = d->nbands;
+  const double P = 4.0 * L * L * nimg;
+  PlaneSet in = c.bands;
+  PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
+  shx::ThreshSpec th = band_thresh(d, t);
+  st.push_back({"rows_r2c_bands", 2.0 * nb * P,
+                [=] { return shx::launch_rows_r2c(L, in, bandset, nb, nimg, th, d->tw, s); }});
+  stages_spec_rec_tail(d, w, nimg, s, st);
+  stages_pyr_rec(d, w, nimg, c, t, x, xs, s, st);
+}
+
+// Denoise: the decomposition's band row pass and the reconstruction's
+// thresholded band row pass run as one kernel over each band row.
+void stages_denoise(shx_dict* d, const Work& w, int nimg, const float* x, long long xs, const Coeffs& c,
+                    const Thr& t, float* out, long long os, hipStream_t s, std::vector<Stage>& st) {
+  stages_pyr_dec(d, w, nimg, x, xs, c, s, st);
+  stages_spec_dec(d, w, nimg, s, st);
+  const int L = d->L, nb = d->nbands;
+  const double P = 4.0 * L * L * nimg;
+  PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
+  PlaneSet cb = c.bands;
+  shx::ThreshSpec th = band_thresh(d, t);
+  // reads T, writes the coefficients and U (in place of T, row by row)
+  st.push_back({"rows_c2r_r2c_bands", 3.0 * nb * P,
+                [=] { return shx::launch_rows_c2r_r2c(L, bandset, cb, bandset, nb, nimg, th, d->tw, s); }});
+  stages_spec_rec_tail(d, w, nimg, s, st);
+  stages_pyr_rec(d, w, nimg, c, t, out, os, s, st);
 }
 
 int run_stages(const std::vector<Stage>& st) {
@@ -224,12 +373,6 @@ int run_stages(const std::vector<Stage>& st) {
     if (e != hipSuccess) return fail(SHX_ERR_HIP, "%s launch failed: %s", g.name.c_str(), hipGetErrorString(e));
   }
   return SHX_OK;
-}
-
-int dict_inverse(shx_dict* d, const float* const* coeff, const float* th, float* x, hipStream_t s) {
-  std::vector<Stage> st;
-  stages_rec(d, coeff, th, x, s, st);
-  return run_stages(st);
 }
 
 std::vector<float> nsst_ht_thresholds(const shx_dict* d, double sigma, const double* T, const double* sc) {
@@ -245,13 +388,63 @@ std::vector<float> nsst_ht_thresholds(const shx_dict* d, double sigma, const dou
   return th;
 }
 
+int check_cells(const shx_dict* d, const void* const* coeff) {
+  if (!coeff) return fail(SHX_ERR_ARG, "null coefficient cells");
+  for (int j = 0; j <= d->S; ++j)
+    if (!coeff[j]) return fail(SHX_ERR_ARG, "coefficient cell %d is null", j);
+  return SHX_OK;
+}
+
+int profile_stages(const std::vector<Stage>& st, hipStream_t s, int reps, int max_stages, double* stage_ms,
+                   double* stage_bytes, char* names, int name_len, int* nstages) {
+  const int n = (int)st.size();
+  *nstages = n;
+  if (n > max_stages) return fail(SHX_ERR_ARG, "%d stages, room for %d", n, max_stages);
+  std::vector<hipEvent_t> ev(n + 1);
+  for (auto& e : ev) SHX_HIP(hipEventCreate(&e));
+  std::vector<double> acc(n, 0.0);
+  int rc = SHX_OK;
+  for (int r = 0; r < reps && rc == SHX_OK; ++r) {
+    if (hipEventRecord(ev[0], s) != hipSuccess) {
+      rc = fail(SHX_ERR_HIP, "event record failed");
+      break;
+    }
+    for (int i = 0; i < n && rc == SHX_OK; ++i) {
+      hipError_t e = st[i].run();
+      if (e == hipSuccess) e = hipEventRecord(ev[i + 1], s);
+      if (e != hipSuccess) rc = fail(SHX_ERR_HIP, "%s: %s", st[i].name.c_str(), hipGetErrorString(e));
+    }
+    if (rc != SHX_OK) break;
+    if (hipEventSynchronize(ev[n]) != hipSuccess) {
+      rc = fail(SHX_ERR_HIP, "event sync failed");
+      break;
+    }
+    for (int i = 0; i < n; ++i) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+      acc[i] += ms;
+    }
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  if (rc != SHX_OK) return rc;
+  for (int i = 0; i < n; ++i) {
+    stage_ms[i] = acc[i] / reps;
+    if (stage_bytes) stage_bytes[i] = st[i].bytes;
+    if (names && name_len > 0) {
+      strncpy(names + (size_t)i * name_len, st[i].name.c_str(), name_len - 1);
+      names[(size_t)i * name_len + name_len - 1] = 0;
+    }
+  }
+  return SHX_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 extern "C" {
 
 const char* shx_last_error(void) { return g_err.c_str(); }
-int shx_version(void) { return 100; }
+int shx_version(void) { return 200; }
 
 double shx_maxflat_factor_error(void) { return shx::maxflat_factor_error(); }
 
@@ -299,6 +492,7 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
   d->map.first[nscales] = nb;
   d->map.nscales = nscales;
   d->map.nbands = nb;
+  shx::band_map_groups(d->map, 4);
   d->nbands = nb;
   if (hipGetDevice(&d->device) != hipSuccess) {
     delete d;
@@ -318,17 +512,18 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
                            hb.data() + (size_t)d->map.first[j] * L * 2);
   }
   shx::twiddles_fp32(L, tw.data());
-  const size_t pl = (size_t)L * L * sizeof(float);
   const size_t bytes_hm = hm.size() * sizeof(float), bytes_hb = hb.size() * sizeof(float);
   hipError_t e = hipSuccess;
   if ((e = hipMalloc(&d->Hmain, bytes_hm)) != hipSuccess || (e = hipMalloc(&d->Hbeta, bytes_hb)) != hipSuccess ||
-      (e = hipMalloc(&d->tw, tw.size() * sizeof(float))) != hipSuccess ||
-      (e = hipMalloc(&d->planes, pl * (nscales + 2))) != hipSuccess ||
-      (e = hipMalloc(&d->spec, pl * nscales)) != hipSuccess || (e = hipMalloc(&d->band, pl * nb)) != hipSuccess) {
+      (e = hipMalloc(&d->tw, tw.size() * sizeof(float))) != hipSuccess) {
     shx_clear_shearing_filters(d);
     return fail(SHX_ERR_ALLOC, "device allocation failed: %s", hipGetErrorString(e));
   }
-  d->bytes = bytes_hm + bytes_hb + tw.size() * sizeof(float) + pl * (nscales + 2) + pl * nscales + pl * nb;
+  if (int r = alloc_work(d, 1, d->w)) {
+    shx_clear_shearing_filters(d);
+    return r;
+  }
+  d->bytes = bytes_hm + bytes_hb + tw.size() * sizeof(float) + d->w.bytes;
   if ((e = hipMemcpy(d->Hmain, hm.data(), bytes_hm, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(d->Hbeta, hb.data(), bytes_hb, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(d->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
@@ -344,9 +539,7 @@ int shx_clear_shearing_filters(shx_dict* d) {
   hipFree(d->Hmain);
   hipFree(d->Hbeta);
   hipFree(d->tw);
-  hipFree(d->planes);
-  hipFree(d->spec);
-  hipFree(d->band);
+  free_work(d->w);
   delete d;
   return SHX_OK;
 }
@@ -364,85 +557,197 @@ size_t shx_dict_bytes(const shx_dict* d) { return d ? d->bytes : 0; }
 
 int shx_shear_trans(shx_dict* d, const float* x, float* const* coeff, void* stream) {
   if (int r = check_dict(d)) return r;
-  if (!x || !coeff) return fail(SHX_ERR_ARG, "null input or output");
-  for (int j = 0; j <= d->S; ++j)
-    if (!coeff[j]) return fail(SHX_ERR_ARG, "coefficient cell %d is null", j);
+  if (!x) return fail(SHX_ERR_ARG, "null input");
+  if (int r = check_cells(d, (const void* const*)coeff)) return r;
   std::vector<Stage> st;
-  stages_dec(d, x, coeff, (hipStream_t)stream, st);
+  stages_dec(d, d->w, 1, x, 0, cell_coeffs(d, coeff), (hipStream_t)stream, st);
   return run_stages(st);
 }
 
 int shx_inverse_shear(shx_dict* d, const float* const* coeff, float* x, void* stream) {
   if (int r = check_dict(d)) return r;
-  if (!x || !coeff) return fail(SHX_ERR_ARG, "null input or output");
-  for (int j = 0; j <= d->S; ++j)
-    if (!coeff[j]) return fail(SHX_ERR_ARG, "coefficient cell %d is null", j);
-  return dict_inverse(d, coeff, nullptr, x, (hipStream_t)stream);
+  if (!x) return fail(SHX_ERR_ARG, "null output");
+  if (int r = check_cells(d, (const void* const*)coeff)) return r;
+  std::vector<Stage> st;
+  stages_rec(d, d->w, 1, cell_coeffs(d, const_cast<float* const*>(coeff)), Thr{}, x, 0, (hipStream_t)stream, st);
+  return run_stages(st);
 }
 
 int shx_inverse_shear_thresh(shx_dict* d, const float* const* coeff, const float* th, float* x, void* stream) {
   if (int r = check_dict(d)) return r;
-  if (!x || !coeff || !th) return fail(SHX_ERR_ARG, "null input, thresholds or output");
-  for (int j = 0; j <= d->S; ++j)
-    if (!coeff[j]) return fail(SHX_ERR_ARG, "coefficient cell %d is null", j);
-  return dict_inverse(d, coeff, th, x, (hipStream_t)stream);
+  if (!x || !th) return fail(SHX_ERR_ARG, "null thresholds or output");
+  if (int r = check_cells(d, (const void* const*)coeff)) return r;
+  std::vector<Stage> st;
+  Thr t;
+  t.mode = 1;
+  t.host = th;
+  stages_rec(d, d->w, 1, cell_coeffs(d, const_cast<float* const*>(coeff)), t, x, 0, (hipStream_t)stream, st);
+  return run_stages(st);
 }
 
 int shx_denoise(shx_dict* d, const float* x, float* const* coeff, double sigma, const double* T, const double* sc,
                 float* out, void* stream) {
   if (int r = check_dict(d)) return r;
-  if (!T || !sc) return fail(SHX_ERR_ARG, "null threshold scalars");
+  if (!x || !out || !T || !sc) return fail(SHX_ERR_ARG, "null input, output or threshold scalars");
+  if (int r = check_cells(d, (const void* const*)coeff)) return r;
   std::vector<float> th = nsst_ht_thresholds(d, sigma, T, sc);
-  if (int r = shx_shear_trans(d, x, coeff, stream)) return r;
-  return shx_inverse_shear_thresh(d, coeff, th.data(), out, stream);
+  Thr t;
+  t.mode = 1;
+  t.host = th.data();
+  std::vector<Stage> st;
+  stages_denoise(d, d->w, 1, x, 0, cell_coeffs(d, coeff), t, out, 0, (hipStream_t)stream, st);
+  return run_stages(st);
 }
 
 int shx_denoise_profile(shx_dict* d, const float* x, float* const* coeff, double sigma, const double* T,
                         const double* sc, float* out, int reps, int max_stages, double* stage_ms, double* stage_bytes,
                         char* names, int name_len, int* nstages, void* stream) {
   if (int r = check_dict(d)) return r;
-  if (!x || !coeff || !out || !T || !sc || !stage_ms || !nstages || reps < 1)
+  if (!x || !out || !T || !sc || !stage_ms || !nstages || reps < 1)
     return fail(SHX_ERR_ARG, "invalid profiling arguments");
-  hipStream_t s = (hipStream_t)stream;
+  if (int r = check_cells(d, (const void* const*)coeff)) return r;
   std::vector<float> th = nsst_ht_thresholds(d, sigma, T, sc);
+  Thr t;
+  t.mode = 1;
+  t.host = th.data();
   std::vector<Stage> st;
-  stages_dec(d, x, coeff, s, st);
-  stages_rec(d, coeff, th.data(), out, s, st);
-  const int n = (int)st.size();
-  *nstages = n;
-  if (n > max_stages) return fail(SHX_ERR_ARG, "%d stages, room for %d", n, max_stages);
-  std::vector<hipEvent_t> ev(n + 1);
-  for (auto& e : ev) SHX_HIP(hipEventCreate(&e));
-  std::vector<double> acc(n, 0.0);
-  int rc = SHX_OK;
-  for (int r = 0; r < reps && rc == SHX_OK; ++r) {
-    if (hipEventRecord(ev[0], s) != hipSuccess) { rc = fail(SHX_ERR_HIP, "event record failed"); break; }
-    for (int i = 0; i < n && rc == SHX_OK; ++i) {
-      hipError_t e = st[i].run();
-      if (e == hipSuccess) e = hipEventRecord(ev[i + 1], s);
-      if (e != hipSuccess) rc = fail(SHX_ERR_HIP, "%s: %s", st[i].name.c_str(), hipGetErrorString(e));
-    }
-    if (rc != SHX_OK) break;
-    if (hipEventSynchronize(ev[n]) != hipSuccess) { rc = fail(SHX_ERR_HIP, "event sync failed"); break; }
-    for (int i = 0; i < n; ++i) {
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
-      acc[i] += ms;
-    }
+  stages_denoise(d, d->w, 1, x, 0, cell_coeffs(d, coeff), t, out, 0, (hipStream_t)stream, st);
+  return profile_stages(st, (hipStream_t)stream, reps, max_stages, stage_ms, stage_bytes, names, name_len, nstages);
+}
+
+// ---- batched form ------------------------------------------------------------
+int shx_batch_create(shx_dict* d, int nimg, shx_batch** out) {
+  if (!out) return fail(SHX_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (int r = check_dict(d)) return r;
+  if (nimg < 1 || nimg > 65535) return fail(SHX_ERR_ARG, "batch of %d images", nimg);
+  auto b = new shx_batch();
+  b->d = d;
+  b->nimg = nimg;
+  if (int r = alloc_work(d, nimg, b->w)) {
+    delete b;
+    return r;
   }
-  for (auto& e : ev) hipEventDestroy(e);
-  if (rc != SHX_OK) return rc;
-  for (int i = 0; i < n; ++i) {
-    stage_ms[i] = acc[i] / reps;
-    if (stage_bytes) stage_bytes[i] = st[i].bytes;
-    if (names && name_len > 0) {
-      strncpy(names + (size_t)i * name_len, st[i].name.c_str(), name_len - 1);
-      names[(size_t)i * name_len + name_len - 1] = 0;
-    }
-  }
+  *out = b;
   return SHX_OK;
 }
 
+int shx_batch_destroy(shx_batch* b) {
+  if (!b) return SHX_OK;
+  free_work(b->w);
+  delete b;
+  return SHX_OK;
+}
+
+size_t shx_batch_bytes(const shx_batch* b) { return b ? b->w.bytes : 0; }
+
+static int batch_args(shx_batch* b, const void* x, const void* c, const void* o) {
+  if (!b) return fail(SHX_ERR_ARG, "null batch");
+  if (int r = check_dict(b->d)) return r;
+  if (!x || !c || !o) return fail(SHX_ERR_ARG, "null device buffer");
+  return SHX_OK;
+}
+
+int shx_batch_shear_trans(shx_batch* b, const float* x, float* coeff, void* stream) {
+  if (int r = batch_args(b, x, coeff, coeff)) return r;
+  shx_dict* d = b->d;
+  std::vector<Stage> st;
+  stages_dec(d, b->w, b->nimg, x, (long long)d->L * d->L, stack_coeffs(d, coeff), (hipStream_t)stream, st);
+  return run_stages(st);
+}
+
+int shx_batch_inverse_shear(shx_batch* b, const float* coeff, const float* th_dev, float* out, void* stream) {
+  if (int r = batch_args(b, coeff, coeff, out)) return r;
+  shx_dict* d = b->d;
+  Thr t;
+  if (th_dev) {
+    t.mode = 2;
+    t.dev = th_dev;
+  }
+  std::vector<Stage> st;
+  stages_rec(d, b->w, b->nimg, stack_coeffs(d, const_cast<float*>(coeff)), t, out, (long long)d->L * d->L,
+             (hipStream_t)stream, st);
+  return run_stages(st);
+}
+
+int shx_batch_denoise(shx_batch* b, const float* x, float* coeff, const float* th_dev, float* out, void* stream) {
+  if (int r = batch_args(b, x, coeff, out)) return r;
+  if (!th_dev) return fail(SHX_ERR_ARG, "null threshold table");
+  shx_dict* d = b->d;
+  Thr t;
+  t.mode = 2;
+  t.dev = th_dev;
+  std::vector<Stage> st;
+  const long long pl = (long long)d->L * d->L;
+  stages_denoise(d, b->w, b->nimg, x, pl, stack_coeffs(d, coeff), t, out, pl, (hipStream_t)stream, st);
+  return run_stages(st);
+}
+
+// ---- graphs --------------------------------------------------------------------
+int shx_capture_begin(void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!s) return fail(SHX_ERR_ARG, "capture needs a non-default stream");
+  SHX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  return SHX_OK;
+}
+
+int shx_capture_end(void* stream, shx_graph** out) {
+  if (!out) return fail(SHX_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  hipGraph_t g = nullptr;
+  SHX_HIP(hipStreamEndCapture(s, &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(g);
+    return fail(SHX_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+  }
+  auto gr = new shx_graph();
+  gr->graph = g;
+  gr->exec = ex;
+  *out = gr;
+  return SHX_OK;
+}
+
+int shx_graph_launch(shx_graph* g, void* stream) {
+  if (!g) return fail(SHX_ERR_ARG, "null graph");
+  SHX_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return SHX_OK;
+}
+
+int shx_graph_destroy(shx_graph* g) {
+  if (!g) return SHX_OK;
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  delete g;
+  return SHX_OK;
+}
+
+int shx_denoise_graph(shx_dict* d, const float* x, float* const* coeff, double sigma, const double* T,
+                      const double* sc, float* out, shx_graph** g) {
+  if (!g) return fail(SHX_ERR_ARG, "null output pointer");
+  *g = nullptr;
+  if (int r = check_dict(d)) return r;
+  hipStream_t s = nullptr;
+  SHX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  int rc = shx_capture_begin(s);
+  if (rc == SHX_OK) {
+    int rc2 = shx_denoise(d, x, coeff, sigma, T, sc, out, s);
+    shx_graph* gr = nullptr;
+    rc = shx_capture_end(s, &gr);
+    if (rc2 != SHX_OK) {
+      shx_graph_destroy(gr);
+      rc = rc2;
+    } else if (rc == SHX_OK) {
+      *g = gr;
+    }
+  }
+  hipStreamDestroy(s);
+  return rc;
+}
+
+// ---- thresholds, pyramid, helpers -------------------------------------------------
 int shx_thresh(float* const* cells, int ncells, const int* cell_planes, const int64_t* plane_elems, double lambda,
                int option, const double* E, int E_rows, int E_cols, const double* sc, int n_sc, int opt, double* maxi,
                void* stream) {
@@ -492,7 +797,7 @@ int shx_thresh(float* const* cells, int ncells, const int* cell_planes, const in
       t.invE[i] = invE[b + i];
       mx = std::max(mx, len[b + i]);
     }
-    const int blocks = (int)std::min<long long>(1024, (mx + 255) / 256);
+    const int blocks = (int)std::max<long long>(1, std::min<long long>(1024, (mx + 255) / 256));
     hipLaunchKernelGGL(k_hard_thresh, dim3(blocks, t.n), dim3(256), 0, s, t, opt == 1 ? 1 : 0, dmax);
     SHX_HIP(hipGetLastError());
   }
@@ -514,6 +819,8 @@ int shx_atrousdec(const float* x, int L, int nlevels, float* const* y, float* wo
   if (!is_pow2(L) || L < 32) return fail(SHX_ERR_UNSUPPORTED, "image size %d not supported", L);
   if (nlevels < 1 || nlevels > 9 || L % (1 << (nlevels - 1)) != 0)
     return fail(SHX_ERR_ARG, "invalid number of levels %d", nlevels);
+  for (int j = 0; j <= nlevels; ++j)
+    if (!y[j]) return fail(SHX_ERR_ARG, "output plane %d is null", j);
   float* w = work;
   if (!w) SHX_HIP(hipMalloc(&w, sizeof(float) * 2 * L * L));
   hipError_t e = pyr_dec(x, L, nlevels, y, w, w + (size_t)L * L, (hipStream_t)stream);
@@ -530,9 +837,11 @@ int shx_atrousrec(const float* const* y, int L, int nlevels, float* x, float* wo
   if (!is_pow2(L) || L < 32) return fail(SHX_ERR_UNSUPPORTED, "image size %d not supported", L);
   if (nlevels < 1 || nlevels > 9 || L % (1 << (nlevels - 1)) != 0)
     return fail(SHX_ERR_ARG, "invalid number of levels %d", nlevels);
+  for (int j = 0; j <= nlevels; ++j)
+    if (!y[j]) return fail(SHX_ERR_ARG, "input plane %d is null", j);
   float* w = work;
   if (!w) SHX_HIP(hipMalloc(&w, sizeof(float) * 2 * L * L));
-  hipError_t e = pyr_rec(y, L, nlevels, x, w, w + (size_t)L * L, -1.0f, nullptr, (hipStream_t)stream);
+  hipError_t e = pyr_rec(y, L, nlevels, x, w, w + (size_t)L * L, nullptr, (hipStream_t)stream);
   if (!work) {
     hipStreamSynchronize((hipStream_t)stream);
     hipFree(w);
@@ -555,7 +864,7 @@ int shx_re_weight_band(const float* x, int L, const double* coeff, int ncoeff, f
   for (int j = 0; j < ncoeff; ++j) y[j] = w + j * pl;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = pyr_dec(x, L, nlev, y, w + ncoeff * pl, w + (ncoeff + 1) * pl, s);
-  if (e == hipSuccess) e = pyr_rec(y, L, nlev, out, w + ncoeff * pl, w + (ncoeff + 1) * pl, -1.0f, coeff, s);
+  if (e == hipSuccess) e = pyr_rec(y, L, nlev, out, w + ncoeff * pl, w + (ncoeff + 1) * pl, coeff, s);
   if (!work) {
     hipStreamSynchronize(s);
     hipFree(w);
@@ -570,7 +879,7 @@ int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream)
   double* acc = nullptr;
   SHX_HIP(hipMalloc(&acc, sizeof(double)));
   SHX_HIP(hipMemsetAsync(acc, 0, sizeof(double), s));
-  const int blocks = (int)std::min<long long>(2048, (n + 255) / 256);
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(2048, (n + 255) / 256));
   if (n > 0) hipLaunchKernelGGL(k_norm_pow, dim3(blocks), dim3(256), 0, s, x, (long long)n, (float)p, acc);
   SHX_HIP(hipGetLastError());
   SHX_HIP(hipMemcpyAsync(out, acc, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -581,7 +890,7 @@ int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream)
 
 int shx_softth(float* x, int64_t n, double t, void* stream) {
   if (!x) return fail(SHX_ERR_ARG, "null input");
-  const int blocks = (int)std::min<long long>(4096, (n + 255) / 256);
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(4096, (n + 255) / 256));
   if (n > 0) hipLaunchKernelGGL(k_softth, dim3(blocks), dim3(256), 0, (hipStream_t)stream, x, (long long)n, (float)t);
   SHX_HIP(hipGetLastError());
   return SHX_OK;

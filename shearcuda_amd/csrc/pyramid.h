@@ -33,9 +33,24 @@ double maxflat_factor_error();
 // True when every filter has the rank the kernels are specialised for.
 bool maxflat_ok();
 
-hipError_t launch_atrous_dec(const float* x, float* ylow, float* yhigh, int L, int m, const DecFilters& f,
-                             hipStream_t s);
-hipError_t launch_atrous_rec(const float* xlow, const float* yhigh, float* out, int L, int m, const RecFilters& f,
-                             float th_x, float sx, float sy, hipStream_t s);
+// Per-image element strides of the three planes a level touches
+// (dec: x, ylow, yhigh; rec: xlow, yhigh, out).  0 for a single image.
+struct ImgStrides {
+  long long a, b, c;
+};
+
+// Hard threshold applied to the rec level's xlow input: none, one value, or
+// one per image in device memory (dev[img * stride]).
+struct LowThresh {
+  int mode;  // 0 none, 1 v, 2 dev
+  float v;
+  const float* dev;
+  int stride;
+};
+
+hipError_t launch_atrous_dec(const float* x, float* ylow, float* yhigh, const ImgStrides& st, int nimg, int L, int m,
+                             const DecFilters& f, hipStream_t s);
+hipError_t launch_atrous_rec(const float* xlow, const float* yhigh, float* out, const ImgStrides& st, int nimg, int L,
+                             int m, const RecFilters& f, const LowThresh& lt, float sx, float sy, hipStream_t s);
 
 }  // namespace shx

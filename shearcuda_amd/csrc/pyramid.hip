@@ -90,14 +90,18 @@ __device__ __forceinline__ void load_tile(float* __restrict__ tile, const float*
 // grid: (ceil(L/m/TW), ceil(L/m/TH), m*m)
 template <int TH, int TW, int VS, int HS, int T>
 __global__ void __launch_bounds__(T) k_atrous_dec(const float* __restrict__ x, float* __restrict__ ylow,
-                                                   float* __restrict__ yhigh, int L, int m, DecFilters f) {
+                                                   float* __restrict__ yhigh, ImgStrides st3, int L, int m,
+                                                   DecFilters f) {
   constexpr int W = TW + 2 * HALO;
   constexpr int WP = W | 1;  // odd row stride for the intermediates
   constexpr int R0N = 2, R1N = 4;
   __shared__ float tile[(TH + 2 * HALO) * W];
   __shared__ float tv[R0N + R1N][TH * WP];
-  const int ph = blockIdx.z;
+  const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
   const int pr = ph / m, pc = ph - pr * m;
+  x += (size_t)img * st3.a;
+  ylow += (size_t)img * st3.b;
+  yhigh += (size_t)img * st3.c;
   const int nsub = L / m;
   const int R0 = blockIdx.y * TH, C0 = blockIdx.x * TW;
   load_tile<TH, TW, T>(tile, x, L, m, pr, pc, R0, C0, -1.0f, 1.0f);
@@ -153,15 +157,19 @@ __global__ void __launch_bounds__(T) k_atrous_dec(const float* __restrict__ x, f
 // Reconstruction level: out = g0 * (sx * keep(xlow, th)) + g1 * (sy * yhigh).
 template <int TH, int TW, int VS, int HS, int T>
 __global__ void __launch_bounds__(T) k_atrous_rec(const float* __restrict__ xlow, const float* __restrict__ yhigh,
-                                                   float* __restrict__ out, int L, int m, RecFilters f, float th_x,
-                                                   float sx, float sy) {
+                                                   float* __restrict__ out, ImgStrides st3, int L, int m, RecFilters f,
+                                                   LowThresh lt, float sx, float sy) {
   constexpr int W = TW + 2 * HALO;
   constexpr int WP = W | 1;
   constexpr int RGN = 3, RHN = 3;
   __shared__ float tile[2][(TH + 2 * HALO) * W];
   __shared__ float tv[RGN + RHN][TH * WP];
-  const int ph = blockIdx.z;
+  const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
   const int pr = ph / m, pc = ph - pr * m;
+  xlow += (size_t)img * st3.a;
+  yhigh += (size_t)img * st3.b;
+  out += (size_t)img * st3.c;
+  const float th_x = lt.mode == 0 ? -1.0f : (lt.mode == 1 ? lt.v : lt.dev[(size_t)img * lt.stride]);
   const int nsub = L / m;
   const int R0 = blockIdx.y * TH, C0 = blockIdx.x * TW;
   load_tile<TH, TW, T>(tile[0], xlow, L, m, pr, pc, R0, C0, th_x, sx);
@@ -217,21 +225,21 @@ namespace {
 constexpr int PTH = 16, PTW = 32, PVS = 4, PHS = 2, PT = 256;
 }
 
-hipError_t launch_atrous_dec(const float* x, float* ylow, float* yhigh, int L, int m, const DecFilters& f,
-                             hipStream_t s) {
-  if (m < 1 || L % m != 0) return hipErrorInvalidValue;
+hipError_t launch_atrous_dec(const float* x, float* ylow, float* yhigh, const ImgStrides& st, int nimg, int L, int m,
+                             const DecFilters& f, hipStream_t s) {
+  if (m < 1 || L % m != 0 || nimg < 1) return hipErrorInvalidValue;
   const int nsub = L / m;
-  dim3 grid((nsub + PTW - 1) / PTW, (nsub + PTH - 1) / PTH, m * m);
-  hipLaunchKernelGGL((k_atrous_dec<PTH, PTW, PVS, PHS, PT>), grid, dim3(PT), 0, s, x, ylow, yhigh, L, m, f);
+  dim3 grid((nsub + PTW - 1) / PTW, (nsub + PTH - 1) / PTH, m * m * nimg);
+  hipLaunchKernelGGL((k_atrous_dec<PTH, PTW, PVS, PHS, PT>), grid, dim3(PT), 0, s, x, ylow, yhigh, st, L, m, f);
   return hipGetLastError();
 }
 
-hipError_t launch_atrous_rec(const float* xlow, const float* yhigh, float* out, int L, int m, const RecFilters& f,
-                             float th_x, float sx, float sy, hipStream_t s) {
-  if (m < 1 || L % m != 0) return hipErrorInvalidValue;
+hipError_t launch_atrous_rec(const float* xlow, const float* yhigh, float* out, const ImgStrides& st, int nimg, int L,
+                             int m, const RecFilters& f, const LowThresh& lt, float sx, float sy, hipStream_t s) {
+  if (m < 1 || L % m != 0 || nimg < 1) return hipErrorInvalidValue;
   const int nsub = L / m;
-  dim3 grid((nsub + PTW - 1) / PTW, (nsub + PTH - 1) / PTH, m * m);
-  hipLaunchKernelGGL((k_atrous_rec<PTH, PTW, PVS, PHS, PT>), grid, dim3(PT), 0, s, xlow, yhigh, out, L, m, f, th_x,
+  dim3 grid((nsub + PTW - 1) / PTW, (nsub + PTH - 1) / PTH, m * m * nimg);
+  hipLaunchKernelGGL((k_atrous_rec<PTH, PTW, PVS, PHS, PT>), grid, dim3(PT), 0, s, xlow, yhigh, out, st, L, m, f, lt,
                      sx, sy);
   return hipGetLastError();
 }

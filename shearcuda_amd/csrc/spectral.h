@@ -5,44 +5,72 @@
 
 namespace shx {
 
-// A set of planes spread over up to SHX_MAX_SCALES base pointers ("cells"):
-// plane p lives in cell j with off[j] <= p < off[j+1], at
-// cell[j] + (p - off[j]) * plane_elems.
+// A set of planes spread over up to SHX_MAX_SCALES+1 base pointers ("cells"):
+// plane p of image b lives in cell j with off[j] <= p < off[j+1], at
+//   cell[j] + (p - off[j]) * plane_elems + b * img_stride.
 struct PlaneSet {
   int ncells;
   int off[SHX_MAX_SCALES + 2];
   float* cell[SHX_MAX_SCALES + 1];
+  long long img_stride;  // floats between consecutive images (0: one image)
 };
 
-// Per-band hard threshold (keep iff |v| > v[p]).
-struct BandThresh {
-  float v[SHX_MAX_BANDS];
+// Hard thresholds for a row pass: none, per band in the kernel arguments
+// (one image), or per image and band in device memory.
+struct ThreshSpec {
+  int mode;              // 0 none, 1 arg[p], 2 dev[b * dev_stride + dev_off + p]
+  int dev_stride;
+  int dev_off;
+  const float* dev;
+  float arg[SHX_MAX_BANDS];
 };
 
-// Band p -> scale, and the first band of each scale.
+// Output plane p of a summing row pass reads input planes first[p]..first[p+1]-1.
+struct SumTable {
+  int first[SHX_MAX_SCALES + 1];
+};
+
+// Band p -> scale, first band of each scale, and the direction groups the
+// reconstruction column pass splits each scale into.
 struct BandMap {
   int nscales;
   int nbands;
   int first[SHX_MAX_SCALES + 1];
   unsigned char scale_of[SHX_MAX_BANDS];
+  int ngroups;
+  int group_first[SHX_MAX_SCALES + 1];        // first group of each scale
+  unsigned short group_band0[SHX_MAX_BANDS + 1];  // band range of group g: [band0[g], band0[g+1])
 };
 
-inline PlaneSet single_cell(float* base, int nplanes) {
+// Fill the group fields of a BandMap: each scale's directions are split into
+// groups of at most `gsize` consecutive directions.
+void band_map_groups(BandMap& m, int gsize);
+
+inline PlaneSet single_cell(float* base, int nplanes, long long img_stride = 0) {
   PlaneSet s{};
   s.ncells = 1;
   s.off[0] = 0;
   s.off[1] = nplanes;
   s.cell[0] = base;
+  s.img_stride = img_stride;
   return s;
 }
 
-hipError_t launch_rows_r2c(int L, const PlaneSet& in, const PlaneSet& out, int nplanes, const BandThresh* th,
-                           const float2* tw, hipStream_t s);
-hipError_t launch_rows_c2r(int L, const PlaneSet& in, const PlaneSet& out, int nplanes, const float2* tw,
+// All launches take nimg images; workspace strides are per image in floats.
+hipError_t launch_rows_r2c(int L, const PlaneSet& in, const PlaneSet& out, int nplanes, int nimg,
+                           const ThreshSpec& th, const float2* tw, hipStream_t s);
+// out plane p = C2R of the sum of input planes sum_first[p] .. sum_first[p+1]-1
+// (sum_first == nullptr: input plane p).
+hipError_t launch_rows_c2r(int L, const PlaneSet& in, const PlaneSet& out, int nplanes, int nimg,
+                           const int* sum_first, const float2* tw, hipStream_t s);
+// Denoise fusion: c = C2R(T) written to `band`, then U = R2C(keep(c, th)).
+hipError_t launch_rows_c2r_r2c(int L, const PlaneSet& T, const PlaneSet& band, const PlaneSet& U, int nplanes,
+                               int nimg, const ThreshSpec& th, const float2* tw, hipStream_t s);
+hipError_t launch_cols_dec(int L, const float2* Y1, long long y1_stride, float2* Tout, long long t_stride,
+                           const float2* Hm, const float2* Hb, const BandMap& map, int nimg, const float2* tw,
                            hipStream_t s);
-hipError_t launch_cols_dec(int L, const float2* Y1, float2* Tout, const float2* Hm, const float2* Hb,
-                           const BandMap& map, const float2* tw, hipStream_t s);
-hipError_t launch_cols_rec(int L, const float2* U, float2* V, const float2* Hm, const float2* Hb,
-                           const BandMap& map, const float2* tw, hipStream_t s);
+hipError_t launch_cols_rec(int L, const float2* U, long long u_stride, float2* Q, long long q_stride,
+                           const float2* Hm, const float2* Hb, const BandMap& map, int nimg, const float2* tw,
+                           hipStream_t s);
 
 }  // namespace shx

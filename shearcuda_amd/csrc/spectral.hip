@@ -30,116 +30,214 @@
 
 namespace shx {
 
-__device__ __forceinline__ int plane_cell(const PlaneSet& s, int p) {
+void band_map_groups(BandMap& m, int gsize) {
+  int g = 0;
+  for (int j = 0; j < m.nscales; ++j) {
+    m.group_first[j] = g;
+    for (int b = m.first[j]; b < m.first[j + 1]; b += gsize) m.group_band0[g++] = (unsigned short)b;
+  }
+  m.group_first[m.nscales] = g;
+  m.group_band0[g] = (unsigned short)m.nbands;
+  m.ngroups = g;
+}
+
+__device__ __forceinline__ float* plane_ptr(const PlaneSet& s, int p, size_t plane_elems, int b) {
   int j = 0;
 #pragma unroll 1
   while (j + 1 < s.ncells && p >= s.off[j + 1]) ++j;
-  return j;
+  return s.cell[j] + (size_t)(p - s.off[j]) * plane_elems + (size_t)b * s.img_stride;
 }
 
-__device__ __forceinline__ float* plane_ptr(const PlaneSet& s, int p, size_t plane_elems) {
-  const int j = plane_cell(s, p);
-  return s.cell[j] + (size_t)(p - s.off[j]) * plane_elems;
+__device__ __forceinline__ float thresh_of(const ThreshSpec& t, int p, int b) {
+  return t.mode == 1 ? t.arg[p] : t.dev[(size_t)b * t.dev_stride + t.dev_off + p];
+}
+
+// ---------------------------------------------------------------------------
+// Row-pass building blocks on a block of ROWS rows held in LDS as
+// buf[r * BS + k], N = L/2 complex per row.
+
+// Packed half spectrum of a real row from the FFT of its (even, odd) pairs.
+template <int N>
+__device__ __forceinline__ float2 r2c_post(const float2* row, int k, const float2* __restrict__ tw) {
+  const float2 z = row[k];
+  if (k == 0) return make_float2(z.x + z.y, z.x - z.y);
+  const float2 zc = cconj(row[N - k]);
+  const float2 a = cscale(cadd(z, zc), 0.5f);
+  const float2 b = cscale(csub(z, zc), 0.5f);
+  const float2 wb = cmul(tw[k], b);  // X = a - i w^k b
+  return make_float2(a.x + wb.y, a.y - wb.x);
+}
+
+// FFT input (even + i odd, times L) of the inverse of a packed half spectrum.
+template <int N>
+__device__ __forceinline__ float2 c2r_pre(const float2* row, int k, const float2* __restrict__ tw) {
+  const float2 x = row[k];
+  if (k == 0) return make_float2(x.x + x.y, x.x - x.y);
+  const float2 xc = cconj(row[N - k]);
+  const float2 fe = cadd(x, xc);
+  const float2 fo = cmulc(csub(x, xc), tw[k]);  // * w^{-k}
+  return make_float2(fe.x - fo.y, fe.y + fo.x);  // fe + i fo
+}
+
+// In-place map of every row element through f(row, k) (all reads, barrier,
+// all writes).
+template <int ROWS, int N, int BS, int T, class F>
+__device__ __forceinline__ void rows_map(float2* buf, F f) {
+  constexpr int PER = (ROWS * N + T - 1) / T;
+  float2 z[PER];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = threadIdx.x + e * T;
+    if ((ROWS * N) % T == 0 || idx < ROWS * N) z[e] = f(buf + (idx / N) * BS, idx % N);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = threadIdx.x + e * T;
+    if ((ROWS * N) % T == 0 || idx < ROWS * N) buf[(idx / N) * BS + idx % N] = z[e];
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
 // Row real-to-complex FFT (packed output), optional hard threshold on load.
-// grid: (L / ROWS, nplanes).  LDS: ROWS x (N + 1) float2.
+// grid: (L / ROWS, nplanes, nimg).
 template <int L, int ROWS, int T>
-__global__ void __launch_bounds__(T) k_rows_r2c(PlaneSet in, PlaneSet out, BandThresh th, int use_th,
+__global__ void __launch_bounds__(T) k_rows_r2c(PlaneSet in, PlaneSet out, ThreshSpec th,
                                                  const float2* __restrict__ tw) {
   constexpr int N = L / 2;
   constexpr int BS = N + 1;
   __shared__ float2 buf[ROWS * BS];
   const int tid = threadIdx.x;
-  const int p = blockIdx.y;
+  const int p = blockIdx.y, b = blockIdx.z;
   const int row0 = blockIdx.x * ROWS;
   const size_t pe = (size_t)L * L;
-  const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe)) + (size_t)row0 * N;
-  float2* dst = reinterpret_cast<float2*>(plane_ptr(out, p, pe)) + (size_t)row0 * N;
-  const float t = use_th ? th.v[p] : 0.0f;
+  const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe, b)) + (size_t)row0 * N;
+  float2* dst = reinterpret_cast<float2*>(plane_ptr(out, p, pe, b)) + (size_t)row0 * N;
+  const bool use_th = th.mode != 0;
+  const float t = use_th ? thresh_of(th, p, b) : 0.0f;
 #pragma unroll
   for (int idx = tid; idx < ROWS * N; idx += T) {
-    const int r = idx / N, i = idx % N;
     float2 v = src[idx];
     if (use_th) {
       v.x = hard_keep(v.x, t);
       v.y = hard_keep(v.y, t);
     }
-    buf[r * BS + i] = v;
+    buf[(idx / N) * BS + idx % N] = v;
   }
   __syncthreads();
   fft_lds<N, 2, -1, ROWS, T, false>(buf, BS, 1, tw, tid);
 #pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) {
-    const int r = idx / N, k = idx % N;
-    const float2 z = buf[r * BS + k];
-    float2 o;
-    if (k == 0) {
-      o = make_float2(z.x + z.y, z.x - z.y);
-    } else {
-      const float2 zc = cconj(buf[r * BS + (N - k)]);
-      const float2 a = cscale(cadd(z, zc), 0.5f);
-      const float2 b = cscale(csub(z, zc), 0.5f);
-      // X = a - i * w^k * b
-      const float2 wb = cmul(tw[k], b);
-      o = make_float2(a.x + wb.y, a.y - wb.x);
-    }
-    dst[idx] = o;
-  }
+  for (int idx = tid; idx < ROWS * N; idx += T) dst[idx] = r2c_post<N>(buf + (idx / N) * BS, idx % N, tw);
 }
 
 // ---------------------------------------------------------------------------
-// Row complex-to-real inverse FFT of a packed spectrum (unnormalised: x L).
-template <int L, int ROWS, int T>
-__global__ void __launch_bounds__(T) k_rows_c2r(PlaneSet in, PlaneSet out, const float2* __restrict__ tw) {
+// Row complex-to-real inverse FFT of a packed spectrum (unnormalised: x L),
+// optionally of the sum of several input planes.
+template <int L, int ROWS, int T, bool SUM>
+__global__ void __launch_bounds__(T) k_rows_c2r(PlaneSet in, PlaneSet out, SumTable sum,
+                                                 const float2* __restrict__ tw) {
   constexpr int N = L / 2;
   constexpr int BS = N + 1;
-  constexpr int PER = (ROWS * N + T - 1) / T;
   __shared__ float2 buf[ROWS * BS];
   const int tid = threadIdx.x;
-  const int p = blockIdx.y;
+  const int p = blockIdx.y, b = blockIdx.z;
   const int row0 = blockIdx.x * ROWS;
   const size_t pe = (size_t)L * L;
-  const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe)) + (size_t)row0 * N;
-  float2* dst = reinterpret_cast<float2*>(plane_ptr(out, p, pe)) + (size_t)row0 * N;
+  float2* dst = reinterpret_cast<float2*>(plane_ptr(out, p, pe, b)) + (size_t)row0 * N;
+  if constexpr (SUM) {
+    const int q0 = sum.first[p], q1 = sum.first[p + 1];
 #pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) buf[(idx / N) * BS + idx % N] = src[idx];
-  __syncthreads();
-  float2 z[PER];
-#pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = tid + e * T;
-    if ((ROWS * N) % T == 0 || idx < ROWS * N) {
-      const int r = idx / N, k = idx % N;
-      const float2 x = buf[r * BS + k];
-      if (k == 0) {
-        z[e] = make_float2(x.x + x.y, x.x - x.y);
-      } else {
-        const float2 xc = cconj(buf[r * BS + (N - k)]);
-        const float2 fe = cadd(x, xc);
-        const float2 fo = cmulc(csub(x, xc), tw[k]);  // * w^{-k}
-        z[e] = make_float2(fe.x - fo.y, fe.y + fo.x);  // fe + i*fo
-      }
+    for (int idx = tid; idx < ROWS * N; idx += T) {
+      float2 a = make_float2(0.f, 0.f);
+      for (int q = q0; q < q1; ++q)
+        a = cadd(a, reinterpret_cast<const float2*>(plane_ptr(in, q, pe, b))[(size_t)row0 * N + idx]);
+      buf[(idx / N) * BS + idx % N] = a;
     }
-  }
-  __syncthreads();
+  } else {
+    const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe, b)) + (size_t)row0 * N;
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = tid + e * T;
-    if ((ROWS * N) % T == 0 || idx < ROWS * N) buf[(idx / N) * BS + idx % N] = z[e];
+    for (int idx = tid; idx < ROWS * N; idx += T) buf[(idx / N) * BS + idx % N] = src[idx];
   }
   __syncthreads();
+  rows_map<ROWS, N, BS, T>(buf, [&](const float2* row, int k) { return c2r_pre<N>(row, k, tw); });
   fft_lds<N, 2, +1, ROWS, T, false>(buf, BS, 1, tw, tid);
 #pragma unroll
   for (int idx = tid; idx < ROWS * N; idx += T) dst[idx] = buf[(idx / N) * BS + idx % N];
 }
 
 // ---------------------------------------------------------------------------
-// Column pass of the decomposition: one block per (column tile, band).
-// grid: (N / CT, nbands).  LDS: L x CT float2.
+// Denoise fusion of the decomposition's last row pass with the
+// reconstruction's first: c = C2R(T) is stored (the coefficients the caller
+// receives), then U = R2C(keep(c, th)) is formed from the same rows in LDS.
+template <int L, int ROWS, int T>
+__global__ void __launch_bounds__(T) k_rows_c2r_r2c(PlaneSet Tin, PlaneSet band, PlaneSet Uout, ThreshSpec th,
+                                                     const float2* __restrict__ tw) {
+  constexpr int N = L / 2;
+  constexpr int BS = N + 1;
+  __shared__ float2 buf[ROWS * BS];
+  const int tid = threadIdx.x;
+  const int p = blockIdx.y, b = blockIdx.z;
+  const int row0 = blockIdx.x * ROWS;
+  const size_t pe = (size_t)L * L;
+  const float2* src = reinterpret_cast<const float2*>(plane_ptr(Tin, p, pe, b)) + (size_t)row0 * N;
+  float2* cdst = reinterpret_cast<float2*>(plane_ptr(band, p, pe, b)) + (size_t)row0 * N;
+  float2* udst = reinterpret_cast<float2*>(plane_ptr(Uout, p, pe, b)) + (size_t)row0 * N;
+  const float t = th.mode != 0 ? thresh_of(th, p, b) : -1.0f;
+#pragma unroll
+  for (int idx = tid; idx < ROWS * N; idx += T) buf[(idx / N) * BS + idx % N] = src[idx];
+  __syncthreads();
+  rows_map<ROWS, N, BS, T>(buf, [&](const float2* row, int k) { return c2r_pre<N>(row, k, tw); });
+  fft_lds<N, 2, +1, ROWS, T, false>(buf, BS, 1, tw, tid);
+#pragma unroll
+  for (int idx = tid; idx < ROWS * N; idx += T) {
+    float2& v = buf[(idx / N) * BS + idx % N];
+    const float2 c = v;
+    cdst[idx] = c;
+    if (th.mode != 0) v = make_float2(hard_keep(c.x, t), hard_keep(c.y, t));
+  }
+  __syncthreads();
+  fft_lds<N, 2, -1, ROWS, T, false>(buf, BS, 1, tw, tid);
+#pragma unroll
+  for (int idx = tid; idx < ROWS * N; idx += T) udst[idx] = r2c_post<N>(buf + (idx / N) * BS, idx % N, tw);
+}
+
+// ---------------------------------------------------------------------------
+// Column-tile helpers: tile of L rows x CT complex columns in LDS, buf[u*CT+c].
 template <int L, int CT, int T>
-__global__ void __launch_bounds__(T) k_cols_dec(const float2* __restrict__ Y1, float2* __restrict__ Tout,
+__device__ __forceinline__ void tile_load(float2* buf, const float2* __restrict__ src) {
+  constexpr int N = L / 2;
+#pragma unroll
+  for (int idx = threadIdx.x; idx < L * CT; idx += T) buf[idx] = src[(size_t)(idx / CT) * N + idx % CT];
+}
+
+template <int L, int CT, int T>
+__device__ __forceinline__ void tile_store(float2* __restrict__ dst, const float2* buf) {
+  constexpr int N = L / 2;
+#pragma unroll
+  for (int idx = threadIdx.x; idx < L * CT; idx += T) dst[(size_t)(idx / CT) * N + idx % CT] = buf[idx];
+}
+
+// Filter a transformed column tile by band p's spectrum (CONJ: by its
+// conjugate).  Slot 0 of tile c0 == 0 is the packed (DC, Nyquist) column.
+template <int L, int CT, bool CONJ>
+__device__ __forceinline__ float2 filt(const float2* buf, int idx, int c0, const float2* __restrict__ H,
+                                       const float2* __restrict__ Hb) {
+  constexpr int N = L / 2;
+  const int u = idx / CT, c = idx % CT;
+  const float2 P = buf[idx];
+  const float2 h = H[(size_t)u * N + c];
+  if (c0 + c == 0) {
+    const float2 Pm = cconj(buf[((L - u) & (L - 1)) * CT]);
+    return CONJ ? cadd(cmulc(P, h), cmulc(Pm, Hb[u])) : cadd(cmul(h, P), cmul(Hb[u], Pm));
+  }
+  return CONJ ? cmulc(P, h) : cmul(h, P);
+}
+
+// Column pass of the decomposition: one block per (column tile, band, image).
+template <int L, int CT, int T>
+__global__ void __launch_bounds__(T) k_cols_dec(const float2* __restrict__ Y1, long long y1_stride,
+                                                 float2* __restrict__ Tout, long long t_stride,
                                                  const float2* __restrict__ Hmain, const float2* __restrict__ Hbeta,
                                                  BandMap map, const float2* __restrict__ tw) {
   constexpr int N = L / 2;
@@ -147,29 +245,18 @@ __global__ void __launch_bounds__(T) k_cols_dec(const float2* __restrict__ Y1, f
   __shared__ float2 buf[L * CT];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * CT;
-  const int p = blockIdx.y;
+  const int p = blockIdx.y, b = blockIdx.z;
   const int j = map.scale_of[p];
-  const float2* src = Y1 + (size_t)j * L * N + c0;
-#pragma unroll
-  for (int idx = tid; idx < L * CT; idx += T) buf[idx] = src[(size_t)(idx / CT) * N + idx % CT];
+  tile_load<L, CT, T>(buf, Y1 + (size_t)b * y1_stride + (size_t)j * L * N + c0);
   __syncthreads();
   fft_lds<L, 1, -1, CT, T, true>(buf, 1, CT, tw, tid);
   const float2* H = Hmain + (size_t)p * L * N + c0;
+  const float2* Hb = Hbeta + (size_t)p * L;
   float2 q[PER];
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     const int idx = tid + e * T;
-    if ((L * CT) % T == 0 || idx < L * CT) {
-      const int u = idx / CT, c = idx % CT;
-      const float2 P = buf[idx];
-      const float2 h = H[(size_t)u * N + c];
-      if (c0 + c == 0) {
-        const float2 Pm = cconj(buf[((L - u) & (L - 1)) * CT]);
-        q[e] = cadd(cmul(h, P), cmul(Hbeta[(size_t)p * L + u], Pm));
-      } else {
-        q[e] = cmul(h, P);
-      }
-    }
+    if ((L * CT) % T == 0 || idx < L * CT) q[e] = filt<L, CT, false>(buf, idx, c0, H, Hb);
   }
   __syncthreads();
 #pragma unroll
@@ -179,16 +266,15 @@ __global__ void __launch_bounds__(T) k_cols_dec(const float2* __restrict__ Y1, f
   }
   __syncthreads();
   fft_lds<L, 1, +1, CT, T, true>(buf, 1, CT, tw, tid);
-  float2* dst = Tout + (size_t)p * L * N + c0;
-#pragma unroll
-  for (int idx = tid; idx < L * CT; idx += T) dst[(size_t)(idx / CT) * N + idx % CT] = buf[idx];
+  tile_store<L, CT, T>(Tout + (size_t)b * t_stride + (size_t)p * L * N + c0, buf);
 }
 
-// ---------------------------------------------------------------------------
-// Column pass of the reconstruction: one block per (column tile, scale),
-// accumulating conj(H_k) . FFT_cols(U_k) over the scale's directions.
+// Column pass of the reconstruction: one block per (column tile, direction
+// group, image) forms IFFT_cols(sum_{k in group} conj(H_k) . FFT_cols(U_k));
+// the row pass that follows sums the groups of a scale.
 template <int L, int CT, int T>
-__global__ void __launch_bounds__(T) k_cols_rec(const float2* __restrict__ U, float2* __restrict__ V,
+__global__ void __launch_bounds__(T) k_cols_rec(const float2* __restrict__ U, long long u_stride,
+                                                 float2* __restrict__ Q, long long q_stride,
                                                  const float2* __restrict__ Hmain, const float2* __restrict__ Hbeta,
                                                  BandMap map, const float2* __restrict__ tw) {
   constexpr int N = L / 2;
@@ -196,31 +282,21 @@ __global__ void __launch_bounds__(T) k_cols_rec(const float2* __restrict__ U, fl
   __shared__ float2 buf[L * CT];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * CT;
-  const int j = blockIdx.y;
+  const int g = blockIdx.y, b = blockIdx.z;
   float2 acc[PER];
 #pragma unroll
   for (int e = 0; e < PER; ++e) acc[e] = make_float2(0.f, 0.f);
-  for (int p = map.first[j]; p < map.first[j + 1]; ++p) {
-    const float2* src = U + (size_t)p * L * N + c0;
-#pragma unroll
-    for (int idx = tid; idx < L * CT; idx += T) buf[idx] = src[(size_t)(idx / CT) * N + idx % CT];
+  const int p0 = map.group_band0[g], p1 = map.group_band0[g + 1];
+  for (int p = p0; p < p1; ++p) {
+    tile_load<L, CT, T>(buf, U + (size_t)b * u_stride + (size_t)p * L * N + c0);
     __syncthreads();
     fft_lds<L, 1, -1, CT, T, true>(buf, 1, CT, tw, tid);
     const float2* H = Hmain + (size_t)p * L * N + c0;
+    const float2* Hb = Hbeta + (size_t)p * L;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
       const int idx = tid + e * T;
-      if ((L * CT) % T == 0 || idx < L * CT) {
-        const int u = idx / CT, c = idx % CT;
-        const float2 P = buf[idx];
-        const float2 h = H[(size_t)u * N + c];
-        if (c0 + c == 0) {
-          const float2 Pm = cconj(buf[((L - u) & (L - 1)) * CT]);
-          acc[e] = cadd(acc[e], cadd(cmulc(P, h), cmulc(Pm, Hbeta[(size_t)p * L + u])));
-        } else {
-          acc[e] = cadd(acc[e], cmulc(P, h));
-        }
-      }
+      if ((L * CT) % T == 0 || idx < L * CT) acc[e] = cadd(acc[e], filt<L, CT, true>(buf, idx, c0, H, Hb));
     }
     __syncthreads();
   }
@@ -231,9 +307,7 @@ __global__ void __launch_bounds__(T) k_cols_rec(const float2* __restrict__ U, fl
   }
   __syncthreads();
   fft_lds<L, 1, +1, CT, T, true>(buf, 1, CT, tw, tid);
-  float2* dst = V + (size_t)j * L * N + c0;
-#pragma unroll
-  for (int idx = tid; idx < L * CT; idx += T) dst[(size_t)(idx / CT) * N + idx % CT] = buf[idx];
+  tile_store<L, CT, T>(Q + (size_t)b * q_stride + (size_t)g * L * N + c0, buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -248,100 +322,82 @@ struct SpectralCfg {
 };
 
 template <int L>
-static hipError_t launch_rows_r2c_t(const PlaneSet& in, const PlaneSet& out, int nplanes, const BandThresh* th,
-                                    const float2* tw, hipStream_t s) {
+struct Launch {
   using C = SpectralCfg<L>;
-  static_assert(L % C::ROWS == 0, "rows per block must divide L");
-  BandThresh dummy;
-  hipLaunchKernelGGL((k_rows_r2c<L, C::ROWS, C::RT>), dim3(L / C::ROWS, nplanes), dim3(C::RT), 0, s, in, out,
-                     th ? *th : dummy, th ? 1 : 0, tw);
-  return hipGetLastError();
-}
+  static hipError_t rows_r2c(const PlaneSet& in, const PlaneSet& out, int np, int nimg, const ThreshSpec& th,
+                             const float2* tw, hipStream_t s) {
+    hipLaunchKernelGGL((k_rows_r2c<L, C::ROWS, C::RT>), dim3(L / C::ROWS, np, nimg), dim3(C::RT), 0, s, in, out, th,
+                       tw);
+    return hipGetLastError();
+  }
+  static hipError_t rows_c2r(const PlaneSet& in, const PlaneSet& out, int np, int nimg, const int* sum_first,
+                             const float2* tw, hipStream_t s) {
+    SumTable st{};
+    if (sum_first) {
+      for (int i = 0; i <= np; ++i) st.first[i] = sum_first[i];
+      hipLaunchKernelGGL((k_rows_c2r<L, C::ROWS, C::RT, true>), dim3(L / C::ROWS, np, nimg), dim3(C::RT), 0, s, in,
+                         out, st, tw);
+    } else {
+      hipLaunchKernelGGL((k_rows_c2r<L, C::ROWS, C::RT, false>), dim3(L / C::ROWS, np, nimg), dim3(C::RT), 0, s, in,
+                         out, st, tw);
+    }
+    return hipGetLastError();
+  }
+  static hipError_t rows_c2r_r2c(const PlaneSet& Tin, const PlaneSet& band, const PlaneSet& U, int np, int nimg,
+                                 const ThreshSpec& th, const float2* tw, hipStream_t s) {
+    hipLaunchKernelGGL((k_rows_c2r_r2c<L, C::ROWS, C::RT>), dim3(L / C::ROWS, np, nimg), dim3(C::RT), 0, s, Tin,
+                       band, U, th, tw);
+    return hipGetLastError();
+  }
+  static hipError_t cols_dec(const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
+                             const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
+    hipLaunchKernelGGL((k_cols_dec<L, C::CT, C::CTT>), dim3(C::N / C::CT, map.nbands, nimg), dim3(C::CTT), 0, s, Y1,
+                       ys, To, ts, Hm, Hb, map, tw);
+    return hipGetLastError();
+  }
+  static hipError_t cols_rec(const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
+                             const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
+    hipLaunchKernelGGL((k_cols_rec<L, C::CT, C::CTT>), dim3(C::N / C::CT, map.ngroups, nimg), dim3(C::CTT), 0, s, U,
+                       us, Q, qs, Hm, Hb, map, tw);
+    return hipGetLastError();
+  }
+};
 
-template <int L>
-static hipError_t launch_rows_c2r_t(const PlaneSet& in, const PlaneSet& out, int nplanes, const float2* tw,
-                                    hipStream_t s) {
-  using C = SpectralCfg<L>;
-  hipLaunchKernelGGL((k_rows_c2r<L, C::ROWS, C::RT>), dim3(L / C::ROWS, nplanes), dim3(C::RT), 0, s, in, out, tw);
-  return hipGetLastError();
-}
+#define SHX_BY_L(L, EXPR_T)                       \
+  switch (L) {                                    \
+    case 32: { using LT = Launch<32>; return EXPR_T; }     \
+    case 64: { using LT = Launch<64>; return EXPR_T; }     \
+    case 128: { using LT = Launch<128>; return EXPR_T; }   \
+    case 256: { using LT = Launch<256>; return EXPR_T; }   \
+    case 512: { using LT = Launch<512>; return EXPR_T; }   \
+    case 1024: { using LT = Launch<1024>; return EXPR_T; } \
+    default: return hipErrorInvalidValue;         \
+  }
 
-template <int L>
-static hipError_t launch_cols_dec_t(const float2* Y1, float2* Tout, const float2* Hm, const float2* Hb,
-                                    const BandMap& map, const float2* tw, hipStream_t s) {
-  using C = SpectralCfg<L>;
-  hipLaunchKernelGGL((k_cols_dec<L, C::CT, C::CTT>), dim3(C::N / C::CT, map.nbands), dim3(C::CTT), 0, s, Y1, Tout,
-                     Hm, Hb, map, tw);
-  return hipGetLastError();
-}
-
-template <int L>
-static hipError_t launch_cols_rec_t(const float2* U, float2* V, const float2* Hm, const float2* Hb,
-                                    const BandMap& map, const float2* tw, hipStream_t s) {
-  using C = SpectralCfg<L>;
-  hipLaunchKernelGGL((k_cols_rec<L, C::CT, C::CTT>), dim3(C::N / C::CT, map.nscales), dim3(C::CTT), 0, s, U, V, Hm,
-                     Hb, map, tw);
-  return hipGetLastError();
-}
-
-hipError_t launch_rows_r2c(int L, const PlaneSet& in, const PlaneSet& out, int nplanes, const BandThresh* th,
+hipError_t launch_rows_r2c(int L, const PlaneSet& in, const PlaneSet& out, int np, int nimg, const ThreshSpec& th,
                            const float2* tw, hipStream_t s) {
-#define CALL(LL) launch_rows_r2c_t<LL>(in, out, nplanes, th, tw, s)
-  switch (L) {
-    case 32: return CALL(32);
-    case 64: return CALL(64);
-    case 128: return CALL(128);
-    case 256: return CALL(256);
-    case 512: return CALL(512);
-    case 1024: return CALL(1024);
-    default: return hipErrorInvalidValue;
-  }
-#undef CALL
+  SHX_BY_L(L, LT::rows_r2c(in, out, np, nimg, th, tw, s))
 }
 
-hipError_t launch_rows_c2r(int L, const PlaneSet& in, const PlaneSet& out, int nplanes, const float2* tw,
-                           hipStream_t s) {
-#define CALL(LL) launch_rows_c2r_t<LL>(in, out, nplanes, tw, s)
-  switch (L) {
-    case 32: return CALL(32);
-    case 64: return CALL(64);
-    case 128: return CALL(128);
-    case 256: return CALL(256);
-    case 512: return CALL(512);
-    case 1024: return CALL(1024);
-    default: return hipErrorInvalidValue;
-  }
-#undef CALL
+hipError_t launch_rows_c2r(int L, const PlaneSet& in, const PlaneSet& out, int np, int nimg, const int* sum_first,
+                           const float2* tw, hipStream_t s) {
+  if (sum_first && np > SHX_MAX_SCALES) return hipErrorInvalidValue;
+  SHX_BY_L(L, LT::rows_c2r(in, out, np, nimg, sum_first, tw, s))
 }
 
-hipError_t launch_cols_dec(int L, const float2* Y1, float2* Tout, const float2* Hm, const float2* Hb,
-                           const BandMap& map, const float2* tw, hipStream_t s) {
-#define CALL(LL) launch_cols_dec_t<LL>(Y1, Tout, Hm, Hb, map, tw, s)
-  switch (L) {
-    case 32: return CALL(32);
-    case 64: return CALL(64);
-    case 128: return CALL(128);
-    case 256: return CALL(256);
-    case 512: return CALL(512);
-    case 1024: return CALL(1024);
-    default: return hipErrorInvalidValue;
-  }
-#undef CALL
+hipError_t launch_rows_c2r_r2c(int L, const PlaneSet& Tin, const PlaneSet& band, const PlaneSet& U, int np, int nimg,
+                               const ThreshSpec& th, const float2* tw, hipStream_t s) {
+  SHX_BY_L(L, LT::rows_c2r_r2c(Tin, band, U, np, nimg, th, tw, s))
 }
 
-hipError_t launch_cols_rec(int L, const float2* U, float2* V, const float2* Hm, const float2* Hb,
-                           const BandMap& map, const float2* tw, hipStream_t s) {
-#define CALL(LL) launch_cols_rec_t<LL>(U, V, Hm, Hb, map, tw, s)
-  switch (L) {
-    case 32: return CALL(32);
-    case 64: return CALL(64);
-    case 128: return CALL(128);
-    case 256: return CALL(256);
-    case 512: return CALL(512);
-    case 1024: return CALL(1024);
-    default: return hipErrorInvalidValue;
-  }
-#undef CALL
+hipError_t launch_cols_dec(int L, const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
+                           const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
+  SHX_BY_L(L, LT::cols_dec(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s))
+}
+
+hipError_t launch_cols_rec(int L, const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
+                           const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
+  SHX_BY_L(L, LT::cols_rec(U, us, Q, qs, Hm, Hb, map, nimg, tw, s))
 }
 
 }  // namespace shx

@@ -199,3 +199,79 @@ def test_norm_and_softth(api):
     assert api.norm_cuda(t, 1) == pytest.approx(np.abs(a.astype(np.float64)).sum(), rel=1e-6)
     api.softth_cuda(t, 0.5)
     assert np.array_equal(t.cpu().numpy(), (np.sign(a) * np.maximum(np.abs(a) - np.float32(0.5), 0)).astype(np.float32))
+
+
+def test_denoise_graph_replays_match_eager(api):
+    print("case denoise graph", flush=True)
+    L, sigma = 256, 20.0
+    _, x = image(L, seed=21, sigma=sigma)
+    shear = api.make_dictionary(L)
+    xg = to_gpu(x)
+    sc = api.nsst_scalars_cuda(shear, to_gpu(np.random.default_rng(3).standard_normal((L, L))))
+    ref, _ = api.denoise_cuda(xg, shear, sigma, sc)
+    ref = ref.cpu().numpy()
+    g, Ct, out = api.denoise_graph(xg, shear, sigma, sc)
+    for r in range(3):
+        out.zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), ref), r
+        xg.mul_(1.0)  # eager work between replays
+    g.close()
+    shear.close()
+
+
+@pytest.mark.parametrize("L,nimg", [(256, 3), (512, 2)])
+def test_batch_matches_plain(api, L, nimg):
+    print("case batch L=%d nimg=%d" % (L, nimg), flush=True)
+    shear = api.make_dictionary(L)
+    xs = np.stack([image(L, seed=30 + i, sigma=10.0 + 5 * i)[1] for i in range(nimg)])
+    xg = to_gpu(xs)
+    sc = api.nsst_scalars_cuda(shear, to_gpu(np.random.default_rng(4).standard_normal((L, L))))
+    sig = [10.0 + 5 * i for i in range(nimg)]
+    th = api.batch_thresholds(shear, sig, sc)
+    b = api.Batch(shear, nimg)
+    coeff = b.alloc_coeffs()
+    out = torch.empty_like(xg)
+    b.denoise(xg, coeff, th, out)
+    # forward and plain inverse through the batch API
+    coeff2 = b.alloc_coeffs()
+    b.shear_trans(xg, coeff2)
+    rt = torch.empty_like(xg)
+    b.inverse_shear(coeff2, rt)
+    torch.cuda.synchronize()
+    for i in range(nimg):
+        pd, Ct = api.denoise_cuda(xg[i].contiguous(), shear, sig[i], sc)
+        assert np.array_equal(out[i].cpu().numpy(), pd.cpu().numpy()), i
+        flat = torch.cat([Ct[0][None]] + list(Ct[1:])).cpu().numpy()
+        assert np.array_equal(coeff[i].cpu().numpy(), flat), i
+        assert np.array_equal(coeff2[i].cpu().numpy(), flat), i
+        assert_close(rt[i].cpu().numpy(), xs[i], what="batch round trip %d" % i)
+    b.close()
+    shear.close()
+
+
+def test_mixed_size_batches_in_one_graph(api):
+    print("case mixed graph", flush=True)
+    plans = []
+    for L, n in [(256, 2), (512, 1)]:
+        shear = api.make_dictionary(L)
+        xs = np.stack([image(L, seed=40 + i)[1] for i in range(n)])
+        sc = api.nsst_scalars_cuda(shear, to_gpu(np.random.default_rng(5).standard_normal((L, L))))
+        b = api.Batch(shear, n)
+        plans.append((shear, b, to_gpu(xs), b.alloc_coeffs(), api.batch_thresholds(shear, [20.0] * n, sc),
+                      torch.empty((n, L, L), device="cuda"), sc))
+    g = api.capture(lambda: [b.denoise(x, c, t, o) for (_, b, x, c, t, o, _) in plans])
+    for _ in range(2):
+        for p in plans:
+            p[5].zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        for shear, b, x, c, t, o, sc in plans:
+            for i in range(b.nimg):
+                pd, _ = api.denoise_cuda(x[i].contiguous(), shear, 20.0, sc)
+                assert np.array_equal(o[i].cpu().numpy(), pd.cpu().numpy())
+    g.close()
+    for p in plans:
+        p[1].close()
+        p[0].close()
