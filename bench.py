@@ -107,12 +107,13 @@ def profile_plain(shear, xg, sc, sigma, reps):
 def bench_plain(L, sigma, steps, warmup):
     from shearcuda_amd import api
     shear, xg, sc = plain_setup(L, sigma)
+    sched = api.autotune(shear)
     g, Ct, out = api.denoise_graph(xg, shear, sigma, sc)
     el = _timed(g.launch, steps, warmup)
     stages = profile_plain(shear, xg, sc, sigma, max(20, steps))
     g.close()
     shear.close()
-    return el * 1e3 / steps, stages
+    return el * 1e3 / steps, stages, sched
 
 
 # mixed-size batch: 48 x 256^2, 32 x 512^2, 16 x 1024^2, sigma ~ U[5, 40]
@@ -135,7 +136,8 @@ def bench_batched(steps, warmup, mix=BATCH_MIX):
         xs = np.stack([noisy_ellipses(L, sigma=float(sig[i]), seed=seed + i)[1] for i in range(n)]).astype(np.float32)
         seed += n
         b = api.Batch(shear, n)
-        plans.append(dict(L=L, n=n, shear=shear, sc=sc, sig=sig, b=b, x=torch.from_numpy(xs).cuda(),
+        sched = api.autotune(shear, b)
+        plans.append(dict(sched=sched, L=L, n=n, shear=shear, sc=sc, sig=sig, b=b, x=torch.from_numpy(xs).cuda(),
                           c=b.alloc_coeffs(), th=api.batch_thresholds(shear, sig, sc),
                           out=torch.empty((n, L, L), device="cuda")))
         npx += n * L * L
@@ -159,7 +161,8 @@ def bench_batched(steps, warmup, mix=BATCH_MIX):
     return {"images": sum(n for _, n in mix), "pixels": npx, "mix": [[L, n] for L, n in mix],
             "ms_per_launch": round(el * 1e3 / steps, 4), "Mpixel_s": round(batched, 2),
             "plain_image_by_image_Mpixel_s": round(plain, 2), "speedup_vs_plain": round(batched / plain, 3),
-            "path_roofline_frac": round(400.0 * npx / (el / steps) / 1e9 / HBM_PEAK_GBS, 4)}
+            "path_roofline_frac": round(400.0 * npx / (el / steps) / 1e9 / HBM_PEAK_GBS, 4),
+            "schedules": {str(p["L"]): p["sched"] for p in plans}}
 
 
 def main():
@@ -179,7 +182,7 @@ def main():
     torch.cuda.set_device(0)
 
     L, sigma = args.size, 20.0
-    ms, stages = bench_plain(L, sigma, args.steps, args.warmup)
+    ms, stages, sched = bench_plain(L, sigma, args.steps, args.warmup)
     value = L * L / (ms * 1e-3) / 1e6
     dom = max(stages, key=lambda s: s["ms"])
     roof = {"kernel": dom["name"], "bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS,
@@ -194,14 +197,14 @@ def main():
         "config": {"workload": "plain 2D NSST hard-threshold denoise (dec+thresh+rec), one %dx%d FP32 image per step, "
                                "maxflat 4 levels, directions [8,8,16,16], Meyer sizes [32,32,16,16], T=[0,3,4], "
                                "sigma=20, one hipGraph launch per step" % (L, L),
-                   "L": L, "images_per_step": 1, "form": "plain"},
+                   "L": L, "images_per_step": 1, "form": "plain", "schedule": sched},
         "roofline": roof,
         "path_roofline": {"bytes_per_step": path_bytes, "achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
         "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()} for s in stages],
     }
     if L == 512:
-        ms1k, _ = bench_plain(1024, sigma, max(20, args.steps // 4), 5)
+        ms1k, _, _ = bench_plain(1024, sigma, max(20, args.steps // 4), 5)
         res["plain_1024"] = {"ms_per_image": round(ms1k, 5), "Mpixel_s": round(1024 * 1024 / (ms1k * 1e-3) / 1e6, 2),
                              "path_roofline_frac": round(400.0 * 1024 * 1024 / (ms1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if not args.no_batched:

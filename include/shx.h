@@ -118,6 +118,16 @@ int shx_batch_inverse_shear(shx_batch* batch, const float* coeff, const float* t
 int shx_batch_denoise(shx_batch* batch, const float* x, float* coeff, const float* th_dev, float* out,
                       void* stream);
 
+/* Autotuner: times the pyramid tile shapes and the reconstruction's
+ * directions-per-block grouping on the plan's own workspace (the dictionary's
+ * one-image plan when batch is NULL) and keeps the fastest.  chosen (may be
+ * NULL) receives {pyramid variant, group}.  Allocates scratch and
+ * synchronises: call it outside graph capture.  No counterpart in the
+ * reference. */
+int shx_autotune(shx_dict* dict, shx_batch* batch, int reps, int* chosen, void* stream);
+/* Sets a schedule explicitly (pyramid variant 0..2, group 2/4/8). */
+int shx_set_schedule(shx_dict* dict, shx_batch* batch, int pyr_variant, int group);
+
 /* ---- HIP graphs --------------------------------------------------------------
  * Any sequence of shx calls that neither allocates nor synchronises
  * (shx_shear_trans, shx_inverse_shear*, shx_denoise, shx_batch_*) can be

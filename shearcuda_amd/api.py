@@ -450,3 +450,17 @@ def batch_thresholds(shear, sigmas, scalars, Tscalars=DEFAULT_TSCALARS):
     image noise level."""
     rows = [band_thresholds(shear, float(s), scalars, Tscalars) for s in np.atleast_1d(sigmas)]
     return torch.from_numpy(np.array(rows, dtype=np.float32)).to(shear.device)
+
+
+def autotune(shear, batch=None, reps=5):
+    """Pick the pyramid tile shape and the reconstruction's directions per
+    column block for a plan by timing them (outside any graph capture).
+    Returns {'pyramid_variant': v, 'group': g}."""
+    chosen = (ctypes.c_int * 2)()
+    check(lib.shx_autotune(shear.handle, batch._h if batch is not None else None, int(reps), chosen, _stream()))
+    return {"pyramid_variant": chosen[0], "group": chosen[1]}
+
+
+def set_schedule(shear, pyramid_variant, group, batch=None):
+    check(lib.shx_set_schedule(shear.handle, batch._h if batch is not None else None, int(pyramid_variant),
+                               int(group)))

@@ -18,8 +18,9 @@ LIB = os.path.join(HERE, "libshx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["api.hip", "pyramid.hip", "spectral.hip", "maxflat.cpp", "dict_host.cpp"]
+GEN = os.path.join(BUILD, "gen")
 
-COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + os.path.join(ROOT, "include")]
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + GEN, "-I" + os.path.join(ROOT, "include")]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
@@ -37,8 +38,23 @@ def _compile(src):
     return obj
 
 
+def _generate():
+    """Compile and run gen_factors (host) to write maxflat_factors.h."""
+    os.makedirs(GEN, exist_ok=True)
+    exe = os.path.join(BUILD, "gen_factors")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I" + CSRC, os.path.join(CSRC, "gen_factors.cpp"),
+           os.path.join(CSRC, "maxflat.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("generator build failed: %s\n%s" % (r.stdout, r.stderr))
+    r = subprocess.run([exe, os.path.join(GEN, "maxflat_factors.h")], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("maxflat factor generation failed: %s %s" % (r.stdout, r.stderr))
+
+
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
+    _generate()
     jobs = min(len(SOURCES), int(os.environ.get("MAX_JOBS", "8")))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))

@@ -51,6 +51,10 @@ struct Work {
   float2* q = nullptr;      // per image: ngroups packed partial reconstructions
   long long s_planes = 0, s_spec = 0, s_band = 0, s_q = 0;  // floats per image
   size_t bytes = 0;
+  // schedule picked by the autotuner (defaults until shx_autotune runs)
+  int pyr_variant = 0;  // pyramid tile shape
+  int group = 4;        // directions per reconstruction column block
+  BandMap map{};        // the dictionary's band map with this plan's groups
 };
 
 struct shx_dict {
@@ -131,8 +135,7 @@ hipError_t pyr_dec(const float* x, int L, int nlev, float* const* y, float* tA, 
   const float* in = x;
   for (int lev = 0; lev < nlev; ++lev) {
     float* low = (lev == nlev - 1) ? y[0] : ((lev & 1) ? tB : tA);
-    hipError_t e = shx::launch_atrous_dec(in, low, y[nlev - lev], shx::ImgStrides{0, 0, 0}, 1, L, 1 << lev,
-                                          shx::dec_filters(), s);
+    hipError_t e = shx::launch_atrous_dec(in, low, y[nlev - lev], shx::ImgStrides{0, 0, 0}, 1, L, 1 << lev, s);
     if (e != hipSuccess) return e;
     in = low;
   }
@@ -149,7 +152,7 @@ hipError_t pyr_rec(const float* const* y, int L, int nlev, float* out, float* tA
     const bool first = (lev == nlev - 1);
     const float sx = (first && w) ? (float)w[0] : 1.0f;
     const float sy = w ? (float)w[j] : 1.0f;
-    hipError_t e = shx::launch_atrous_rec(xl, y[j], o, shx::ImgStrides{0, 0, 0}, 1, L, 1 << lev, shx::rec_filters(),
+    hipError_t e = shx::launch_atrous_rec(xl, y[j], o, shx::ImgStrides{0, 0, 0}, 1, L, 1 << lev,
                                           shx::LowThresh{0, 0.f, nullptr, 0}, sx, sy, s);
     if (e != hipSuccess) return e;
     xl = o;
@@ -165,12 +168,26 @@ int check_dict(const shx_dict* d) {
   return SHX_OK;
 }
 
+// largest number of direction groups any schedule uses (groups of 2)
+int max_groups(const shx_dict* d) {
+  int g = 0;
+  for (int j = 0; j < d->S; ++j) g += (d->ndir[j] + 1) / 2;
+  return g;
+}
+
+void set_group(const shx_dict* d, Work& w, int g) {
+  w.group = g;
+  w.map = d->map;
+  shx::band_map_groups(w.map, g);
+}
+
 int alloc_work(const shx_dict* d, int nimg, Work& w) {
   const long long pl = (long long)d->L * d->L;
+  set_group(d, w, nimg == 1 ? 4 : 4);
   w.s_planes = pl * (d->S + 2);
   w.s_spec = pl * d->S;
   w.s_band = pl * d->nbands;
-  w.s_q = pl * d->map.ngroups;
+  w.s_q = pl * max_groups(d);
   const size_t n = (size_t)nimg;
   hipError_t e;
   if ((e = hipMalloc(&w.planes, sizeof(float) * w.s_planes * n)) != hipSuccess ||
@@ -187,7 +204,9 @@ void free_work(Work& w) {
   hipFree(w.spec);
   hipFree(w.band);
   hipFree(w.q);
-  w = Work{};
+  w.planes = nullptr;
+  w.spec = w.band = w.q = nullptr;
+  w.bytes = 0;
 }
 
 // Coefficients of nimg images: lowpass plane (+ per-image stride) and bands.
@@ -249,7 +268,7 @@ void stages_pyr_dec(shx_dict* d, const Work& w, int nimg, const float* x, long l
     const int m = 1 << lev;
     shx::ImgStrides is{ins, lows, w.s_planes};
     st.push_back({"atrous_dec_m" + std::to_string(m), 3 * P,
-                  [=] { return shx::launch_atrous_dec(in, low, high, is, nimg, L, m, shx::dec_filters(), s); }});
+                  [=] { return shx::launch_atrous_dec(in, low, high, is, nimg, L, m, s, w.pyr_variant); }});
     in = low;
     ins = lows;
   }
@@ -265,7 +284,7 @@ void stages_spec_dec(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::v
                 [=] { return shx::launch_rows_r2c(L, yset, specset, S, nimg, none, d->tw, s); }});
   st.push_back({"cols_dec", (S + 2.0 * nb) * P, [=] {
                   return shx::launch_cols_dec(L, w.spec, w.s_spec / 2, w.band, w.s_band / 2, d->Hmain, d->Hbeta,
-                                              d->map, nimg, d->tw, s);
+                                              w.map, nimg, d->tw, s);
                 }});
 }
 
@@ -283,15 +302,15 @@ shx::ThreshSpec band_thresh(const shx_dict* d, const Thr& t) {
 }
 
 void stages_spec_rec_tail(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::vector<Stage>& st) {
-  const int L = d->L, S = d->S, nb = d->nbands, ng = d->map.ngroups;
+  const int L = d->L, S = d->S, nb = d->nbands, ng = w.map.ngroups;
   const double P = 4.0 * L * L * nimg;
   st.push_back({"cols_rec", (2.0 * nb + ng) * P, [=] {
-                  return shx::launch_cols_rec(L, w.band, w.s_band / 2, w.q, w.s_q / 2, d->Hmain, d->Hbeta, d->map,
+                  return shx::launch_cols_rec(L, w.band, w.s_band / 2, w.q, w.s_q / 2, d->Hmain, d->Hbeta, w.map,
                                               nimg, d->tw, s);
                 }});
   PlaneSet qset = shx::single_cell(reinterpret_cast<float*>(w.q), ng, w.s_q);
   PlaneSet yset = shx::single_cell(w.planes, S, w.s_planes);
-  std::vector<int> sf(d->map.group_first, d->map.group_first + S + 1);
+  std::vector<int> sf(w.map.group_first, w.map.group_first + S + 1);
   st.push_back({"rows_c2r_scales", (ng + S) * P,
                 [=] { return shx::launch_rows_c2r(L, qset, yset, S, nimg, sf.data(), d->tw, s); }});
 }
@@ -315,7 +334,7 @@ void stages_pyr_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const
     const shx::LowThresh tl = (lev == S - 1) ? lt : shx::LowThresh{0, 0.f, nullptr, 0};
     shx::ImgStrides is{xls, w.s_planes, os};
     st.push_back({"atrous_rec_m" + std::to_string(m), 3 * P, [=] {
-                    return shx::launch_atrous_rec(xl, yh, o, is, nimg, L, m, shx::rec_filters(), tl, 1.0f, 1.0f, s);
+                    return shx::launch_atrous_rec(xl, yh, o, is, nimg, L, m, tl, 1.0f, 1.0f, s, w.pyr_variant);
                   }});
     xl = o;
     xls = os;
@@ -434,6 +453,84 @@ int profile_stages(const std::vector<Stage>& st, hipStream_t s, int reps, int ma
       strncpy(names + (size_t)i * name_len, st[i].name.c_str(), name_len - 1);
       names[(size_t)i * name_len + name_len - 1] = 0;
     }
+  }
+  return SHX_OK;
+}
+
+// Mean duration (ms) of `reps` runs of a stage list.
+double time_stages(const std::vector<Stage>& st, hipStream_t s, int reps) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return -1.0;
+  if (hipEventCreate(&b) != hipSuccess) {
+    hipEventDestroy(a);
+    return -1.0;
+  }
+  double ms = -1.0;
+  if (run_stages(st) == SHX_OK && hipEventRecord(a, s) == hipSuccess) {
+    bool ok = true;
+    for (int r = 0; r < reps && ok; ++r) ok = run_stages(st) == SHX_OK;
+    float t = 0.f;
+    if (ok && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+        hipEventElapsedTime(&t, a, b) == hipSuccess)
+      ms = t / reps;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return ms;
+}
+
+// Autotune one plan: pyramid tile shape, then reconstruction direction
+// groups, each by timing the kernels it affects on the plan's own workspace
+// and scratch images.
+int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t s, int* chosen) {
+  const long long pl = (long long)d->L * d->L;
+  float* img = nullptr;
+  if (hipMalloc(&img, sizeof(float) * 2 * pl * nimg) != hipSuccess)
+    return fail(SHX_ERR_ALLOC, "autotune scratch allocation failed");
+  hipMemsetAsync(img, 0, sizeof(float) * 2 * pl * nimg, s);
+  hipMemsetAsync(w.planes, 0, sizeof(float) * w.s_planes * nimg, s);
+  hipMemsetAsync(w.band, 0, sizeof(float) * w.s_band * nimg, s);
+  float* coeff = nullptr;
+  if (hipMalloc(&coeff, sizeof(float) * pl * (1 + d->nbands) * nimg) != hipSuccess) {
+    hipFree(img);
+    return fail(SHX_ERR_ALLOC, "autotune scratch allocation failed");
+  }
+  hipMemsetAsync(coeff, 0, sizeof(float) * pl * (1 + d->nbands) * nimg, s);
+  Coeffs c = stack_coeffs(d, coeff);
+  double best = 1e30;
+  int bestv = 0;
+  for (int v = 0; v < shx::PYR_VARIANTS; ++v) {
+    w.pyr_variant = v;
+    std::vector<Stage> st;
+    stages_pyr_dec(d, w, nimg, img, pl, c, s, st);
+    stages_pyr_rec(d, w, nimg, c, Thr{}, img + pl * nimg, pl, s, st);
+    const double t = time_stages(st, s, reps);
+    if (t > 0 && t < best) {
+      best = t;
+      bestv = v;
+    }
+  }
+  w.pyr_variant = bestv;
+  best = 1e30;
+  int bestg = 4;
+  const int cand[3] = {2, 4, 8};
+  for (int g : cand) {
+    set_group(d, w, g);
+    std::vector<Stage> st;
+    stages_spec_rec_tail(d, w, nimg, s, st);
+    const double t = time_stages(st, s, reps);
+    if (t > 0 && t < best) {
+      best = t;
+      bestg = g;
+    }
+  }
+  set_group(d, w, bestg);
+  hipStreamSynchronize(s);
+  hipFree(img);
+  hipFree(coeff);
+  if (chosen) {
+    chosen[0] = w.pyr_variant;
+    chosen[1] = w.group;
   }
   return SHX_OK;
 }
@@ -681,6 +778,24 @@ int shx_batch_denoise(shx_batch* b, const float* x, float* coeff, const float* t
   const long long pl = (long long)d->L * d->L;
   stages_denoise(d, b->w, b->nimg, x, pl, stack_coeffs(d, coeff), t, out, pl, (hipStream_t)stream, st);
   return run_stages(st);
+}
+
+int shx_autotune(shx_dict* d, shx_batch* b, int reps, int* chosen, void* stream) {
+  if (int r = check_dict(d)) return r;
+  if (b && b->d != d) return fail(SHX_ERR_ARG, "batch belongs to another dictionary");
+  if (reps < 1) reps = 3;
+  Work& w = b ? b->w : d->w;
+  return autotune_work(d, w, b ? b->nimg : 1, reps, (hipStream_t)stream, chosen);
+}
+
+int shx_set_schedule(shx_dict* d, shx_batch* b, int pyr_variant, int group) {
+  if (!d) return fail(SHX_ERR_ARG, "null dictionary");
+  if (pyr_variant < 0 || pyr_variant >= shx::PYR_VARIANTS) return fail(SHX_ERR_ARG, "no pyramid variant %d", pyr_variant);
+  if (group != 2 && group != 4 && group != 8) return fail(SHX_ERR_ARG, "direction group must be 2, 4 or 8");
+  Work& w = b ? b->w : d->w;
+  w.pyr_variant = pyr_variant;
+  set_group(d, w, group);
+  return SHX_OK;
 }
 
 // ---- graphs --------------------------------------------------------------------

@@ -2,7 +2,8 @@
 //
 // A workgroup holds B transforms of N complex points in LDS; element (b, i)
 // lives at buf[b * bs + i * es].  Rows use es = 1, bs = N (+pad); column tiles
-// use es = tile width, bs = 1.  Each stage: every thread loads the R inputs of
+// use es = tile width, bs = 1.  Element index e maps to LDS slot
+// lds_ix<PADSH>(e).  Each stage: every thread loads the R inputs of
 // its butterflies into registers, applies the twiddles and the radix-R DFT,
 // waits at a barrier, and writes the R outputs to their autosorted places, so
 // the transform is in place in LDS and its output is in natural order.
@@ -77,11 +78,19 @@ struct StageRadix {
   static constexpr int value = ((N / Ns) % 8 == 0) ? 8 : (((N / Ns) % 4 == 0) ? 4 : 2);
 };
 
+// LDS index of linear element e with one padding slot every 2^PADSH elements
+// (spreads the strided Stockham writes over the banks).
+template <int PADSH>
+__device__ __forceinline__ int lds_ix(int e) {
+  return PADSH > 0 ? e + (e >> PADSH) : e;
+}
+
 // One workgroup-wide Stockham stage, then the remaining ones (compile-time
 // recursion).  All threads of the block must call it.
 //   BFAST: consecutive threads take consecutive transforms (column tiles),
 //          otherwise consecutive butterflies of one transform (rows).
-template <int N, int TWS, int DIR, int B, int T, bool BFAST, int Ns = 1>
+
+template <int N, int TWS, int DIR, int B, int T, bool BFAST, int PADSH = 0, int Ns = 1>
 __device__ __forceinline__ void fft_lds(float2* buf, int bs, int es, const float2* __restrict__ tw, int tid) {
   if constexpr (Ns < N) {
     constexpr int R = StageRadix<N, Ns>::value;
@@ -96,9 +105,8 @@ __device__ __forceinline__ void fft_lds(float2* buf, int bs, int es, const float
       if ((NB % T) == 0 || q < NB) {
         const int b = BFAST ? (q % B) : (q / NR);
         const int j = BFAST ? (q / B) : (q % NR);
-        const float2* src = buf + b * bs;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[p][r] = src[(j + r * NR) * es];
+        for (int r = 0; r < R; ++r) v[p][r] = buf[lds_ix<PADSH>(b * bs + (j + r * NR) * es)];
         if constexpr (Ns > 1) {
           const int jm = j & (Ns - 1);
 #pragma unroll
@@ -120,13 +128,12 @@ __device__ __forceinline__ void fft_lds(float2* buf, int bs, int es, const float
         const int j = BFAST ? (q / B) : (q % NR);
         const int jm = j & (Ns - 1);
         const int base = (j - jm) * R + jm;
-        float2* dst = buf + b * bs;
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[(base + r * Ns) * es] = v[p][r];
+        for (int r = 0; r < R; ++r) buf[lds_ix<PADSH>(b * bs + (base + r * Ns) * es)] = v[p][r];
       }
     }
     __syncthreads();
-    fft_lds<N, TWS, DIR, B, T, BFAST, Ns * R>(buf, bs, es, tw, tid);
+    fft_lds<N, TWS, DIR, B, T, BFAST, PADSH, Ns * R>(buf, bs, es, tw, tid);
   }
 }
 

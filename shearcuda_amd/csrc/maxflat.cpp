@@ -5,7 +5,8 @@
 // 13x13 (h0, g1) and 19x19 (h1, g0) as atrousfilters.m:167-175 does.  Each full
 // filter is symmetric, so h = sum_k lambda_k u_k u_k^T; a cyclic Jacobi
 // eigensolver in FP64 finds the nonzero terms (h0: 2, h1: 4, g0: 3, g1: 3)
-// and the kernels use v_k = u_k, w_k = lambda_k u_k.
+// and the kernels use v_k = u_k, w_k = lambda_k u_k (written as FP32
+// compile-time constants by gen_factors.cpp).
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -161,21 +162,8 @@ Factors factorize(const Full& f) {
 struct Maxflat {
   Full full[4];
   Factors fac[4];
-  DecFilters dec{};
-  RecFilters rec{};
   bool ok = false;
 };
-
-template <int LEN, int RANK>
-bool fill(SepFilter<LEN, RANK>& s, const Factors& F) {
-  if (F.len != LEN || F.rank != RANK) return false;
-  for (int k = 0; k < RANK; ++k)
-    for (int a = 0; a < LEN; ++a) {
-      s.v[k][a] = (float)F.v[(size_t)k * LEN + a];
-      s.w[k][a] = (float)F.w[(size_t)k * LEN + a];
-    }
-  return true;
-}
 
 const Maxflat& maxflat() {
   static Maxflat M;
@@ -186,16 +174,16 @@ const Maxflat& maxflat() {
     M.full[2] = mirror(&kG0Q[0][0], 10);
     M.full[3] = mirror(&kG1Q[0][0], 7);
     for (int i = 0; i < 4; ++i) M.fac[i] = factorize(M.full[i]);
-    M.ok = fill(M.dec.h0, M.fac[0]) && fill(M.dec.h1, M.fac[1]) && fill(M.rec.g0, M.fac[2]) &&
-           fill(M.rec.g1, M.fac[3]);
+    // ranks the pyramid kernels are specialised for
+    const int want[4] = {2, 4, 3, 3}, len[4] = {13, 19, 19, 13};
+    M.ok = true;
+    for (int i = 0; i < 4; ++i) M.ok = M.ok && M.fac[i].rank == want[i] && M.fac[i].len == len[i];
   });
   return M;
 }
 
 }  // namespace
 
-const DecFilters& dec_filters() { return maxflat().dec; }
-const RecFilters& rec_filters() { return maxflat().rec; }
 bool maxflat_ok() { return maxflat().ok; }
 
 const double* maxflat_filter(int which, int* len) {
@@ -203,6 +191,16 @@ const double* maxflat_filter(int which, int* len) {
   const Full& f = maxflat().full[which];
   if (len) *len = f.len;
   return f.h.data();
+}
+
+bool maxflat_factors_fp64(int which, int* len, int* rank, const double** v, const double** w) {
+  if (which < 0 || which > 3) return false;
+  const Factors& F = maxflat().fac[which];
+  *len = F.len;
+  *rank = F.rank;
+  *v = F.v.data();
+  *w = F.w.data();
+  return true;
 }
 
 double maxflat_factor_error() {

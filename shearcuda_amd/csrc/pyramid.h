@@ -5,31 +5,12 @@
 
 namespace shx {
 
-// Separable factors of a symmetric LEN x LEN filter of rank RANK:
-//   h(a,b) = sum_k v[k][a] * w[k][b]
-template <int LEN, int RANK>
-struct SepFilter {
-  float v[RANK][LEN];
-  float w[RANK][LEN];
-};
-
-struct DecFilters {
-  SepFilter<13, 2> h0;
-  SepFilter<19, 4> h1;
-};
-
-struct RecFilters {
-  SepFilter<19, 3> g0;
-  SepFilter<13, 3> g1;
-};
-
-// Maxflat filter tables (FP64) and their separable factors, built once.
-const DecFilters& dec_filters();
-const RecFilters& rec_filters();
 // Full FP64 filters, row-major LEN x LEN: which = 0 h0, 1 h1, 2 g0, 3 g1.
 const double* maxflat_filter(int which, int* len);
 // Largest |h - sum_k v_k w_k^T| over the four filters (FP64 factors).
 double maxflat_factor_error();
+// FP64 separable factors of filter `which` (v, w: rank x len, row-major).
+bool maxflat_factors_fp64(int which, int* len, int* rank, const double** v, const double** w);
 // True when every filter has the rank the kernels are specialised for.
 bool maxflat_ok();
 
@@ -48,9 +29,11 @@ struct LowThresh {
   int stride;
 };
 
+constexpr int PYR_VARIANTS = 3;  // tile shapes the autotuner chooses from
+
 hipError_t launch_atrous_dec(const float* x, float* ylow, float* yhigh, const ImgStrides& st, int nimg, int L, int m,
-                             const DecFilters& f, hipStream_t s);
+                             hipStream_t s, int variant = 0);
 hipError_t launch_atrous_rec(const float* xlow, const float* yhigh, float* out, const ImgStrides& st, int nimg, int L,
-                             int m, const RecFilters& f, const LowThresh& lt, float sx, float sy, hipStream_t s);
+                             int m, const LowThresh& lt, float sx, float sy, hipStream_t s, int variant = 0);
 
 }  // namespace shx

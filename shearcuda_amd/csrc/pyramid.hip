@@ -17,18 +17,19 @@
 // The maxflat filters have exact low rank (h0: 2, h1: 4, g0: 3, g1: 3), so
 // h = sum_k v_k (x) w_k is applied as RANK vertical 1D passes into LDS
 // followed by RANK horizontal 1D passes, skipping the taps 1 and LEN-2 that
-// are zero in every maxflat filter.
+// are zero in every maxflat filter.  The factors are compile-time constants
+// (maxflat_factors.h), so every tap is an FMA with a literal operand.
+#include "maxflat_factors.h"  // generated at build time by gen_factors.cpp
 #include "pyramid.h"
 
 namespace shx {
 
 constexpr int HALO = 9;
 
+// n is a power of two, so s mod 2n is s & (2n-1) for any sign of s.
 __device__ __forceinline__ int fold_index(int s, int n) {
-  const int p = 2 * n;
-  int t = s % p;
-  if (t < 0) t += p;
-  return t < n ? t : p - 1 - t;
+  const int t = s & (2 * n - 1);
+  return t < n ? t : 2 * n - 1 - t;
 }
 
 template <int LEN>
@@ -38,7 +39,7 @@ __device__ __forceinline__ constexpr bool zero_tap(int a) {
 
 // Vertical pass of one filter component over a strip of VS rows.
 template <int LEN, int VS>
-__device__ __forceinline__ void vpass(const float (&col)[VS + 2 * HALO], const float* __restrict__ v, float (&t)[VS]) {
+__device__ __forceinline__ void vpass(const float (&col)[VS + 2 * HALO], const float (&v)[LEN], float (&t)[VS]) {
   constexpr int off = HALO - (LEN - 1) / 2;
 #pragma unroll
   for (int i = 0; i < VS; ++i) {
@@ -54,7 +55,7 @@ __device__ __forceinline__ void vpass(const float (&col)[VS + 2 * HALO], const f
 
 // Horizontal pass: acc[j] += sum_b w[b] * row[j + b + off]
 template <int LEN, int HS>
-__device__ __forceinline__ void hpass(const float* __restrict__ row, const float* __restrict__ w, float (&acc)[HS]) {
+__device__ __forceinline__ void hpass(const float* __restrict__ row, const float (&w)[LEN], float (&acc)[HS]) {
   constexpr int off = HALO - (LEN - 1) / 2;
   float r[HS + LEN - 1];
 #pragma unroll
@@ -90,11 +91,11 @@ __device__ __forceinline__ void load_tile(float* __restrict__ tile, const float*
 // grid: (ceil(L/m/TW), ceil(L/m/TH), m*m)
 template <int TH, int TW, int VS, int HS, int T>
 __global__ void __launch_bounds__(T) k_atrous_dec(const float* __restrict__ x, float* __restrict__ ylow,
-                                                   float* __restrict__ yhigh, ImgStrides st3, int L, int m,
-                                                   DecFilters f) {
+                                                   float* __restrict__ yhigh, ImgStrides st3, int L, int m) {
   constexpr int W = TW + 2 * HALO;
   constexpr int WP = W | 1;  // odd row stride for the intermediates
-  constexpr int R0N = 2, R1N = 4;
+  constexpr int R0N = mf::H0_RANK, R1N = mf::H1_RANK;
+  static_assert(mf::H0_LEN == 13 && mf::H1_LEN == 19, "maxflat h0/h1 sizes");
   __shared__ float tile[(TH + 2 * HALO) * W];
   __shared__ float tv[R0N + R1N][TH * WP];
   const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
@@ -115,13 +116,13 @@ __global__ void __launch_bounds__(T) k_atrous_dec(const float* __restrict__ x, f
     float t[VS];
 #pragma unroll
     for (int k = 0; k < R0N; ++k) {
-      vpass<13, VS>(col, f.h0.v[k], t);
+      vpass<13, VS>(col, mf::H0_V[k], t);
 #pragma unroll
       for (int i = 0; i < VS; ++i) tv[k][(st * VS + i) * WP + cc] = t[i];
     }
 #pragma unroll
     for (int k = 0; k < R1N; ++k) {
-      vpass<19, VS>(col, f.h1.v[k], t);
+      vpass<19, VS>(col, mf::H1_V[k], t);
 #pragma unroll
       for (int i = 0; i < VS; ++i) tv[R0N + k][(st * VS + i) * WP + cc] = t[i];
     }
@@ -134,9 +135,9 @@ __global__ void __launch_bounds__(T) k_atrous_dec(const float* __restrict__ x, f
 #pragma unroll
     for (int j = 0; j < HS; ++j) a0[j] = a1[j] = 0.f;
 #pragma unroll
-    for (int k = 0; k < R0N; ++k) hpass<13, HS>(&tv[k][r * WP + st * HS], f.h0.w[k], a0);
+    for (int k = 0; k < R0N; ++k) hpass<13, HS>(&tv[k][r * WP + st * HS], mf::H0_W[k], a0);
 #pragma unroll
-    for (int k = 0; k < R1N; ++k) hpass<19, HS>(&tv[R0N + k][r * WP + st * HS], f.h1.w[k], a1);
+    for (int k = 0; k < R1N; ++k) hpass<19, HS>(&tv[R0N + k][r * WP + st * HS], mf::H1_W[k], a1);
     const int R = R0 + r;
     if (R < nsub) {
       const size_t rowoff = (size_t)(pr + m * R) * L;
@@ -157,11 +158,12 @@ __global__ void __launch_bounds__(T) k_atrous_dec(const float* __restrict__ x, f
 // Reconstruction level: out = g0 * (sx * keep(xlow, th)) + g1 * (sy * yhigh).
 template <int TH, int TW, int VS, int HS, int T>
 __global__ void __launch_bounds__(T) k_atrous_rec(const float* __restrict__ xlow, const float* __restrict__ yhigh,
-                                                   float* __restrict__ out, ImgStrides st3, int L, int m, RecFilters f,
-                                                   LowThresh lt, float sx, float sy) {
+                                                   float* __restrict__ out, ImgStrides st3, int L, int m, LowThresh lt,
+                                                   float sx, float sy) {
   constexpr int W = TW + 2 * HALO;
   constexpr int WP = W | 1;
-  constexpr int RGN = 3, RHN = 3;
+  constexpr int RGN = mf::G0_RANK, RHN = mf::G1_RANK;
+  static_assert(mf::G0_LEN == 19 && mf::G1_LEN == 13, "maxflat g0/g1 sizes");
   __shared__ float tile[2][(TH + 2 * HALO) * W];
   __shared__ float tv[RGN + RHN][TH * WP];
   const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
@@ -186,14 +188,14 @@ __global__ void __launch_bounds__(T) k_atrous_rec(const float* __restrict__ xlow
     if (which == 0) {
 #pragma unroll
       for (int k = 0; k < RGN; ++k) {
-        vpass<19, VS>(col, f.g0.v[k], t);
+        vpass<19, VS>(col, mf::G0_V[k], t);
 #pragma unroll
         for (int i = 0; i < VS; ++i) tv[k][(st * VS + i) * WP + cc] = t[i];
       }
     } else {
 #pragma unroll
       for (int k = 0; k < RHN; ++k) {
-        vpass<13, VS>(col, f.g1.v[k], t);
+        vpass<13, VS>(col, mf::G1_V[k], t);
 #pragma unroll
         for (int i = 0; i < VS; ++i) tv[RGN + k][(st * VS + i) * WP + cc] = t[i];
       }
@@ -206,9 +208,9 @@ __global__ void __launch_bounds__(T) k_atrous_rec(const float* __restrict__ xlow
 #pragma unroll
     for (int j = 0; j < HS; ++j) a[j] = 0.f;
 #pragma unroll
-    for (int k = 0; k < RGN; ++k) hpass<19, HS>(&tv[k][r * WP + st * HS], f.g0.w[k], a);
+    for (int k = 0; k < RGN; ++k) hpass<19, HS>(&tv[k][r * WP + st * HS], mf::G0_W[k], a);
 #pragma unroll
-    for (int k = 0; k < RHN; ++k) hpass<13, HS>(&tv[RGN + k][r * WP + st * HS], f.g1.w[k], a);
+    for (int k = 0; k < RHN; ++k) hpass<13, HS>(&tv[RGN + k][r * WP + st * HS], mf::G1_W[k], a);
     const int R = R0 + r;
     if (R < nsub) {
       const size_t rowoff = (size_t)(pr + m * R) * L;
@@ -221,27 +223,65 @@ __global__ void __launch_bounds__(T) k_atrous_rec(const float* __restrict__ xlow
   }
 }
 
-namespace {
-constexpr int PTH = 16, PTW = 32, PVS = 4, PHS = 2, PT = 256;
+// Tile variants (sub-lattice tile TH x TW, vertical/horizontal strips VS/HS)
+// chosen per plan by the autotuner.
+template <int V>
+struct PyrCfg;
+template <>
+struct PyrCfg<0> {
+  static constexpr int TH = 16, TW = 32, VS = 4, HS = 2, T = 256;
+};
+template <>
+struct PyrCfg<1> {
+  static constexpr int TH = 32, TW = 32, VS = 8, HS = 4, T = 256;
+};
+template <>
+struct PyrCfg<2> {
+  static constexpr int TH = 32, TW = 64, VS = 8, HS = 8, T = 256;
+};
+
+template <int V>
+static hipError_t dec_v(const float* x, float* ylow, float* yhigh, const ImgStrides& st, int nimg, int L, int m,
+                        hipStream_t s) {
+  using C = PyrCfg<V>;
+  const int nsub = L / m;
+  dim3 grid((nsub + C::TW - 1) / C::TW, (nsub + C::TH - 1) / C::TH, m * m * nimg);
+  hipLaunchKernelGGL((k_atrous_dec<C::TH, C::TW, C::VS, C::HS, C::T>), grid, dim3(C::T), 0, s, x, ylow, yhigh, st, L,
+                     m);
+  return hipGetLastError();
+}
+
+template <int V>
+static hipError_t rec_v(const float* xlow, const float* yhigh, float* out, const ImgStrides& st, int nimg, int L,
+                        int m, const LowThresh& lt, float sx, float sy, hipStream_t s) {
+  using C = PyrCfg<V>;
+  const int nsub = L / m;
+  dim3 grid((nsub + C::TW - 1) / C::TW, (nsub + C::TH - 1) / C::TH, m * m * nimg);
+  hipLaunchKernelGGL((k_atrous_rec<C::TH, C::TW, C::VS, C::HS, C::T>), grid, dim3(C::T), 0, s, xlow, yhigh, out, st,
+                     L, m, lt, sx, sy);
+  return hipGetLastError();
 }
 
 hipError_t launch_atrous_dec(const float* x, float* ylow, float* yhigh, const ImgStrides& st, int nimg, int L, int m,
-                             const DecFilters& f, hipStream_t s) {
-  if (m < 1 || L % m != 0 || nimg < 1) return hipErrorInvalidValue;
-  const int nsub = L / m;
-  dim3 grid((nsub + PTW - 1) / PTW, (nsub + PTH - 1) / PTH, m * m * nimg);
-  hipLaunchKernelGGL((k_atrous_dec<PTH, PTW, PVS, PHS, PT>), grid, dim3(PT), 0, s, x, ylow, yhigh, st, L, m, f);
-  return hipGetLastError();
+                             hipStream_t s, int variant) {
+  if (m < 1 || L % m != 0 || nimg < 1 || m * m * nimg > 65535) return hipErrorInvalidValue;
+  switch (variant) {
+    case 0: return dec_v<0>(x, ylow, yhigh, st, nimg, L, m, s);
+    case 1: return dec_v<1>(x, ylow, yhigh, st, nimg, L, m, s);
+    case 2: return dec_v<2>(x, ylow, yhigh, st, nimg, L, m, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_atrous_rec(const float* xlow, const float* yhigh, float* out, const ImgStrides& st, int nimg, int L,
-                             int m, const RecFilters& f, const LowThresh& lt, float sx, float sy, hipStream_t s) {
-  if (m < 1 || L % m != 0 || nimg < 1) return hipErrorInvalidValue;
-  const int nsub = L / m;
-  dim3 grid((nsub + PTW - 1) / PTW, (nsub + PTH - 1) / PTH, m * m * nimg);
-  hipLaunchKernelGGL((k_atrous_rec<PTH, PTW, PVS, PHS, PT>), grid, dim3(PT), 0, s, xlow, yhigh, out, st, L, m, f, lt,
-                     sx, sy);
-  return hipGetLastError();
+                             int m, const LowThresh& lt, float sx, float sy, hipStream_t s, int variant) {
+  if (m < 1 || L % m != 0 || nimg < 1 || m * m * nimg > 65535) return hipErrorInvalidValue;
+  switch (variant) {
+    case 0: return rec_v<0>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
+    case 1: return rec_v<1>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
+    case 2: return rec_v<2>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace shx

@@ -53,15 +53,25 @@ __device__ __forceinline__ float thresh_of(const ThreshSpec& t, int p, int b) {
 }
 
 // ---------------------------------------------------------------------------
-// Row-pass building blocks on a block of ROWS rows held in LDS as
-// buf[r * BS + k], N = L/2 complex per row.
+// Row-pass building blocks on a block of ROWS rows held in LDS: element k of
+// row r (N = L/2 complex per row) is linear element r*N + k, at slot
+// lds_ix<RPAD>(r*N + k).
 
-// Packed half spectrum of a real row from the FFT of its (even, odd) pairs.
+constexpr int RPAD = 5;  // row buffers: one pad slot per 32 elements
+constexpr int CPAD = 4;  // column tiles: one pad slot per 16 elements
+
+template <int E, int PADSH>
+constexpr int lds_size() {
+  return E + (E >> PADSH) + 1;
+}
+
+// Packed half spectrum of a real row from the FFT of its (even, odd) pairs;
+// the row starts at linear element e0 of buf.
 template <int N>
-__device__ __forceinline__ float2 r2c_post(const float2* row, int k, const float2* __restrict__ tw) {
-  const float2 z = row[k];
+__device__ __forceinline__ float2 r2c_post(const float2* buf, int e0, int k, const float2* __restrict__ tw) {
+  const float2 z = buf[lds_ix<RPAD>(e0 + k)];
   if (k == 0) return make_float2(z.x + z.y, z.x - z.y);
-  const float2 zc = cconj(row[N - k]);
+  const float2 zc = cconj(buf[lds_ix<RPAD>(e0 + N - k)]);
   const float2 a = cscale(cadd(z, zc), 0.5f);
   const float2 b = cscale(csub(z, zc), 0.5f);
   const float2 wb = cmul(tw[k], b);  // X = a - i w^k b
@@ -70,31 +80,31 @@ __device__ __forceinline__ float2 r2c_post(const float2* row, int k, const float
 
 // FFT input (even + i odd, times L) of the inverse of a packed half spectrum.
 template <int N>
-__device__ __forceinline__ float2 c2r_pre(const float2* row, int k, const float2* __restrict__ tw) {
-  const float2 x = row[k];
+__device__ __forceinline__ float2 c2r_pre(const float2* buf, int e0, int k, const float2* __restrict__ tw) {
+  const float2 x = buf[lds_ix<RPAD>(e0 + k)];
   if (k == 0) return make_float2(x.x + x.y, x.x - x.y);
-  const float2 xc = cconj(row[N - k]);
+  const float2 xc = cconj(buf[lds_ix<RPAD>(e0 + N - k)]);
   const float2 fe = cadd(x, xc);
   const float2 fo = cmulc(csub(x, xc), tw[k]);  // * w^{-k}
   return make_float2(fe.x - fo.y, fe.y + fo.x);  // fe + i fo
 }
 
-// In-place map of every row element through f(row, k) (all reads, barrier,
+// In-place map of every row element through f(e0, k) (all reads, barrier,
 // all writes).
-template <int ROWS, int N, int BS, int T, class F>
+template <int ROWS, int N, int T, class F>
 __device__ __forceinline__ void rows_map(float2* buf, F f) {
   constexpr int PER = (ROWS * N + T - 1) / T;
   float2 z[PER];
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     const int idx = threadIdx.x + e * T;
-    if ((ROWS * N) % T == 0 || idx < ROWS * N) z[e] = f(buf + (idx / N) * BS, idx % N);
+    if ((ROWS * N) % T == 0 || idx < ROWS * N) z[e] = f(idx - idx % N, idx % N);
   }
   __syncthreads();
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     const int idx = threadIdx.x + e * T;
-    if ((ROWS * N) % T == 0 || idx < ROWS * N) buf[(idx / N) * BS + idx % N] = z[e];
+    if ((ROWS * N) % T == 0 || idx < ROWS * N) buf[lds_ix<RPAD>(idx)] = z[e];
   }
   __syncthreads();
 }
@@ -106,8 +116,7 @@ template <int L, int ROWS, int T>
 __global__ void __launch_bounds__(T) k_rows_r2c(PlaneSet in, PlaneSet out, ThreshSpec th,
                                                  const float2* __restrict__ tw) {
   constexpr int N = L / 2;
-  constexpr int BS = N + 1;
-  __shared__ float2 buf[ROWS * BS];
+  __shared__ float2 buf[lds_size<ROWS * N, RPAD>()];
   const int tid = threadIdx.x;
   const int p = blockIdx.y, b = blockIdx.z;
   const int row0 = blockIdx.x * ROWS;
@@ -123,12 +132,12 @@ __global__ void __launch_bounds__(T) k_rows_r2c(PlaneSet in, PlaneSet out, Thres
       v.x = hard_keep(v.x, t);
       v.y = hard_keep(v.y, t);
     }
-    buf[(idx / N) * BS + idx % N] = v;
+    buf[lds_ix<RPAD>(idx)] = v;
   }
   __syncthreads();
-  fft_lds<N, 2, -1, ROWS, T, false>(buf, BS, 1, tw, tid);
+  fft_lds<N, 2, -1, ROWS, T, false, RPAD>(buf, N, 1, tw, tid);
 #pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) dst[idx] = r2c_post<N>(buf + (idx / N) * BS, idx % N, tw);
+  for (int idx = tid; idx < ROWS * N; idx += T) dst[idx] = r2c_post<N>(buf, idx - idx % N, idx % N, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -138,8 +147,7 @@ template <int L, int ROWS, int T, bool SUM>
 __global__ void __launch_bounds__(T) k_rows_c2r(PlaneSet in, PlaneSet out, SumTable sum,
                                                  const float2* __restrict__ tw) {
   constexpr int N = L / 2;
-  constexpr int BS = N + 1;
-  __shared__ float2 buf[ROWS * BS];
+  __shared__ float2 buf[lds_size<ROWS * N, RPAD>()];
   const int tid = threadIdx.x;
   const int p = blockIdx.y, b = blockIdx.z;
   const int row0 = blockIdx.x * ROWS;
@@ -152,18 +160,18 @@ __global__ void __launch_bounds__(T) k_rows_c2r(PlaneSet in, PlaneSet out, SumTa
       float2 a = make_float2(0.f, 0.f);
       for (int q = q0; q < q1; ++q)
         a = cadd(a, reinterpret_cast<const float2*>(plane_ptr(in, q, pe, b))[(size_t)row0 * N + idx]);
-      buf[(idx / N) * BS + idx % N] = a;
+      buf[lds_ix<RPAD>(idx)] = a;
     }
   } else {
     const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe, b)) + (size_t)row0 * N;
 #pragma unroll
-    for (int idx = tid; idx < ROWS * N; idx += T) buf[(idx / N) * BS + idx % N] = src[idx];
+    for (int idx = tid; idx < ROWS * N; idx += T) buf[lds_ix<RPAD>(idx)] = src[idx];
   }
   __syncthreads();
-  rows_map<ROWS, N, BS, T>(buf, [&](const float2* row, int k) { return c2r_pre<N>(row, k, tw); });
-  fft_lds<N, 2, +1, ROWS, T, false>(buf, BS, 1, tw, tid);
+  rows_map<ROWS, N, T>(buf, [&](int e0, int k) { return c2r_pre<N>(buf, e0, k, tw); });
+  fft_lds<N, 2, +1, ROWS, T, false, RPAD>(buf, N, 1, tw, tid);
 #pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) dst[idx] = buf[(idx / N) * BS + idx % N];
+  for (int idx = tid; idx < ROWS * N; idx += T) dst[idx] = buf[lds_ix<RPAD>(idx)];
 }
 
 // ---------------------------------------------------------------------------
@@ -174,8 +182,7 @@ template <int L, int ROWS, int T>
 __global__ void __launch_bounds__(T) k_rows_c2r_r2c(PlaneSet Tin, PlaneSet band, PlaneSet Uout, ThreshSpec th,
                                                      const float2* __restrict__ tw) {
   constexpr int N = L / 2;
-  constexpr int BS = N + 1;
-  __shared__ float2 buf[ROWS * BS];
+  __shared__ float2 buf[lds_size<ROWS * N, RPAD>()];
   const int tid = threadIdx.x;
   const int p = blockIdx.y, b = blockIdx.z;
   const int row0 = blockIdx.x * ROWS;
@@ -185,37 +192,38 @@ __global__ void __launch_bounds__(T) k_rows_c2r_r2c(PlaneSet Tin, PlaneSet band,
   float2* udst = reinterpret_cast<float2*>(plane_ptr(Uout, p, pe, b)) + (size_t)row0 * N;
   const float t = th.mode != 0 ? thresh_of(th, p, b) : -1.0f;
 #pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) buf[(idx / N) * BS + idx % N] = src[idx];
+  for (int idx = tid; idx < ROWS * N; idx += T) buf[lds_ix<RPAD>(idx)] = src[idx];
   __syncthreads();
-  rows_map<ROWS, N, BS, T>(buf, [&](const float2* row, int k) { return c2r_pre<N>(row, k, tw); });
-  fft_lds<N, 2, +1, ROWS, T, false>(buf, BS, 1, tw, tid);
+  rows_map<ROWS, N, T>(buf, [&](int e0, int k) { return c2r_pre<N>(buf, e0, k, tw); });
+  fft_lds<N, 2, +1, ROWS, T, false, RPAD>(buf, N, 1, tw, tid);
 #pragma unroll
   for (int idx = tid; idx < ROWS * N; idx += T) {
-    float2& v = buf[(idx / N) * BS + idx % N];
+    float2& v = buf[lds_ix<RPAD>(idx)];
     const float2 c = v;
     cdst[idx] = c;
     if (th.mode != 0) v = make_float2(hard_keep(c.x, t), hard_keep(c.y, t));
   }
   __syncthreads();
-  fft_lds<N, 2, -1, ROWS, T, false>(buf, BS, 1, tw, tid);
+  fft_lds<N, 2, -1, ROWS, T, false, RPAD>(buf, N, 1, tw, tid);
 #pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) udst[idx] = r2c_post<N>(buf + (idx / N) * BS, idx % N, tw);
+  for (int idx = tid; idx < ROWS * N; idx += T) udst[idx] = r2c_post<N>(buf, idx - idx % N, idx % N, tw);
 }
 
 // ---------------------------------------------------------------------------
-// Column-tile helpers: tile of L rows x CT complex columns in LDS, buf[u*CT+c].
+// Column-tile helpers: tile of L rows x CT complex columns in LDS, element
+// (u, c) at slot lds_ix<CPAD>(u*CT + c).
 template <int L, int CT, int T>
 __device__ __forceinline__ void tile_load(float2* buf, const float2* __restrict__ src) {
   constexpr int N = L / 2;
 #pragma unroll
-  for (int idx = threadIdx.x; idx < L * CT; idx += T) buf[idx] = src[(size_t)(idx / CT) * N + idx % CT];
+  for (int idx = threadIdx.x; idx < L * CT; idx += T) buf[lds_ix<CPAD>(idx)] = src[(size_t)(idx / CT) * N + idx % CT];
 }
 
 template <int L, int CT, int T>
 __device__ __forceinline__ void tile_store(float2* __restrict__ dst, const float2* buf) {
   constexpr int N = L / 2;
 #pragma unroll
-  for (int idx = threadIdx.x; idx < L * CT; idx += T) dst[(size_t)(idx / CT) * N + idx % CT] = buf[idx];
+  for (int idx = threadIdx.x; idx < L * CT; idx += T) dst[(size_t)(idx / CT) * N + idx % CT] = buf[lds_ix<CPAD>(idx)];
 }
 
 // Filter a transformed column tile by band p's spectrum (CONJ: by its
@@ -225,10 +233,10 @@ __device__ __forceinline__ float2 filt(const float2* buf, int idx, int c0, const
                                        const float2* __restrict__ Hb) {
   constexpr int N = L / 2;
   const int u = idx / CT, c = idx % CT;
-  const float2 P = buf[idx];
+  const float2 P = buf[lds_ix<CPAD>(idx)];
   const float2 h = H[(size_t)u * N + c];
   if (c0 + c == 0) {
-    const float2 Pm = cconj(buf[((L - u) & (L - 1)) * CT]);
+    const float2 Pm = cconj(buf[lds_ix<CPAD>(((L - u) & (L - 1)) * CT)]);
     return CONJ ? cadd(cmulc(P, h), cmulc(Pm, Hb[u])) : cadd(cmul(h, P), cmul(Hb[u], Pm));
   }
   return CONJ ? cmulc(P, h) : cmul(h, P);
@@ -242,14 +250,14 @@ __global__ void __launch_bounds__(T) k_cols_dec(const float2* __restrict__ Y1, l
                                                  BandMap map, const float2* __restrict__ tw) {
   constexpr int N = L / 2;
   constexpr int PER = (L * CT + T - 1) / T;
-  __shared__ float2 buf[L * CT];
+  __shared__ float2 buf[lds_size<L * CT, CPAD>()];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * CT;
   const int p = blockIdx.y, b = blockIdx.z;
   const int j = map.scale_of[p];
   tile_load<L, CT, T>(buf, Y1 + (size_t)b * y1_stride + (size_t)j * L * N + c0);
   __syncthreads();
-  fft_lds<L, 1, -1, CT, T, true>(buf, 1, CT, tw, tid);
+  fft_lds<L, 1, -1, CT, T, true, CPAD>(buf, 1, CT, tw, tid);
   const float2* H = Hmain + (size_t)p * L * N + c0;
   const float2* Hb = Hbeta + (size_t)p * L;
   float2 q[PER];
@@ -262,10 +270,10 @@ __global__ void __launch_bounds__(T) k_cols_dec(const float2* __restrict__ Y1, l
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     const int idx = tid + e * T;
-    if ((L * CT) % T == 0 || idx < L * CT) buf[idx] = q[e];
+    if ((L * CT) % T == 0 || idx < L * CT) buf[lds_ix<CPAD>(idx)] = q[e];
   }
   __syncthreads();
-  fft_lds<L, 1, +1, CT, T, true>(buf, 1, CT, tw, tid);
+  fft_lds<L, 1, +1, CT, T, true, CPAD>(buf, 1, CT, tw, tid);
   tile_store<L, CT, T>(Tout + (size_t)b * t_stride + (size_t)p * L * N + c0, buf);
 }
 
@@ -279,7 +287,7 @@ __global__ void __launch_bounds__(T) k_cols_rec(const float2* __restrict__ U, lo
                                                  BandMap map, const float2* __restrict__ tw) {
   constexpr int N = L / 2;
   constexpr int PER = (L * CT + T - 1) / T;
-  __shared__ float2 buf[L * CT];
+  __shared__ float2 buf[lds_size<L * CT, CPAD>()];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * CT;
   const int g = blockIdx.y, b = blockIdx.z;
@@ -290,7 +298,7 @@ __global__ void __launch_bounds__(T) k_cols_rec(const float2* __restrict__ U, lo
   for (int p = p0; p < p1; ++p) {
     tile_load<L, CT, T>(buf, U + (size_t)b * u_stride + (size_t)p * L * N + c0);
     __syncthreads();
-    fft_lds<L, 1, -1, CT, T, true>(buf, 1, CT, tw, tid);
+    fft_lds<L, 1, -1, CT, T, true, CPAD>(buf, 1, CT, tw, tid);
     const float2* H = Hmain + (size_t)p * L * N + c0;
     const float2* Hb = Hbeta + (size_t)p * L;
 #pragma unroll
@@ -303,10 +311,10 @@ __global__ void __launch_bounds__(T) k_cols_rec(const float2* __restrict__ U, lo
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     const int idx = tid + e * T;
-    if ((L * CT) % T == 0 || idx < L * CT) buf[idx] = acc[e];
+    if ((L * CT) % T == 0 || idx < L * CT) buf[lds_ix<CPAD>(idx)] = acc[e];
   }
   __syncthreads();
-  fft_lds<L, 1, +1, CT, T, true>(buf, 1, CT, tw, tid);
+  fft_lds<L, 1, +1, CT, T, true, CPAD>(buf, 1, CT, tw, tid);
   tile_store<L, CT, T>(Q + (size_t)b * q_stride + (size_t)g * L * N + c0, buf);
 }
 
