@@ -1,0 +1,162 @@
+// Register-resident FFT building blocks for gfx950.
+//
+// A transform of N = R1 * R2 points is done in two passes with one LDS
+// transpose between them (the four-step split, with the inner twiddles):
+//   pass A  thread n1 (0..R1-1) holds x[n1 + R1*n2], n2 = 0..R2-1;
+//           DFT of length R2 over n2, then multiply by w_N^(n1*k2)
+//           -> y[n1][k2]
+//   pass B  thread k2 (0..R2-1) holds y[n1][k2], n1 = 0..R1-1;
+//           DFT of length R1 over n1 -> X[k2 + R2*k1], k1 = 0..R1-1
+// After pass B thread k2 holds X at k2 + R2*k1: exactly the pass-A layout of
+// a transform split as (R2, R1).  A forward transform, a pointwise product and
+// an inverse transform therefore chain in registers.
+//
+// The short DFTs (2, 4, 8, 16, 32) are fully unrolled with compile-time
+// twiddles (computed below by constexpr series in FP64, rounded once to FP32);
+// the pass twiddles w_N^m come from a table computed in FP64 on the host.
+#pragma once
+
+#include "fft_lds.h"
+
+namespace shx {
+
+// ---- compile-time twiddles ----------------------------------------------------
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double ce_sin(double x) {
+  double t = x, s = x;
+  for (int k = 1; k < 24; ++k) {
+    t *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    s += t;
+  }
+  return s;
+}
+
+constexpr double ce_cos(double x) {
+  double t = 1.0, s = 1.0;
+  for (int k = 1; k < 24; ++k) {
+    t *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    s += t;
+  }
+  return s;
+}
+
+// exp(-2 pi i k / R) for k < R, angles reduced to [-pi, pi]
+template <int R>
+struct CTw {
+  float re[R];
+  float im[R];
+  constexpr CTw() : re(), im() {
+    for (int k = 0; k < R; ++k) {
+      double a = -2.0 * kPi * k / R;
+      if (a < -kPi) a += 2.0 * kPi;
+      re[k] = (float)ce_cos(a);
+      im[k] = (float)ce_sin(a);
+    }
+  }
+};
+
+template <int R>
+constexpr CTw<R> kCTw{};
+
+// a * exp(DIR * 2 pi i m / R) for a compile-time (after unrolling) m
+template <int R, int DIR>
+__device__ __forceinline__ float2 ctw_mul(float2 a, int m) {
+  m %= R;
+  if (m == 0) return a;
+  if ((4 * m) % R == 0) {  // quarter turns: exact
+    const int q = (4 * m) / R;
+    if (q == 2) return make_float2(-a.x, -a.y);
+    // exp(DIR*i*pi/2 * q): q=1 -> DIR*i, q=3 -> -DIR*i
+    return (q == 1) == (DIR > 0) ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+  }
+  const float wr = kCTw<R>.re[m];
+  const float wi = DIR < 0 ? kCTw<R>.im[m] : -kCTw<R>.im[m];
+  return make_float2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* v);
+
+// length R = A*B in registers: x[a + A*b] -> X[kb + B*ka]
+template <int A, int B, int DIR>
+__device__ __forceinline__ void dft_split(float2* v) {
+  constexpr int R = A * B;
+  float2 t[A][B];
+#pragma unroll
+  for (int a = 0; a < A; ++a) {
+#pragma unroll
+    for (int b = 0; b < B; ++b) t[a][b] = v[a + A * b];
+    dft<B, DIR>(t[a]);
+#pragma unroll
+    for (int kb = 1; kb < B; ++kb) t[a][kb] = ctw_mul<R, DIR>(t[a][kb], a * kb);
+  }
+#pragma unroll
+  for (int kb = 0; kb < B; ++kb) {
+    float2 u[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) u[a] = t[a][kb];
+    dft<A, DIR>(u);
+#pragma unroll
+    for (int ka = 0; ka < A; ++ka) v[kb + B * ka] = u[ka];
+  }
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* v) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
+    dft2<DIR>(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    dft4<DIR>(v[0], v[1], v[2], v[3]);
+  } else if constexpr (R == 8) {
+    dft8<DIR>(v);
+  } else if constexpr (R == 16) {
+    dft_split<4, 4, DIR>(v);
+  } else if constexpr (R == 32) {
+    dft_split<4, 8, DIR>(v);
+  } else {
+    static_assert(R == 64, "unsupported register DFT length");
+    dft_split<8, 8, DIR>(v);
+  }
+}
+
+// N = R1 * R2 split used by the two-pass transforms (R1 <= R2 <= 32).
+template <int N>
+struct Split;
+template <> struct Split<16> { static constexpr int R1 = 4, R2 = 4; };
+template <> struct Split<32> { static constexpr int R1 = 4, R2 = 8; };
+template <> struct Split<64> { static constexpr int R1 = 8, R2 = 8; };
+template <> struct Split<128> { static constexpr int R1 = 8, R2 = 16; };
+template <> struct Split<256> { static constexpr int R1 = 16, R2 = 16; };
+template <> struct Split<512> { static constexpr int R1 = 16, R2 = 32; };
+template <> struct Split<1024> { static constexpr int R1 = 32, R2 = 32; };
+
+// Pass A on registers v[RB] (thread index a in [0, RA)): DFT of length RB,
+// then the pass twiddles w_N^(a*kb), N = RA*RB, from the table tw
+// (tw[m*TWS] = exp(-2 pi i m / N)).
+// For RB > 8 only w^r (r < 8) and w^(8q) are loaded and w^(8q + r) is one
+// product of the two, which keeps the twiddle registers few.
+template <int RA, int RB, int DIR, int TWS>
+__device__ __forceinline__ void passA(float2 (&v)[RB], int a, const float2* __restrict__ tw) {
+  dft<RB, DIR>(v);
+  constexpr int LO = RB < 8 ? RB : 8;
+  float2 lo[LO];
+#pragma unroll
+  for (int r = 1; r < LO; ++r) {
+    lo[r] = tw[a * r * TWS];
+    if constexpr (DIR > 0) lo[r].y = -lo[r].y;
+  }
+#pragma unroll
+  for (int r = 1; r < LO; ++r) v[r] = cmul(v[r], lo[r]);
+#pragma unroll
+  for (int q = 1; q < RB / LO; ++q) {
+    float2 hi = tw[a * q * LO * TWS];
+    if constexpr (DIR > 0) hi.y = -hi.y;
+    v[q * LO] = cmul(v[q * LO], hi);
+#pragma unroll
+    for (int r = 1; r < LO; ++r) v[q * LO + r] = cmul(v[q * LO + r], cmul(hi, lo[r]));
+  }
+}
+
+}  // namespace shx

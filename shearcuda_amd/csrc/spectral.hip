@@ -20,12 +20,15 @@
 //   rows_r2c  : U_k = RFFT_rows(thresh(c_k))
 //   cols_rec  : V = IFFT_cols(sum_k conj(H_k) . FFT_cols(U_k))
 //   rows_c2r  : y_j = IRFFT_rows(V)
+// Every transform is a two-pass register FFT (fft_reg.h): the first pass reads
+// global memory, the second writes it, with one padded LDS transpose between
+// them; forward/inverse pairs chain in registers.
 // H_k holds the normalised filter spectrum times 1/L^2 (the two unnormalised
 // inverse passes each scale by L).  For the packed column, H_k(u,0) and
 // H_k(u,N) act through alpha = (H(u,0)+H(u,N))/2 (stored in slot 0) and
 // beta = (H(u,0)-H(u,N))/2 (stored separately):
 //   Q(u) = alpha(u) P(u) + beta(u) conj(P(-u)).
-#include "fft_lds.h"
+#include "fft_reg.h"
 #include "spectral.h"
 
 namespace shx {
@@ -52,290 +55,387 @@ __device__ __forceinline__ float thresh_of(const ThreshSpec& t, int p, int b) {
   return t.mode == 1 ? t.arg[p] : t.dev[(size_t)b * t.dev_stride + t.dev_off + p];
 }
 
-// ---------------------------------------------------------------------------
-// Row-pass building blocks on a block of ROWS rows held in LDS: element k of
-// row r (N = L/2 complex per row) is linear element r*N + k, at slot
-// lds_ix<RPAD>(r*N + k).
-
-constexpr int RPAD = 5;  // row buffers: one pad slot per 32 elements
-constexpr int CPAD = 4;  // column tiles: one pad slot per 16 elements
+__device__ __forceinline__ float2 shfl2(float2 v, int src) {
+  return make_float2(__shfl(v.x, src), __shfl(v.y, src));
+}
 
 template <int E, int PADSH>
 constexpr int lds_size() {
   return E + (E >> PADSH) + 1;
 }
 
-// Packed half spectrum of a real row from the FFT of its (even, odd) pairs;
-// the row starts at linear element e0 of buf.
-template <int N>
-__device__ __forceinline__ float2 r2c_post(const float2* buf, int e0, int k, const float2* __restrict__ tw) {
-  const float2 z = buf[lds_ix<RPAD>(e0 + k)];
-  if (k == 0) return make_float2(z.x + z.y, z.x - z.y);
-  const float2 zc = cconj(buf[lds_ix<RPAD>(e0 + N - k)]);
+// Packed half spectrum P[k] of a real row from X = FFT(even + i odd) at k and
+// N-k (k > 0): X_real[k] = (X[k] + X*[N-k])/2 - i w_L^k (X[k] - X*[N-k])/2.
+__device__ __forceinline__ float2 r2c_post(float2 z, float2 zm, float2 w) {
+  const float2 zc = cconj(zm);
   const float2 a = cscale(cadd(z, zc), 0.5f);
   const float2 b = cscale(csub(z, zc), 0.5f);
-  const float2 wb = cmul(tw[k], b);  // X = a - i w^k b
+  const float2 wb = cmul(w, b);
   return make_float2(a.x + wb.y, a.y - wb.x);
 }
 
-// FFT input (even + i odd, times L) of the inverse of a packed half spectrum.
-template <int N>
-__device__ __forceinline__ float2 c2r_pre(const float2* buf, int e0, int k, const float2* __restrict__ tw) {
-  const float2 x = buf[lds_ix<RPAD>(e0 + k)];
-  if (k == 0) return make_float2(x.x + x.y, x.x - x.y);
-  const float2 xc = cconj(buf[lds_ix<RPAD>(e0 + N - k)]);
+// Inverse: FFT input (even + i odd, times L) from P at k and N-k (k > 0).
+__device__ __forceinline__ float2 c2r_pre(float2 x, float2 xm, float2 w) {
+  const float2 xc = cconj(xm);
   const float2 fe = cadd(x, xc);
-  const float2 fo = cmulc(csub(x, xc), tw[k]);  // * w^{-k}
-  return make_float2(fe.x - fo.y, fe.y + fo.x);  // fe + i fo
-}
-
-// In-place map of every row element through f(e0, k) (all reads, barrier,
-// all writes).
-template <int ROWS, int N, int T, class F>
-__device__ __forceinline__ void rows_map(float2* buf, F f) {
-  constexpr int PER = (ROWS * N + T - 1) / T;
-  float2 z[PER];
-#pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = threadIdx.x + e * T;
-    if ((ROWS * N) % T == 0 || idx < ROWS * N) z[e] = f(idx - idx % N, idx % N);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = threadIdx.x + e * T;
-    if ((ROWS * N) % T == 0 || idx < ROWS * N) buf[lds_ix<RPAD>(idx)] = z[e];
-  }
-  __syncthreads();
+  const float2 fo = cmulc(csub(x, xc), w);  // * w^{-k}
+  return make_float2(fe.x - fo.y, fe.y + fo.x);
 }
 
 // ---------------------------------------------------------------------------
+// Row passes.  Rows are transforms of N = L/2 complex points (the real row
+// as (even, odd) pairs), split N = R1 x R2.  A block holds ROWS rows and
+// T = ROWS * R2 threads; pass A uses ROWS * R1 of them.
+template <int L>
+struct RowCfg {
+  static constexpr int N = L / 2;
+  static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2;
+  static constexpr int ROWS = (256 / R2) < L ? (256 / R2) : L;
+  static constexpr int T = ROWS * R2;
+  static constexpr int PADSH = ilog2(R1);
+  static constexpr int LDS = lds_size<ROWS * N, PADSH>();
+};
+
+// Forward real-row FFT from pass-A registers (thread (r, n1) holds
+// z[n1 + R1 n2]) to the packed half spectrum, stored to dst (row r).
+// All threads of the block must call it.
+template <int L, int R1, int R2, int PADSH>
+__device__ __forceinline__ void row_fwd_from_A(float2 (&v)[R2], bool validA, int r, int n1, float2* buf,
+                                               float2* __restrict__ dst_rows, bool validB, int rB, int k2,
+                                               const float2* __restrict__ tw) {
+  constexpr int N = L / 2;
+  if (validA) {
+    passA<R1, R2, -1, 2>(v, n1, tw);
+#pragma unroll
+    for (int k = 0; k < R2; ++k) buf[lds_ix<PADSH>(r * N + n1 + R1 * k)] = v[k];
+  }
+  __syncthreads();
+  float2 u[R1];
+#pragma unroll
+  for (int a = 0; a < R1; ++a) u[a] = make_float2(0.f, 0.f);
+  if (validB) {
+#pragma unroll
+    for (int a = 0; a < R1; ++a) u[a] = buf[lds_ix<PADSH>(rB * N + a + R1 * k2)];
+    dft<R1, -1>(u);
+  }
+  const int lane = threadIdx.x & 63;
+  const int src = (lane & ~(R2 - 1)) | ((R2 - k2) & (R2 - 1));
+  float2* drow = dst_rows + (size_t)rB * N;
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const float2 sh = shfl2(u[R1 - 1 - k1], src);
+    if (validB) {
+      const int k = k2 + R2 * k1;
+      const float2 zm = (k2 == 0) ? u[(R1 - k1) & (R1 - 1)] : sh;
+      drow[k] = (k == 0) ? make_float2(u[0].x + u[0].y, u[0].x - u[0].y) : r2c_post(u[k1], zm, tw[k]);
+    }
+  }
+}
+
+// Inverse of a packed row: thread (r, n1) gathers P[k], P[N-k] for
+// k = n1 + R1 n2 (via ld) and returns, in pass-B layout, z[k2 + R2 k1]
+// (= L * (x[2k], x[2k+1])).  All threads of the block must call it.
+template <int L, int R1, int R2, int PADSH, class LD>
+__device__ __forceinline__ void row_inv_to_B(float2 (&u)[R1], bool validA, int r, int n1, float2* buf, bool validB,
+                                             int rB, int k2, const float2* __restrict__ tw, LD ld) {
+  constexpr int N = L / 2;
+  if (validA) {
+    float2 v[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int k = n1 + R1 * n2;
+      const float2 x = ld(r, k);
+      const float2 xm = ld(r, (N - k) & (N - 1));
+      v[n2] = (k == 0) ? make_float2(x.x + x.y, x.x - x.y) : c2r_pre(x, xm, tw[k]);
+    }
+    passA<R1, R2, +1, 2>(v, n1, tw);
+#pragma unroll
+    for (int k = 0; k < R2; ++k) buf[lds_ix<PADSH>(r * N + n1 + R1 * k)] = v[k];
+  }
+  __syncthreads();
+  if (validB) {
+#pragma unroll
+    for (int a = 0; a < R1; ++a) u[a] = buf[lds_ix<PADSH>(rB * N + a + R1 * k2)];
+    dft<R1, +1>(u);
+  }
+}
+
 // Row real-to-complex FFT (packed output), optional hard threshold on load.
 // grid: (L / ROWS, nplanes, nimg).
-template <int L, int ROWS, int T>
-__global__ void __launch_bounds__(T) k_rows_r2c(PlaneSet in, PlaneSet out, ThreshSpec th,
-                                                 const float2* __restrict__ tw) {
-  constexpr int N = L / 2;
-  __shared__ float2 buf[lds_size<ROWS * N, RPAD>()];
+template <int L>
+__global__ void __launch_bounds__(RowCfg<L>::T) k_rows_r2c(PlaneSet in, PlaneSet out, ThreshSpec th,
+                                                            const float2* __restrict__ tw) {
+  using C = RowCfg<L>;
+  constexpr int N = C::N, R1 = C::R1, R2 = C::R2;
+  __shared__ float2 buf[C::LDS];
   const int tid = threadIdx.x;
   const int p = blockIdx.y, b = blockIdx.z;
-  const int row0 = blockIdx.x * ROWS;
+  const int row0 = blockIdx.x * C::ROWS;
   const size_t pe = (size_t)L * L;
   const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe, b)) + (size_t)row0 * N;
   float2* dst = reinterpret_cast<float2*>(plane_ptr(out, p, pe, b)) + (size_t)row0 * N;
   const bool use_th = th.mode != 0;
   const float t = use_th ? thresh_of(th, p, b) : 0.0f;
+  const int r = tid / R1, n1 = tid % R1;
+  const bool validA = r < C::ROWS;
+  float2 v[R2];
+  if (validA) {
 #pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) {
-    float2 v = src[idx];
-    if (use_th) {
-      v.x = hard_keep(v.x, t);
-      v.y = hard_keep(v.y, t);
+    for (int n2 = 0; n2 < R2; ++n2) {
+      float2 x = src[(size_t)r * N + n1 + R1 * n2];
+      if (use_th) x = make_float2(hard_keep(x.x, t), hard_keep(x.y, t));
+      v[n2] = x;
     }
-    buf[lds_ix<RPAD>(idx)] = v;
   }
-  __syncthreads();
-  fft_lds<N, 2, -1, ROWS, T, false, RPAD>(buf, N, 1, tw, tid);
-#pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) dst[idx] = r2c_post<N>(buf, idx - idx % N, idx % N, tw);
+  row_fwd_from_A<L, R1, R2, C::PADSH>(v, validA, r, n1, buf, dst, true, tid / R2, tid % R2, tw);
 }
 
-// ---------------------------------------------------------------------------
 // Row complex-to-real inverse FFT of a packed spectrum (unnormalised: x L),
 // optionally of the sum of several input planes.
-template <int L, int ROWS, int T, bool SUM>
-__global__ void __launch_bounds__(T) k_rows_c2r(PlaneSet in, PlaneSet out, SumTable sum,
-                                                 const float2* __restrict__ tw) {
-  constexpr int N = L / 2;
-  __shared__ float2 buf[lds_size<ROWS * N, RPAD>()];
+template <int L, bool SUM>
+__global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r(PlaneSet in, PlaneSet out, SumTable sum,
+                                                            const float2* __restrict__ tw) {
+  using C = RowCfg<L>;
+  constexpr int N = C::N, R1 = C::R1, R2 = C::R2;
+  __shared__ float2 buf[C::LDS];
   const int tid = threadIdx.x;
   const int p = blockIdx.y, b = blockIdx.z;
-  const int row0 = blockIdx.x * ROWS;
+  const int row0 = blockIdx.x * C::ROWS;
   const size_t pe = (size_t)L * L;
   float2* dst = reinterpret_cast<float2*>(plane_ptr(out, p, pe, b)) + (size_t)row0 * N;
+  const int r = tid / R1, n1 = tid % R1, rB = tid / R2, k2 = tid % R2;
+  float2 u[R1];
   if constexpr (SUM) {
     const int q0 = sum.first[p], q1 = sum.first[p + 1];
-#pragma unroll
-    for (int idx = tid; idx < ROWS * N; idx += T) {
+    auto ld = [&](int rr, int k) {
       float2 a = make_float2(0.f, 0.f);
       for (int q = q0; q < q1; ++q)
-        a = cadd(a, reinterpret_cast<const float2*>(plane_ptr(in, q, pe, b))[(size_t)row0 * N + idx]);
-      buf[lds_ix<RPAD>(idx)] = a;
-    }
+        a = cadd(a, reinterpret_cast<const float2*>(plane_ptr(in, q, pe, b))[(size_t)(row0 + rr) * N + k]);
+      return a;
+    };
+    row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, ld);
   } else {
     const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe, b)) + (size_t)row0 * N;
-#pragma unroll
-    for (int idx = tid; idx < ROWS * N; idx += T) buf[lds_ix<RPAD>(idx)] = src[idx];
+    auto ld = [&](int rr, int k) { return src[(size_t)rr * N + k]; };
+    row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, ld);
   }
-  __syncthreads();
-  rows_map<ROWS, N, T>(buf, [&](int e0, int k) { return c2r_pre<N>(buf, e0, k, tw); });
-  fft_lds<N, 2, +1, ROWS, T, false, RPAD>(buf, N, 1, tw, tid);
 #pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) dst[idx] = buf[lds_ix<RPAD>(idx)];
+  for (int k1 = 0; k1 < R1; ++k1) dst[(size_t)rB * N + k2 + R2 * k1] = u[k1];
 }
 
-// ---------------------------------------------------------------------------
 // Denoise fusion of the decomposition's last row pass with the
 // reconstruction's first: c = C2R(T) is stored (the coefficients the caller
-// receives), then U = R2C(keep(c, th)) is formed from the same rows in LDS.
-template <int L, int ROWS, int T>
-__global__ void __launch_bounds__(T) k_rows_c2r_r2c(PlaneSet Tin, PlaneSet band, PlaneSet Uout, ThreshSpec th,
-                                                     const float2* __restrict__ tw) {
-  constexpr int N = L / 2;
-  __shared__ float2 buf[lds_size<ROWS * N, RPAD>()];
+// receives), then U = R2C(keep(c, th)) continues from the same registers with
+// the split swapped (R2, R1).
+template <int L>
+__global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, PlaneSet band, PlaneSet Uout,
+                                                                ThreshSpec th, const float2* __restrict__ tw) {
+  using C = RowCfg<L>;
+  constexpr int N = C::N, R1 = C::R1, R2 = C::R2;
+  __shared__ float2 buf[C::LDS];
   const int tid = threadIdx.x;
   const int p = blockIdx.y, b = blockIdx.z;
-  const int row0 = blockIdx.x * ROWS;
+  const int row0 = blockIdx.x * C::ROWS;
   const size_t pe = (size_t)L * L;
   const float2* src = reinterpret_cast<const float2*>(plane_ptr(Tin, p, pe, b)) + (size_t)row0 * N;
   float2* cdst = reinterpret_cast<float2*>(plane_ptr(band, p, pe, b)) + (size_t)row0 * N;
   float2* udst = reinterpret_cast<float2*>(plane_ptr(Uout, p, pe, b)) + (size_t)row0 * N;
-  const float t = th.mode != 0 ? thresh_of(th, p, b) : -1.0f;
+  const bool use_th = th.mode != 0;
+  const float t = use_th ? thresh_of(th, p, b) : 0.0f;
+  const int r = tid / R1, n1 = tid % R1, rB = tid / R2, k2 = tid % R2;
+  float2 u[R1];
+  auto ld = [&](int rr, int k) { return src[(size_t)rr * N + k]; };
+  row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, ld);
 #pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) buf[lds_ix<RPAD>(idx)] = src[idx];
-  __syncthreads();
-  rows_map<ROWS, N, T>(buf, [&](int e0, int k) { return c2r_pre<N>(buf, e0, k, tw); });
-  fft_lds<N, 2, +1, ROWS, T, false, RPAD>(buf, N, 1, tw, tid);
-#pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) {
-    float2& v = buf[lds_ix<RPAD>(idx)];
-    const float2 c = v;
-    cdst[idx] = c;
-    if (th.mode != 0) v = make_float2(hard_keep(c.x, t), hard_keep(c.y, t));
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const float2 c = u[k1];
+    cdst[(size_t)rB * N + k2 + R2 * k1] = c;
+    u[k1] = use_th ? make_float2(hard_keep(c.x, t), hard_keep(c.y, t)) : c;
   }
-  __syncthreads();
-  fft_lds<N, 2, -1, ROWS, T, false, RPAD>(buf, N, 1, tw, tid);
-#pragma unroll
-  for (int idx = tid; idx < ROWS * N; idx += T) udst[idx] = r2c_post<N>(buf, idx - idx % N, idx % N, tw);
+  __syncthreads();  // pass-B reads of buf are complete
+  // forward transform split (R2, R1): thread k2 is pass-A index n1' = k2
+  row_fwd_from_A<L, R2, R1, C::PADSH>(u, true, rB, k2, buf, udst, tid < C::ROWS * R1, tid / R1, tid % R1, tw);
 }
 
 // ---------------------------------------------------------------------------
-// Column-tile helpers: tile of L rows x CT complex columns in LDS, element
-// (u, c) at slot lds_ix<CPAD>(u*CT + c).
-template <int L, int CT, int T>
-__device__ __forceinline__ void tile_load(float2* buf, const float2* __restrict__ src) {
-  constexpr int N = L / 2;
-#pragma unroll
-  for (int idx = threadIdx.x; idx < L * CT; idx += T) buf[lds_ix<CPAD>(idx)] = src[(size_t)(idx / CT) * N + idx % CT];
-}
+// Column passes: a tile of CT columns of a packed spectrum (L rows, row
+// stride L/2 complex); each column is a transform of N = L points split
+// R1 x R2.  T = CT * R2 threads; element (u, c) of the LDS tile is linear
+// element u*CT + c.
+template <int L>
+struct ColCfg {
+  static constexpr int R1 = Split<L>::R1, R2 = Split<L>::R2;
+  static constexpr int CT = (L >= 1024) ? 8 : ((L / 2) < 16 ? (L / 2) : 16);
+  static constexpr int T = CT * R2;
+  static constexpr int PADSH = 7;
+  static constexpr int LDS = lds_size<L * CT, PADSH>();
+};
 
-template <int L, int CT, int T>
-__device__ __forceinline__ void tile_store(float2* __restrict__ dst, const float2* buf) {
-  constexpr int N = L / 2;
+// Forward column transform: from global (row stride L/2) to pass-B layout
+// registers u[k1] = X[k2 + R2 k1] of thread (c, k2).
+template <int L>
+__device__ __forceinline__ void col_fwd_to_B(float2 (&u)[ColCfg<L>::R1], const float2* __restrict__ src,
+                                             float2* buf, const float2* __restrict__ tw) {
+  using C = ColCfg<L>;
+  constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
+  const int tid = threadIdx.x;
+  const int c = tid % CT, n1 = tid / CT, k2 = tid / CT;
+  if (n1 < R1) {
+    float2 v[R2];
 #pragma unroll
-  for (int idx = threadIdx.x; idx < L * CT; idx += T) dst[(size_t)(idx / CT) * N + idx % CT] = buf[lds_ix<CPAD>(idx)];
-}
-
-// Filter a transformed column tile by band p's spectrum (CONJ: by its
-// conjugate).  Slot 0 of tile c0 == 0 is the packed (DC, Nyquist) column.
-template <int L, int CT, bool CONJ>
-__device__ __forceinline__ float2 filt(const float2* buf, int idx, int c0, const float2* __restrict__ H,
-                                       const float2* __restrict__ Hb) {
-  constexpr int N = L / 2;
-  const int u = idx / CT, c = idx % CT;
-  const float2 P = buf[lds_ix<CPAD>(idx)];
-  const float2 h = H[(size_t)u * N + c];
-  if (c0 + c == 0) {
-    const float2 Pm = cconj(buf[lds_ix<CPAD>(((L - u) & (L - 1)) * CT)]);
-    return CONJ ? cadd(cmulc(P, h), cmulc(Pm, Hb[u])) : cadd(cmul(h, P), cmul(Hb[u], Pm));
+    for (int n2 = 0; n2 < R2; ++n2) v[n2] = src[(size_t)(n1 + R1 * n2) * NH + c];
+    passA<R1, R2, -1, 1>(v, n1, tw);
+#pragma unroll
+    for (int k = 0; k < R2; ++k) buf[lds_ix<C::PADSH>((n1 + R1 * k) * CT + c)] = v[k];
   }
-  return CONJ ? cmulc(P, h) : cmul(h, P);
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < R1; ++a) u[a] = buf[lds_ix<C::PADSH>((a + R1 * k2) * CT + c)];
+  dft<R1, -1>(u);
+}
+
+// Inverse column transform from pass-B layout registers (thread (c, k2)
+// holding Q[k2 + R2 k1]) to global (row stride L/2).  Starts with a barrier
+// so buf may still be in use by the caller.
+template <int L>
+__device__ __forceinline__ void col_inv_from_B(float2 (&q)[ColCfg<L>::R1], float2* __restrict__ dst, float2* buf,
+                                               const float2* __restrict__ tw) {
+  using C = ColCfg<L>;
+  constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
+  const int tid = threadIdx.x;
+  const int c = tid % CT, k2 = tid / CT;
+  passA<R2, R1, +1, 1>(q, k2, tw);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < R1; ++k) buf[lds_ix<C::PADSH>((k2 + R2 * k) * CT + c)] = q[k];
+  __syncthreads();
+  const int kb = tid / CT;
+  if (kb < R1) {
+    float2 w[R2];
+#pragma unroll
+    for (int a = 0; a < R2; ++a) w[a] = buf[lds_ix<C::PADSH>((a + R2 * kb) * CT + c)];
+    dft<R2, +1>(w);
+#pragma unroll
+    for (int k1 = 0; k1 < R2; ++k1) dst[(size_t)(kb + R1 * k1) * NH + c] = w[k1];
+  }
+}
+
+// Filter pass-B registers by band spectrum H (CONJ: its conjugate).  In the
+// block holding column 0 the packed (DC, Nyquist) column needs X(-u): it is
+// exchanged through LDS (block-uniform branch).
+template <int L, bool CONJ, bool ACC>
+__device__ __forceinline__ void col_filter(float2 (&u)[ColCfg<L>::R1], float2 (&out)[ColCfg<L>::R1], int c0,
+                                           const float2* __restrict__ H, const float2* __restrict__ Hb, float2* buf) {
+  using C = ColCfg<L>;
+  constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
+  const int tid = threadIdx.x;
+  const int c = tid % CT, k2 = tid / CT;
+  float2 pm[R1];
+  if (c0 == 0) {
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) buf[lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c)] = u[k1];
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int uu = k2 + R2 * k1;
+      pm[k1] = cconj(buf[lds_ix<C::PADSH>(((L - uu) & (L - 1)) * CT)]);
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const int uu = k2 + R2 * k1;
+    const float2 h = H[(size_t)uu * NH + c];
+    float2 q;
+    if (c0 == 0 && c == 0) {
+      const float2 hb = Hb[uu];
+      q = CONJ ? cadd(cmulc(u[k1], h), cmulc(pm[k1], hb)) : cadd(cmul(h, u[k1]), cmul(hb, pm[k1]));
+    } else {
+      q = CONJ ? cmulc(u[k1], h) : cmul(h, u[k1]);
+    }
+    out[k1] = ACC ? cadd(out[k1], q) : q;
+  }
 }
 
 // Column pass of the decomposition: one block per (column tile, band, image).
-template <int L, int CT, int T>
-__global__ void __launch_bounds__(T) k_cols_dec(const float2* __restrict__ Y1, long long y1_stride,
-                                                 float2* __restrict__ Tout, long long t_stride,
-                                                 const float2* __restrict__ Hmain, const float2* __restrict__ Hbeta,
-                                                 BandMap map, const float2* __restrict__ tw) {
-  constexpr int N = L / 2;
-  constexpr int PER = (L * CT + T - 1) / T;
-  __shared__ float2 buf[lds_size<L * CT, CPAD>()];
-  const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * CT;
+template <int L>
+__global__ void __launch_bounds__(ColCfg<L>::T) k_cols_dec(const float2* __restrict__ Y1, long long y1_stride,
+                                                            float2* __restrict__ Tout, long long t_stride,
+                                                            const float2* __restrict__ Hmain,
+                                                            const float2* __restrict__ Hbeta, BandMap map,
+                                                            const float2* __restrict__ tw) {
+  using C = ColCfg<L>;
+  constexpr int NH = L / 2;
+  __shared__ float2 buf[C::LDS];
+  const int c0 = blockIdx.x * C::CT;
   const int p = blockIdx.y, b = blockIdx.z;
   const int j = map.scale_of[p];
-  tile_load<L, CT, T>(buf, Y1 + (size_t)b * y1_stride + (size_t)j * L * N + c0);
-  __syncthreads();
-  fft_lds<L, 1, -1, CT, T, true, CPAD>(buf, 1, CT, tw, tid);
-  const float2* H = Hmain + (size_t)p * L * N + c0;
-  const float2* Hb = Hbeta + (size_t)p * L;
-  float2 q[PER];
-#pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = tid + e * T;
-    if ((L * CT) % T == 0 || idx < L * CT) q[e] = filt<L, CT, false>(buf, idx, c0, H, Hb);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = tid + e * T;
-    if ((L * CT) % T == 0 || idx < L * CT) buf[lds_ix<CPAD>(idx)] = q[e];
-  }
-  __syncthreads();
-  fft_lds<L, 1, +1, CT, T, true, CPAD>(buf, 1, CT, tw, tid);
-  tile_store<L, CT, T>(Tout + (size_t)b * t_stride + (size_t)p * L * N + c0, buf);
+  float2 u[C::R1], q[C::R1];
+  col_fwd_to_B<L>(u, Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0, buf, tw);
+  col_filter<L, false, false>(u, q, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
+  col_inv_from_B<L>(q, Tout + (size_t)b * t_stride + (size_t)p * L * NH + c0, buf, tw);
 }
 
 // Column pass of the reconstruction: one block per (column tile, direction
 // group, image) forms IFFT_cols(sum_{k in group} conj(H_k) . FFT_cols(U_k));
-// the row pass that follows sums the groups of a scale.
-template <int L, int CT, int T>
-__global__ void __launch_bounds__(T) k_cols_rec(const float2* __restrict__ U, long long u_stride,
-                                                 float2* __restrict__ Q, long long q_stride,
-                                                 const float2* __restrict__ Hmain, const float2* __restrict__ Hbeta,
-                                                 BandMap map, const float2* __restrict__ tw) {
-  constexpr int N = L / 2;
-  constexpr int PER = (L * CT + T - 1) / T;
-  __shared__ float2 buf[lds_size<L * CT, CPAD>()];
-  const int tid = threadIdx.x;
+// the row pass that follows sums the groups of a scale.  For L >= 512 the
+// running sum lives in a second LDS tile (each thread owns its elements), which
+// keeps the register footprint of the 32-point passes below the spill point.
+template <int L>
+struct ColRecAcc {
+  static constexpr bool LDS = L >= 512;
+  static constexpr int SIZE = LDS ? ColCfg<L>::LDS : 1;
+};
+
+template <int L>
+__global__ void __launch_bounds__(ColCfg<L>::T) k_cols_rec(const float2* __restrict__ U, long long u_stride,
+                                                            float2* __restrict__ Q, long long q_stride,
+                                                            const float2* __restrict__ Hmain,
+                                                            const float2* __restrict__ Hbeta, BandMap map,
+                                                            const float2* __restrict__ tw) {
+  using C = ColCfg<L>;
+  constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
+  __shared__ float2 buf[C::LDS];
+  __shared__ float2 accl[ColRecAcc<L>::SIZE];
   const int c0 = blockIdx.x * CT;
   const int g = blockIdx.y, b = blockIdx.z;
-  float2 acc[PER];
+  const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
+  float2 acc[R1];
 #pragma unroll
-  for (int e = 0; e < PER; ++e) acc[e] = make_float2(0.f, 0.f);
+  for (int k = 0; k < R1; ++k) acc[k] = make_float2(0.f, 0.f);
   const int p0 = map.group_band0[g], p1 = map.group_band0[g + 1];
   for (int p = p0; p < p1; ++p) {
-    tile_load<L, CT, T>(buf, U + (size_t)b * u_stride + (size_t)p * L * N + c0);
-    __syncthreads();
-    fft_lds<L, 1, -1, CT, T, true, CPAD>(buf, 1, CT, tw, tid);
-    const float2* H = Hmain + (size_t)p * L * N + c0;
-    const float2* Hb = Hbeta + (size_t)p * L;
+    float2 u[R1];
+    __syncthreads();  // previous direction's reads of buf are complete
+    col_fwd_to_B<L>(u, U + (size_t)b * u_stride + (size_t)p * L * NH + c0, buf, tw);
+    if constexpr (ColRecAcc<L>::LDS) {
+      float2 q[R1];
+      col_filter<L, true, false>(u, q, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-      const int idx = tid + e * T;
-      if ((L * CT) % T == 0 || idx < L * CT) acc[e] = cadd(acc[e], filt<L, CT, true>(buf, idx, c0, H, Hb));
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const int e = lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c);
+        accl[e] = (p == p0) ? q[k1] : cadd(accl[e], q[k1]);
+      }
+    } else {
+      col_filter<L, true, true>(u, acc, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
     }
-    __syncthreads();
   }
+  if constexpr (ColRecAcc<L>::LDS) {
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = tid + e * T;
-    if ((L * CT) % T == 0 || idx < L * CT) buf[lds_ix<CPAD>(idx)] = acc[e];
+    for (int k1 = 0; k1 < R1; ++k1) acc[k1] = accl[lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c)];
   }
-  __syncthreads();
-  fft_lds<L, 1, +1, CT, T, true, CPAD>(buf, 1, CT, tw, tid);
-  tile_store<L, CT, T>(Q + (size_t)b * q_stride + (size_t)g * L * N + c0, buf);
+  col_inv_from_B<L>(acc, Q + (size_t)b * q_stride + (size_t)g * L * NH + c0, buf, tw);
 }
 
 // ---------------------------------------------------------------------------
 // Host-side launchers, one instantiation per supported L.
 template <int L>
-struct SpectralCfg {
-  static constexpr int N = L / 2;
-  static constexpr int ROWS = (L >= 512) ? 4 : (2048 / L < 16 ? 2048 / L : 16);
-  static constexpr int RT = 256;
-  static constexpr int CT = (N >= 8) ? 8 : N;
-  static constexpr int CTT = (L >= 512) ? 512 : 256;
-};
-
-template <int L>
 struct Launch {
-  using C = SpectralCfg<L>;
+  using R = RowCfg<L>;
+  using C = ColCfg<L>;
   static hipError_t rows_r2c(const PlaneSet& in, const PlaneSet& out, int np, int nimg, const ThreshSpec& th,
                              const float2* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_rows_r2c<L, C::ROWS, C::RT>), dim3(L / C::ROWS, np, nimg), dim3(C::RT), 0, s, in, out, th,
-                       tw);
+    hipLaunchKernelGGL((k_rows_r2c<L>), dim3(L / R::ROWS, np, nimg), dim3(R::T), 0, s, in, out, th, tw);
     return hipGetLastError();
   }
   static hipError_t rows_c2r(const PlaneSet& in, const PlaneSet& out, int np, int nimg, const int* sum_first,
@@ -343,30 +443,27 @@ struct Launch {
     SumTable st{};
     if (sum_first) {
       for (int i = 0; i <= np; ++i) st.first[i] = sum_first[i];
-      hipLaunchKernelGGL((k_rows_c2r<L, C::ROWS, C::RT, true>), dim3(L / C::ROWS, np, nimg), dim3(C::RT), 0, s, in,
-                         out, st, tw);
+      hipLaunchKernelGGL((k_rows_c2r<L, true>), dim3(L / R::ROWS, np, nimg), dim3(R::T), 0, s, in, out, st, tw);
     } else {
-      hipLaunchKernelGGL((k_rows_c2r<L, C::ROWS, C::RT, false>), dim3(L / C::ROWS, np, nimg), dim3(C::RT), 0, s, in,
-                         out, st, tw);
+      hipLaunchKernelGGL((k_rows_c2r<L, false>), dim3(L / R::ROWS, np, nimg), dim3(R::T), 0, s, in, out, st, tw);
     }
     return hipGetLastError();
   }
   static hipError_t rows_c2r_r2c(const PlaneSet& Tin, const PlaneSet& band, const PlaneSet& U, int np, int nimg,
                                  const ThreshSpec& th, const float2* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_rows_c2r_r2c<L, C::ROWS, C::RT>), dim3(L / C::ROWS, np, nimg), dim3(C::RT), 0, s, Tin,
-                       band, U, th, tw);
+    hipLaunchKernelGGL((k_rows_c2r_r2c<L>), dim3(L / R::ROWS, np, nimg), dim3(R::T), 0, s, Tin, band, U, th, tw);
     return hipGetLastError();
   }
   static hipError_t cols_dec(const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
                              const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_cols_dec<L, C::CT, C::CTT>), dim3(C::N / C::CT, map.nbands, nimg), dim3(C::CTT), 0, s, Y1,
-                       ys, To, ts, Hm, Hb, map, tw);
+    hipLaunchKernelGGL((k_cols_dec<L>), dim3(L / 2 / C::CT, map.nbands, nimg), dim3(C::T), 0, s, Y1, ys, To, ts, Hm,
+                       Hb, map, tw);
     return hipGetLastError();
   }
   static hipError_t cols_rec(const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
                              const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_cols_rec<L, C::CT, C::CTT>), dim3(C::N / C::CT, map.ngroups, nimg), dim3(C::CTT), 0, s, U,
-                       us, Q, qs, Hm, Hb, map, tw);
+    hipLaunchKernelGGL((k_cols_rec<L>), dim3(L / 2 / C::CT, map.ngroups, nimg), dim3(C::T), 0, s, U, us, Q, qs, Hm,
+                       Hb, map, tw);
     return hipGetLastError();
   }
 };
