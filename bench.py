@@ -173,6 +173,9 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle work (0 = skip)")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-plain-1024", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the NSST path has no multi-GPU form here); "
@@ -180,6 +183,13 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     L, sigma = args.size, 20.0
     ms, stages, sched = bench_plain(L, sigma, args.steps, args.warmup)
@@ -203,7 +213,7 @@ def main():
                           "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
         "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()} for s in stages],
     }
-    if L == 512:
+    if L == 512 and not args.no_plain_1024:
         ms1k, _, _ = bench_plain(1024, sigma, max(20, args.steps // 4), 5)
         res["plain_1024"] = {"ms_per_image": round(ms1k, 5), "Mpixel_s": round(1024 * 1024 / (ms1k * 1e-3) / 1e6, 2),
                              "path_roofline_frac": round(400.0 * 1024 * 1024 / (ms1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
