@@ -497,34 +497,36 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t s, int* 
   }
   hipMemsetAsync(coeff, 0, sizeof(float) * pl * (1 + d->nbands) * nimg, s);
   Coeffs c = stack_coeffs(d, coeff);
-  double best = 1e30;
+  // candidates are timed in three interleaved rounds; each keeps its best time
+  double tp[shx::PYR_VARIANTS];
+  for (double& t : tp) t = 1e30;
+  for (int round = 0; round < 3; ++round)
+    for (int v = 0; v < shx::PYR_VARIANTS; ++v) {
+      w.pyr_variant = v;
+      std::vector<Stage> st;
+      stages_pyr_dec(d, w, nimg, img, pl, c, s, st);
+      stages_pyr_rec(d, w, nimg, c, Thr{}, img + pl * nimg, pl, s, st);
+      const double t = time_stages(st, s, reps);
+      if (t > 0 && t < tp[v]) tp[v] = t;
+    }
   int bestv = 0;
-  for (int v = 0; v < shx::PYR_VARIANTS; ++v) {
-    w.pyr_variant = v;
-    std::vector<Stage> st;
-    stages_pyr_dec(d, w, nimg, img, pl, c, s, st);
-    stages_pyr_rec(d, w, nimg, c, Thr{}, img + pl * nimg, pl, s, st);
-    const double t = time_stages(st, s, reps);
-    if (t > 0 && t < best) {
-      best = t;
-      bestv = v;
-    }
-  }
+  for (int v = 1; v < shx::PYR_VARIANTS; ++v)
+    if (tp[v] < tp[bestv]) bestv = v;
   w.pyr_variant = bestv;
-  best = 1e30;
-  int bestg = 4;
   const int cand[3] = {2, 4, 8};
-  for (int g : cand) {
-    set_group(d, w, g);
-    std::vector<Stage> st;
-    stages_spec_rec_tail(d, w, nimg, s, st);
-    const double t = time_stages(st, s, reps);
-    if (t > 0 && t < best) {
-      best = t;
-      bestg = g;
+  double tg[3] = {1e30, 1e30, 1e30};
+  for (int round = 0; round < 3; ++round)
+    for (int i = 0; i < 3; ++i) {
+      set_group(d, w, cand[i]);
+      std::vector<Stage> st;
+      stages_spec_rec_tail(d, w, nimg, s, st);
+      const double t = time_stages(st, s, reps);
+      if (t > 0 && t < tg[i]) tg[i] = t;
     }
-  }
-  set_group(d, w, bestg);
+  int bi = 0;
+  for (int i = 1; i < 3; ++i)
+    if (tg[i] < tg[bi]) bi = i;
+  set_group(d, w, cand[bi]);
   hipStreamSynchronize(s);
   hipFree(img);
   hipFree(coeff);
