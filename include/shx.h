@@ -118,6 +118,20 @@ int shx_batch_inverse_shear(shx_batch* batch, const float* coeff, const float* t
 int shx_batch_denoise(shx_batch* batch, const float* x, float* coeff, const float* th_dev, float* out,
                       void* stream);
 
+/* ---- adjoints (gradients) -------------------------------------------------------
+ * Transposes of the analysis A = shear_trans and the synthesis
+ * S = inverse_shear, for gradient computations (no counterpart in the
+ * reference).  A^T maps coefficient gradients (cells) to an image gradient;
+ * S^T maps an image gradient to coefficient gradients.  The directional stage
+ * is a tight frame, so its transpose is the directional reconstruction; the
+ * pyramid levels are transposed exactly, including the symmetric extension
+ * (zero-padded correlation on the extended grid, then fold-add). */
+int shx_shear_trans_adjoint(shx_dict* dict, const float* const* coeff_bar, float* x_bar, void* stream);
+int shx_inverse_shear_adjoint(shx_dict* dict, const float* x_bar, float* const* coeff_bar, void* stream);
+/* Batched forms (contiguous stacks as in shx_batch_shear_trans). */
+int shx_batch_shear_trans_adjoint(shx_batch* batch, const float* coeff_bar, float* x_bar, void* stream);
+int shx_batch_inverse_shear_adjoint(shx_batch* batch, const float* x_bar, float* coeff_bar, void* stream);
+
 /* Autotuner: times the pyramid tile shapes and the reconstruction's
  * directions-per-block grouping on the plan's own workspace (the dictionary's
  * one-image plan when batch is NULL) and keeps the fastest.  chosen (may be
@@ -166,6 +180,24 @@ int shx_atrousrec(const float* const* y, int L, int nlevels, float* x, float* wo
  * numel(coeff)-1 levels (shearcuda/src_cuda/re_weight_band_cuda.cpp:59-183).
  * work: ncoeff + 2 planes or NULL (allocated per call). */
 int shx_re_weight_band(const float* x, int L, const double* coeff, int ncoeff, float* y, float* work, void* stream);
+
+/* ---- undecimated periodic 2D wavelet transform (Rice Wavelet Toolbox) -------
+ * [yl, yh] = mrdwt(x, h, L) and x = mirdwt(yl, yh, h, L)
+ * (shearcuda/src/mrdwt_r.h:70-207, mirdwt_r.h:92-218; GPU entry points
+ * shearcuda/src_cuda/mrdwt_cuda.cpp:58-253, mirdwt_cuda.cpp:58-255).
+ * x, yl: nimg x m x n; yh: nimg x m x 3nL with level l in columns
+ * [3n(l-1), 3nl) as [lh hl hh]; h: lh (even) scaling-filter taps (host FP64);
+ * m and n divisible by 2^L.  work: 2*m*n*nimg floats or NULL (allocated).
+ * The *_adjoint entry points apply the transposes (mrdwt^T, mirdwt^T); they
+ * have no counterpart in the reference and serve gradient computations. */
+int shx_mrdwt(const float* x, int m, int n, const double* h, int lh, int L, float* yl, float* yh, float* work,
+              int nimg, void* stream);
+int shx_mirdwt(const float* yl, const float* yh, int m, int n, const double* h, int lh, int L, float* x, float* work,
+               int nimg, void* stream);
+int shx_mrdwt_adjoint(const float* yl_bar, const float* yh_bar, int m, int n, const double* h, int lh, int L,
+                      float* x_bar, float* work, int nimg, void* stream);
+int shx_mirdwt_adjoint(const float* x_bar, int m, int n, const double* h, int lh, int L, float* yl_bar,
+                       float* yh_bar, float* work, int nimg, void* stream);
 
 /* Sum |x|^p over n elements, result (p-th root not taken) in *out (host).
  * norm_cuda's reduction (shearcuda/src_cuda/norm_cuda.cpp:172-215). */

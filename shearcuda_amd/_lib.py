@@ -45,6 +45,10 @@ _SIGNATURES = [
     ("shx_batch_shear_trans", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("shx_batch_inverse_shear", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("shx_batch_denoise", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx_shear_trans_adjoint", c_int, [c_void_p, P(c_void_p), c_void_p, c_void_p]),
+    ("shx_inverse_shear_adjoint", c_int, [c_void_p, c_void_p, P(c_void_p), c_void_p]),
+    ("shx_batch_shear_trans_adjoint", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx_batch_inverse_shear_adjoint", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("shx_autotune", c_int, [c_void_p, c_void_p, c_int, P(c_int), c_void_p]),
     ("shx_set_schedule", c_int, [c_void_p, c_void_p, c_int, c_int]),
     ("shx_capture_begin", c_int, [c_void_p]),
@@ -58,6 +62,14 @@ _SIGNATURES = [
     ("shx_atrousdec", c_int, [c_void_p, c_int, c_int, P(c_void_p), c_void_p, c_void_p]),
     ("shx_atrousrec", c_int, [P(c_void_p), c_int, c_int, c_void_p, c_void_p, c_void_p]),
     ("shx_re_weight_band", c_int, [c_void_p, c_int, P(c_double), c_int, c_void_p, c_void_p, c_void_p]),
+    ("shx_mrdwt", c_int, [c_void_p, c_int, c_int, P(c_double), c_int, c_int, c_void_p, c_void_p, c_void_p, c_int,
+                          c_void_p]),
+    ("shx_mirdwt", c_int, [c_void_p, c_void_p, c_int, c_int, P(c_double), c_int, c_int, c_void_p, c_void_p, c_int,
+                           c_void_p]),
+    ("shx_mrdwt_adjoint", c_int, [c_void_p, c_void_p, c_int, c_int, P(c_double), c_int, c_int, c_void_p, c_void_p,
+                                  c_int, c_void_p]),
+    ("shx_mirdwt_adjoint", c_int, [c_void_p, c_int, c_int, P(c_double), c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                   c_int, c_void_p]),
     ("shx_norm_pow", c_int, [c_void_p, c_int64, c_double, P(c_double), c_void_p]),
     ("shx_softth", c_int, [c_void_p, c_int64, c_double, c_void_p]),
     ("shx_maxflat_factor_error", c_double, []),

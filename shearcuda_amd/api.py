@@ -169,6 +169,29 @@ def inverse_shear_cuda(Ct, pfilt, shear, out=None):
     return out
 
 
+def shear_trans_adjoint_cuda(Ct_bar, shear, out=None):
+    """Transpose of shear_trans_cuda: coefficient gradients -> image gradient."""
+    if len(Ct_bar) != shear.nscales + 1:
+        raise ValueError("Incorrect number of cell elements")
+    for i, c in enumerate(Ct_bar):
+        _require_gpu(c, "Ct_bar{%d}" % (i + 1))
+    if out is None:
+        out = torch.empty((shear.L, shear.L), device=Ct_bar[0].device, dtype=torch.float32)
+    check(lib.shx_shear_trans_adjoint(shear.handle, ptr_array([c.data_ptr() for c in Ct_bar]),
+                                      ctypes.c_void_p(out.data_ptr()), _stream()))
+    return out
+
+
+def inverse_shear_adjoint_cuda(x_bar, shear, Ct_bar=None):
+    """Transpose of inverse_shear_cuda: image gradient -> coefficient gradients."""
+    _require_gpu(x_bar, "x_bar")
+    if Ct_bar is None:
+        Ct_bar = shear.alloc_coeffs()
+    check(lib.shx_inverse_shear_adjoint(shear.handle, ctypes.c_void_p(x_bar.data_ptr()),
+                                        ptr_array([c.data_ptr() for c in Ct_bar]), _stream()))
+    return Ct_bar
+
+
 def inverse_shear_thresh_cuda(Ct, th, shear, out=None):
     """Inverse NSST of hard-thresholded bands (thresholds fused into the
     reconstruction's first read; Ct is not modified).  th: 1 + sum(D_j)
@@ -435,6 +458,18 @@ class Batch:
                                           ctypes.c_void_p(out.data_ptr()), _stream()))
         return out
 
+    def shear_trans_adjoint(self, coeff_bar, x_bar):
+        self._chk(coeff_bar, x_bar)
+        check(lib.shx_batch_shear_trans_adjoint(self._h, ctypes.c_void_p(coeff_bar.data_ptr()),
+                                                ctypes.c_void_p(x_bar.data_ptr()), _stream()))
+        return x_bar
+
+    def inverse_shear_adjoint(self, x_bar, coeff_bar):
+        self._chk(x_bar, coeff_bar)
+        check(lib.shx_batch_inverse_shear_adjoint(self._h, ctypes.c_void_p(x_bar.data_ptr()),
+                                                  ctypes.c_void_p(coeff_bar.data_ptr()), _stream()))
+        return coeff_bar
+
     def denoise(self, x, coeff, th, out):
         self._chk(x, coeff, out)
         _require_gpu(th, "th")
@@ -464,3 +499,68 @@ def autotune(shear, batch=None, reps=5):
 def set_schedule(shear, pyramid_variant, group, batch=None):
     check(lib.shx_set_schedule(shear.handle, batch._h if batch is not None else None, int(pyramid_variant),
                                int(group)))
+
+
+# ---- undecimated wavelet transform -----------------------------------------------
+
+def _rdwt_dims(x):
+    if x.dim() == 2:
+        return 1, x.shape[0], x.shape[1]
+    if x.dim() == 3:
+        return x.shape[0], x.shape[1], x.shape[2]
+    raise ValueError("expected (m, n) or (nimg, m, n)")
+
+
+def _h_arg(h):
+    h = np.ascontiguousarray(np.asarray(h, dtype=np.float64).ravel())
+    return h, h.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def mrdwt_cuda(x, h, L):
+    """[yl, yh] = mrdwt_cuda(x, h, L) (shearcuda/src_cuda/mrdwt_cuda.cpp:58-253);
+    x is (m, n) or a batch (nimg, m, n); yh is (..., m, 3*n*L)."""
+    _require_gpu(x)
+    nimg, m, n = _rdwt_dims(x)
+    h, hp = _h_arg(h)
+    yl = torch.empty_like(x)
+    yh = torch.empty(x.shape[:-1] + (3 * n * L,), device=x.device, dtype=torch.float32)
+    check(lib.shx_mrdwt(ctypes.c_void_p(x.data_ptr()), m, n, hp, h.size, int(L), ctypes.c_void_p(yl.data_ptr()),
+                        ctypes.c_void_p(yh.data_ptr()), None, nimg, _stream()))
+    return yl, yh
+
+
+def mirdwt_cuda(yl, yh, h, L):
+    """x = mirdwt_cuda(yl, yh, h, L) (shearcuda/src_cuda/mirdwt_cuda.cpp:58-255)."""
+    _require_gpu(yl, "yl")
+    _require_gpu(yh, "yh")
+    nimg, m, n = _rdwt_dims(yl)
+    h, hp = _h_arg(h)
+    x = torch.empty_like(yl)
+    check(lib.shx_mirdwt(ctypes.c_void_p(yl.data_ptr()), ctypes.c_void_p(yh.data_ptr()), m, n, hp, h.size, int(L),
+                         ctypes.c_void_p(x.data_ptr()), None, nimg, _stream()))
+    return x
+
+
+def mrdwt_adjoint_cuda(yl_bar, yh_bar, h, L):
+    """Transpose of mrdwt: (yl_bar, yh_bar) -> x_bar."""
+    _require_gpu(yl_bar, "yl_bar")
+    _require_gpu(yh_bar, "yh_bar")
+    nimg, m, n = _rdwt_dims(yl_bar)
+    h, hp = _h_arg(h)
+    xb = torch.empty_like(yl_bar)
+    check(lib.shx_mrdwt_adjoint(ctypes.c_void_p(yl_bar.data_ptr()), ctypes.c_void_p(yh_bar.data_ptr()), m, n, hp,
+                                h.size, int(L), ctypes.c_void_p(xb.data_ptr()), None, nimg, _stream()))
+    return xb
+
+
+def mirdwt_adjoint_cuda(x_bar, h, L):
+    """Transpose of mirdwt: x_bar -> (yl_bar, yh_bar)."""
+    _require_gpu(x_bar, "x_bar")
+    nimg, m, n = _rdwt_dims(x_bar)
+    h, hp = _h_arg(h)
+    yl = torch.empty_like(x_bar)
+    yh = torch.empty(x_bar.shape[:-1] + (3 * n * L,), device=x_bar.device, dtype=torch.float32)
+    check(lib.shx_mirdwt_adjoint(ctypes.c_void_p(x_bar.data_ptr()), m, n, hp, h.size, int(L),
+                                 ctypes.c_void_p(yl.data_ptr()), ctypes.c_void_p(yh.data_ptr()), None, nimg,
+                                 _stream()))
+    return yl, yh

@@ -14,6 +14,7 @@
 #include "../../include/shx.h"
 #include "dict_host.h"
 #include "pyramid.h"
+#include "rdwt.h"
 #include "spectral.h"
 
 using shx::BandMap;
@@ -49,7 +50,8 @@ struct Work {
   float2* spec = nullptr;   // per image: S packed spectra (Y1 / rec input of the row pass)
   float2* band = nullptr;   // per image: nbands packed spectra (T / U)
   float2* q = nullptr;      // per image: ngroups packed partial reconstructions
-  long long s_planes = 0, s_spec = 0, s_band = 0, s_q = 0;  // floats per image
+  float* ext = nullptr;     // per image: two extended-grid planes (adjoint levels)
+  long long s_planes = 0, s_spec = 0, s_band = 0, s_q = 0, s_ext = 0;  // floats per image
   size_t bytes = 0;
   // schedule picked by the autotuner (defaults until shx_autotune runs)
   int pyr_variant = 0;  // pyramid tile shape
@@ -188,14 +190,19 @@ int alloc_work(const shx_dict* d, int nimg, Work& w) {
   w.s_spec = pl * d->S;
   w.s_band = pl * d->nbands;
   w.s_q = pl * max_groups(d);
+  {
+    const long long E = shx::adjoint_ext_side(d->L, 1 << (d->S - 1));
+    w.s_ext = 2 * E * E;
+  }
   const size_t n = (size_t)nimg;
   hipError_t e;
   if ((e = hipMalloc(&w.planes, sizeof(float) * w.s_planes * n)) != hipSuccess ||
       (e = hipMalloc(&w.spec, sizeof(float) * w.s_spec * n)) != hipSuccess ||
       (e = hipMalloc(&w.band, sizeof(float) * w.s_band * n)) != hipSuccess ||
-      (e = hipMalloc(&w.q, sizeof(float) * w.s_q * n)) != hipSuccess)
+      (e = hipMalloc(&w.q, sizeof(float) * w.s_q * n)) != hipSuccess ||
+      (e = hipMalloc(&w.ext, sizeof(float) * w.s_ext * n)) != hipSuccess)
     return fail(SHX_ERR_ALLOC, "workspace allocation (%d images of %d^2) failed: %s", nimg, d->L, hipGetErrorString(e));
-  w.bytes = sizeof(float) * n * (w.s_planes + w.s_spec + w.s_band + w.s_q);
+  w.bytes = sizeof(float) * n * (w.s_planes + w.s_spec + w.s_band + w.s_q + w.s_ext);
   return SHX_OK;
 }
 
@@ -204,7 +211,9 @@ void free_work(Work& w) {
   hipFree(w.spec);
   hipFree(w.band);
   hipFree(w.q);
+  hipFree(w.ext);
   w.planes = nullptr;
+  w.ext = nullptr;
   w.spec = w.band = w.q = nullptr;
   w.bytes = 0;
 }
@@ -384,6 +393,81 @@ void stages_denoise(shx_dict* d, const Work& w, int nimg, const float* x, long l
                 [=] { return shx::launch_rows_c2r_r2c(L, bandset, cb, bandset, nb, nimg, th, d->tw, s); }});
   stages_spec_rec_tail(d, w, nimg, s, st);
   stages_pyr_rec(d, w, nimg, c, t, out, os, s, st);
+}
+
+// Adjoint of the pyramid decomposition: gradients of the lowpass (c.low) and
+// of the pyramid bands (workspace ypyr planes) -> gradient of the image gx.
+void stages_pyr_dec_adjoint(shx_dict* d, const Work& w, int nimg, const Coeffs& c, float* gx, long long gxs,
+                            hipStream_t s, std::vector<Stage>& st) {
+  const int L = d->L, S = d->S;
+  const long long pl = (long long)L * L;
+  const double P = 4.0 * pl * nimg;
+  const float* glow = c.low;
+  long long gls = c.low_stride;
+  for (int lev = S - 1; lev >= 0; --lev) {
+    const int m = 1 << lev;
+    const float* gh = w.planes + (size_t)(S - lev - 1) * pl;
+    float* o = (lev == 0) ? gx : w.planes + (size_t)(S + (lev & 1)) * pl;
+    const long long os = (lev == 0) ? gxs : w.s_planes;
+    shx::ImgStrides is{gls, w.s_planes, os};
+    st.push_back({"atrous_dec_adj_m" + std::to_string(m), 3 * P, [=] {
+                    return shx::launch_atrous_dec_adjoint(glow, gh, o, w.ext, is, w.s_ext, nimg, L, m, s);
+                  }});
+    glow = o;
+    gls = os;
+  }
+}
+
+// Adjoint of the pyramid reconstruction: gradient of the output image gout ->
+// gradients of the lowpass (c.low) and of the pyramid bands (ypyr planes).
+void stages_pyr_rec_adjoint(shx_dict* d, const Work& w, int nimg, const float* gout, long long gos, const Coeffs& c,
+                            hipStream_t s, std::vector<Stage>& st) {
+  const int L = d->L, S = d->S;
+  const long long pl = (long long)L * L;
+  const double P = 4.0 * pl * nimg;
+  const float* gi = gout;
+  long long gis = gos;
+  for (int lev = 0; lev < S; ++lev) {
+    const int m = 1 << lev;
+    const bool last = lev == S - 1;
+    float* glow = last ? c.low : w.planes + (size_t)(S + (lev & 1)) * pl;
+    const long long gls = last ? c.low_stride : w.s_planes;
+    float* gh = w.planes + (size_t)(S - lev - 1) * pl;
+    shx::ImgStrides is{gis, gls, w.s_planes};
+    st.push_back({"atrous_rec_adj_m" + std::to_string(m), 3 * P, [=] {
+                    return shx::launch_atrous_rec_adjoint(gi, glow, gh, w.ext, is, w.s_ext, nimg, L, m, s);
+                  }});
+    gi = glow;
+    gis = gls;
+  }
+}
+
+// A^T: coefficient gradients -> image gradient (directional stage: its
+// adjoint is the directional reconstruction, the filters form a tight frame).
+void stages_dec_adjoint(shx_dict* d, const Work& w, int nimg, const Coeffs& c, float* gx, long long gxs,
+                        hipStream_t s, std::vector<Stage>& st) {
+  const int L = d->L, nb = d->nbands;
+  const double P = 4.0 * L * L * nimg;
+  PlaneSet in = c.bands;
+  PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
+  shx::ThreshSpec none{};
+  st.push_back({"rows_r2c_bands", 2.0 * nb * P,
+                [=] { return shx::launch_rows_r2c(L, in, bandset, nb, nimg, none, d->tw, s); }});
+  stages_spec_rec_tail(d, w, nimg, s, st);
+  stages_pyr_dec_adjoint(d, w, nimg, c, gx, gxs, s, st);
+}
+
+// S^T: image gradient -> coefficient gradients.
+void stages_rec_adjoint(shx_dict* d, const Work& w, int nimg, const float* gx, long long gxs, const Coeffs& c,
+                        hipStream_t s, std::vector<Stage>& st) {
+  stages_pyr_rec_adjoint(d, w, nimg, gx, gxs, c, s, st);
+  stages_spec_dec(d, w, nimg, s, st);
+  const int L = d->L, nb = d->nbands;
+  const double P = 4.0 * L * L * nimg;
+  PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
+  PlaneSet out = c.bands;
+  st.push_back({"rows_c2r_bands", 2.0 * nb * P,
+                [=] { return shx::launch_rows_c2r(L, bandset, out, nb, nimg, nullptr, d->tw, s); }});
 }
 
 int run_stages(const std::vector<Stage>& st) {
@@ -800,6 +884,44 @@ int shx_set_schedule(shx_dict* d, shx_batch* b, int pyr_variant, int group) {
   return SHX_OK;
 }
 
+// ---- adjoints ----------------------------------------------------------------------
+int shx_shear_trans_adjoint(shx_dict* d, const float* const* coeff_bar, float* x_bar, void* stream) {
+  if (int r = check_dict(d)) return r;
+  if (!x_bar) return fail(SHX_ERR_ARG, "null output");
+  if (int r = check_cells(d, (const void* const*)coeff_bar)) return r;
+  std::vector<Stage> st;
+  stages_dec_adjoint(d, d->w, 1, cell_coeffs(d, const_cast<float* const*>(coeff_bar)), x_bar, 0,
+                     (hipStream_t)stream, st);
+  return run_stages(st);
+}
+
+int shx_inverse_shear_adjoint(shx_dict* d, const float* x_bar, float* const* coeff_bar, void* stream) {
+  if (int r = check_dict(d)) return r;
+  if (!x_bar) return fail(SHX_ERR_ARG, "null input");
+  if (int r = check_cells(d, (const void* const*)coeff_bar)) return r;
+  std::vector<Stage> st;
+  stages_rec_adjoint(d, d->w, 1, x_bar, 0, cell_coeffs(d, coeff_bar), (hipStream_t)stream, st);
+  return run_stages(st);
+}
+
+int shx_batch_shear_trans_adjoint(shx_batch* b, const float* coeff_bar, float* x_bar, void* stream) {
+  if (int r = batch_args(b, coeff_bar, coeff_bar, x_bar)) return r;
+  shx_dict* d = b->d;
+  std::vector<Stage> st;
+  stages_dec_adjoint(d, b->w, b->nimg, stack_coeffs(d, const_cast<float*>(coeff_bar)), x_bar,
+                     (long long)d->L * d->L, (hipStream_t)stream, st);
+  return run_stages(st);
+}
+
+int shx_batch_inverse_shear_adjoint(shx_batch* b, const float* x_bar, float* coeff_bar, void* stream) {
+  if (int r = batch_args(b, x_bar, coeff_bar, coeff_bar)) return r;
+  shx_dict* d = b->d;
+  std::vector<Stage> st;
+  stages_rec_adjoint(d, b->w, b->nimg, x_bar, (long long)d->L * d->L, stack_coeffs(d, coeff_bar),
+                     (hipStream_t)stream, st);
+  return run_stages(st);
+}
+
 // ---- graphs --------------------------------------------------------------------
 int shx_capture_begin(void* stream) {
   hipStream_t s = (hipStream_t)stream;
@@ -988,6 +1110,94 @@ int shx_re_weight_band(const float* x, int L, const double* coeff, int ncoeff, f
   }
   SHX_HIP(e);
   return SHX_OK;
+}
+
+}  // extern "C"
+
+// ---- undecimated wavelet transform ---------------------------------------------
+namespace {
+// taps of the analysis (a0, a1) or synthesis (g0, g1) passes from the scaling
+// filter h (mrdwt_r.h:134-140, mirdwt_r.h:136-141)
+shx::RdwtFilt rdwt_taps(const double* h, int lh, bool synthesis, double scale) {
+  shx::RdwtFilt f{};
+  f.lh = lh;
+  for (int j = 0; j < lh; ++j) {
+    if (!synthesis) {
+      f.f0[j] = (float)(scale * h[j]);
+      f.f1[j] = (float)(scale * h[lh - 1 - j] * (((lh - 1 - j) % 2 == 0) ? -1.0 : 1.0));
+    } else {
+      f.f0[j] = (float)(scale * h[j] / 2.0);
+      f.f1[j] = (float)(scale * h[lh - 1 - j] / 2.0 * ((j % 2 == 1) ? -1.0 : 1.0));
+    }
+  }
+  return f;
+}
+
+int rdwt_check(int m, int n, const double* h, int lh, int L, int nimg) {
+  if (!h) return fail(SHX_ERR_ARG, "null filter");
+  if (lh < 2 || lh > shx::RDWT_MAX_TAPS || lh % 2) return fail(SHX_ERR_ARG, "filter length %d (even, <= %d)", lh,
+                                                               shx::RDWT_MAX_TAPS);
+  if (L < 1 || L > 12) return fail(SHX_ERR_ARG, "invalid number of levels %d", L);
+  if (m < 1 || n < 1 || m % (1 << L) || n % (1 << L))
+    return fail(SHX_ERR_ARG, "The row and the column dimension must be divisible by 2^L");
+  if (nimg < 1) return fail(SHX_ERR_ARG, "batch of %d images", nimg);
+  return SHX_OK;
+}
+
+template <class F>
+int with_tmp(float* work, size_t floats, hipStream_t s, F f) {
+  float* w = work;
+  if (!w) SHX_HIP(hipMalloc(&w, sizeof(float) * floats));
+  hipError_t e = f(w);
+  if (!work) {
+    hipStreamSynchronize(s);
+    hipFree(w);
+  }
+  SHX_HIP(e);
+  return SHX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int shx_mrdwt(const float* x, int m, int n, const double* h, int lh, int L, float* yl, float* yh, float* work,
+              int nimg, void* stream) {
+  if (int r = rdwt_check(m, n, h, lh, L, nimg)) return r;
+  if (!x || !yl || !yh) return fail(SHX_ERR_ARG, "null input or output");
+  hipStream_t s = (hipStream_t)stream;
+  const shx::RdwtFilt f = rdwt_taps(h, lh, false, 1.0);
+  return with_tmp(work, 2 * (size_t)m * n * nimg, s,
+                  [&](float* t) { return shx::rdwt_analysis(x, yl, yh, t, m, n, L, f, nimg, s); });
+}
+
+int shx_mirdwt(const float* yl, const float* yh, int m, int n, const double* h, int lh, int L, float* x, float* work,
+               int nimg, void* stream) {
+  if (int r = rdwt_check(m, n, h, lh, L, nimg)) return r;
+  if (!x || !yl || !yh) return fail(SHX_ERR_ARG, "null input or output");
+  hipStream_t s = (hipStream_t)stream;
+  const shx::RdwtFilt f = rdwt_taps(h, lh, true, 1.0);
+  return with_tmp(work, 2 * (size_t)m * n * nimg, s,
+                  [&](float* t) { return shx::rdwt_synthesis(yl, yh, x, t, m, n, L, f, nimg, s); });
+}
+
+int shx_mrdwt_adjoint(const float* yl_bar, const float* yh_bar, int m, int n, const double* h, int lh, int L,
+                      float* x_bar, float* work, int nimg, void* stream) {
+  if (int r = rdwt_check(m, n, h, lh, L, nimg)) return r;
+  if (!x_bar || !yl_bar || !yh_bar) return fail(SHX_ERR_ARG, "null input or output");
+  hipStream_t s = (hipStream_t)stream;
+  const shx::RdwtFilt f = rdwt_taps(h, lh, true, 2.0);  // synthesis passes with the analysis taps
+  return with_tmp(work, 2 * (size_t)m * n * nimg, s,
+                  [&](float* t) { return shx::rdwt_synthesis(yl_bar, yh_bar, x_bar, t, m, n, L, f, nimg, s); });
+}
+
+int shx_mirdwt_adjoint(const float* x_bar, int m, int n, const double* h, int lh, int L, float* yl_bar,
+                       float* yh_bar, float* work, int nimg, void* stream) {
+  if (int r = rdwt_check(m, n, h, lh, L, nimg)) return r;
+  if (!x_bar || !yl_bar || !yh_bar) return fail(SHX_ERR_ARG, "null input or output");
+  hipStream_t s = (hipStream_t)stream;
+  const shx::RdwtFilt f = rdwt_taps(h, lh, false, 0.5);  // analysis passes with the synthesis taps
+  return with_tmp(work, 2 * (size_t)m * n * nimg, s,
+                  [&](float* t) { return shx::rdwt_analysis(x_bar, yl_bar, yh_bar, t, m, n, L, f, nimg, s); });
 }
 
 int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream) {

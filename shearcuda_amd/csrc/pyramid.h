@@ -36,4 +36,17 @@ hipError_t launch_atrous_dec(const float* x, float* ylow, float* yhigh, const Im
 hipError_t launch_atrous_rec(const float* xlow, const float* yhigh, float* out, const ImgStrides& st, int nimg, int L,
                              int m, const LowThresh& lt, float sx, float sy, hipStream_t s, int variant = 0);
 
+// Adjoints of one level (gradients).  zext: scratch of the extended grid,
+// side adjoint_ext_side(L, m), one plane (dec) or two planes (rec) per image,
+// zstride floats between images.
+//   dec level y_low = C_h0 E x, y_high = C_h1 E x:
+//       gx = E^T (C_h0^T glow + C_h1^T ghigh);  strides {glow, ghigh, gx}
+//   rec level out = C_g0 E x_low + C_g1 E y:
+//       glow = E^T C_g0^T gout, ghigh = E^T C_g1^T gout;  strides {gout, glow, ghigh}
+int adjoint_ext_side(int L, int m);
+hipError_t launch_atrous_dec_adjoint(const float* glow, const float* ghigh, float* gx, float* zext,
+                                     const ImgStrides& st, long long zstride, int nimg, int L, int m, hipStream_t s);
+hipError_t launch_atrous_rec_adjoint(const float* gout, float* glow, float* ghigh, float* zext, const ImgStrides& st,
+                                     long long zstride, int nimg, int L, int m, hipStream_t s);
+
 }  // namespace shx

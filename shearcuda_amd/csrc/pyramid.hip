@@ -14,6 +14,12 @@
 // (with a halo of H = 9 sub-lattice samples) in LDS; the symmetric extension
 // is folded into that loader.
 //
+// The transposes (for gradients) use the same tiles: the adjoint of a level
+// with symmetric filters is the same correlation applied to the zero-padded
+// output gradient, evaluated on the extended grid [-9m, L+9m)^2, followed by
+// a fold-add of the extended grid back onto [0, L)^2 (the transpose of the
+// symmetric extension).
+//
 // The maxflat filters have exact low rank (h0: 2, h1: 4, g0: 3, g1: 3), so
 // h = sum_k v_k (x) w_k is applied as RANK vertical 1D passes into LDS
 // followed by RANK horizontal 1D passes, skipping the taps 1 and LEN-2 that
@@ -36,6 +42,34 @@ template <int LEN>
 __device__ __forceinline__ constexpr bool zero_tap(int a) {
   return a == 1 || a == LEN - 2;
 }
+
+// Filter traits: 0 = h0, 1 = h1, 2 = g0, 3 = g1 (maxflat_factors.h).
+template <int F>
+struct Filt;
+template <>
+struct Filt<0> {
+  static constexpr int LEN = mf::H0_LEN, RANK = mf::H0_RANK;
+  static constexpr const float (&V)[RANK][LEN] = mf::H0_V;
+  static constexpr const float (&W)[RANK][LEN] = mf::H0_W;
+};
+template <>
+struct Filt<1> {
+  static constexpr int LEN = mf::H1_LEN, RANK = mf::H1_RANK;
+  static constexpr const float (&V)[RANK][LEN] = mf::H1_V;
+  static constexpr const float (&W)[RANK][LEN] = mf::H1_W;
+};
+template <>
+struct Filt<2> {
+  static constexpr int LEN = mf::G0_LEN, RANK = mf::G0_RANK;
+  static constexpr const float (&V)[RANK][LEN] = mf::G0_V;
+  static constexpr const float (&W)[RANK][LEN] = mf::G0_W;
+};
+template <>
+struct Filt<3> {
+  static constexpr int LEN = mf::G1_LEN, RANK = mf::G1_RANK;
+  static constexpr const float (&V)[RANK][LEN] = mf::G1_V;
+  static constexpr const float (&W)[RANK][LEN] = mf::G1_W;
+};
 
 // Vertical pass of one filter component over a strip of VS rows.
 template <int LEN, int VS>
@@ -70,84 +104,103 @@ __device__ __forceinline__ void hpass(const float* __restrict__ row, const float
   }
 }
 
-// Load a (TH+2H) x (TW+2H) sub-lattice tile with symmetric folding.
-template <int TH, int TW, int T>
+// Load a (TH+2H) x (TW+2H) sub-lattice tile centred on input rows R0.. and
+// columns C0..: symmetric fold (forward) or zero padding (adjoint).
+template <int TH, int TW, int T, bool ZERO>
 __device__ __forceinline__ void load_tile(float* __restrict__ tile, const float* __restrict__ x, int L, int m, int pr,
                                           int pc, int R0, int C0, float th, float scale) {
   constexpr int W = TW + 2 * HALO;
   constexpr int NT = (TH + 2 * HALO) * W;
   for (int idx = threadIdx.x; idx < NT; idx += T) {
     const int rr = idx / W, cc = idx - rr * W;
-    const int gr = fold_index(pr + m * (R0 - HALO + rr), L);
-    const int gc = fold_index(pc + m * (C0 - HALO + cc), L);
-    float v = x[(size_t)gr * L + gc];
-    if (th >= 0.f) v = hard_keep(v, th);
+    int gr = pr + m * (R0 - HALO + rr);
+    int gc = pc + m * (C0 - HALO + cc);
+    float v;
+    if constexpr (ZERO) {
+      v = (gr >= 0 && gr < L && gc >= 0 && gc < L) ? x[(size_t)gr * L + gc] : 0.0f;
+    } else {
+      gr = fold_index(gr, L);
+      gc = fold_index(gc, L);
+      v = x[(size_t)gr * L + gc];
+      if (th >= 0.f) v = hard_keep(v, th);
+    }
     tile[idx] = v * scale;
   }
 }
 
-// ---------------------------------------------------------------------------
-// Decomposition level: one input, two outputs (h0 lowpass, h1 highpass).
-// grid: (ceil(L/m/TW), ceil(L/m/TH), m*m)
-template <int TH, int TW, int VS, int HS, int T>
-__global__ void __launch_bounds__(T) k_atrous_dec(const float* __restrict__ x, float* __restrict__ ylow,
-                                                   float* __restrict__ yhigh, ImgStrides st3, int L, int m) {
-  constexpr int W = TW + 2 * HALO;
-  constexpr int WP = W | 1;  // odd row stride for the intermediates
-  constexpr int R0N = mf::H0_RANK, R1N = mf::H1_RANK;
-  static_assert(mf::H0_LEN == 13 && mf::H1_LEN == 19, "maxflat h0/h1 sizes");
-  __shared__ float tile[(TH + 2 * HALO) * W];
-  __shared__ float tv[R0N + R1N][TH * WP];
-  const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
-  const int pr = ph / m, pc = ph - pr * m;
-  x += (size_t)img * st3.a;
-  ylow += (size_t)img * st3.b;
-  yhigh += (size_t)img * st3.c;
-  const int nsub = L / m;
-  const int R0 = blockIdx.y * TH, C0 = blockIdx.x * TW;
-  load_tile<TH, TW, T>(tile, x, L, m, pr, pc, R0, C0, -1.0f, 1.0f);
-  __syncthreads();
-  // vertical passes: items = W columns x TH/VS strips
-  for (int it = threadIdx.x; it < W * (TH / VS); it += T) {
-    const int cc = it % W, st = it / W;
+// Vertical passes of filter F into tv[k0 .. k0+RANK) for all tile columns.
+template <int F, int TH, int W, int WP, int VS, int T, int NTV>
+__device__ __forceinline__ void vpasses(const float* __restrict__ tile, float (*tv)[TH * WP], int k0, int it0,
+                                        int nit) {
+  using Fi = Filt<F>;
+  for (int it = threadIdx.x + it0; it < it0 + nit; it += T) {
+    const int rel = it - it0;
+    const int cc = rel % W, st = rel / W;
     float col[VS + 2 * HALO];
 #pragma unroll
     for (int i = 0; i < VS + 2 * HALO; ++i) col[i] = tile[(st * VS + i) * W + cc];
     float t[VS];
 #pragma unroll
-    for (int k = 0; k < R0N; ++k) {
-      vpass<13, VS>(col, mf::H0_V[k], t);
+    for (int k = 0; k < Fi::RANK; ++k) {
+      vpass<Fi::LEN, VS>(col, Fi::V[k], t);
 #pragma unroll
-      for (int i = 0; i < VS; ++i) tv[k][(st * VS + i) * WP + cc] = t[i];
-    }
-#pragma unroll
-    for (int k = 0; k < R1N; ++k) {
-      vpass<19, VS>(col, mf::H1_V[k], t);
-#pragma unroll
-      for (int i = 0; i < VS; ++i) tv[R0N + k][(st * VS + i) * WP + cc] = t[i];
+      for (int i = 0; i < VS; ++i) tv[k0 + k][(st * VS + i) * WP + cc] = t[i];
     }
   }
+}
+
+// Output grid of a level: the image (forward) or the extended grid of the
+// adjoint (EXT), E = L + 2*HALO*m per side, in sub-lattice rows of `m`.
+__device__ __forceinline__ int out_side(int L, int m, bool ext) { return ext ? L + 2 * HALO * m : L; }
+
+// ---------------------------------------------------------------------------
+// One input, two outputs: ya = FA (x) x, yb = FB (x) x.
+//   forward decomposition level: FA = h0, FB = h1
+//   adjoint of a reconstruction level (ADJ): FA = g0, FB = g1 on the
+//   zero-padded gradient, outputs on the extended grid.
+// grid: (ceil(nout/TW), ceil(nout/TH), m*m*nimg), nout = out_side/m.
+template <int FA, int FB, bool ADJ, int TH, int TW, int VS, int HS, int T>
+__global__ void __launch_bounds__(T) k_pyr_one2two(const float* __restrict__ x, float* __restrict__ ya,
+                                                    float* __restrict__ yb, ImgStrides st3, int L, int m) {
+  using A = Filt<FA>;
+  using B = Filt<FB>;
+  constexpr int W = TW + 2 * HALO;
+  constexpr int WP = W | 1;  // odd row stride for the intermediates
+  __shared__ float tile[(TH + 2 * HALO) * W];
+  __shared__ float tv[A::RANK + B::RANK][TH * WP];
+  const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
+  const int pr = ph / m, pc = ph - pr * m;
+  x += (size_t)img * st3.a;
+  ya += (size_t)img * st3.b;
+  yb += (size_t)img * st3.c;
+  const int E = out_side(L, m, ADJ);
+  const int nout = E / m;
+  const int R0 = blockIdx.y * TH, C0 = blockIdx.x * TW;
+  const int shift = ADJ ? HALO : 0;  // extended row R' sits on input row R' - HALO
+  load_tile<TH, TW, T, ADJ>(tile, x, L, m, pr, pc, R0 - shift, C0 - shift, -1.0f, 1.0f);
   __syncthreads();
-  // horizontal passes: items = TH rows x TW/HS strips, lanes along rows
+  vpasses<FA, TH, W, WP, VS, T, 0>(tile, tv, 0, 0, W * (TH / VS));
+  vpasses<FB, TH, W, WP, VS, T, 0>(tile, tv, A::RANK, 0, W * (TH / VS));
+  __syncthreads();
   for (int it = threadIdx.x; it < TH * (TW / HS); it += T) {
     const int r = it % TH, st = it / TH;
     float a0[HS], a1[HS];
 #pragma unroll
     for (int j = 0; j < HS; ++j) a0[j] = a1[j] = 0.f;
 #pragma unroll
-    for (int k = 0; k < R0N; ++k) hpass<13, HS>(&tv[k][r * WP + st * HS], mf::H0_W[k], a0);
+    for (int k = 0; k < A::RANK; ++k) hpass<A::LEN, HS>(&tv[k][r * WP + st * HS], A::W[k], a0);
 #pragma unroll
-    for (int k = 0; k < R1N; ++k) hpass<19, HS>(&tv[R0N + k][r * WP + st * HS], mf::H1_W[k], a1);
+    for (int k = 0; k < B::RANK; ++k) hpass<B::LEN, HS>(&tv[A::RANK + k][r * WP + st * HS], B::W[k], a1);
     const int R = R0 + r;
-    if (R < nsub) {
-      const size_t rowoff = (size_t)(pr + m * R) * L;
+    if (R < nout) {
+      const size_t rowoff = (size_t)(pr + m * R) * E;
 #pragma unroll
       for (int j = 0; j < HS; ++j) {
-        const int C = C0 + st * HS + j;
-        if (C < nsub) {
-          const size_t o = rowoff + pc + (size_t)m * C;
-          ylow[o] = a0[j];
-          yhigh[o] = a1[j];
+        const int Cc = C0 + st * HS + j;
+        if (Cc < nout) {
+          const size_t o = rowoff + pc + (size_t)m * Cc;
+          ya[o] = a0[j];
+          yb[o] = a1[j];
         }
       }
     }
@@ -155,49 +208,56 @@ __global__ void __launch_bounds__(T) k_atrous_dec(const float* __restrict__ x, f
 }
 
 // ---------------------------------------------------------------------------
-// Reconstruction level: out = g0 * (sx * keep(xlow, th)) + g1 * (sy * yhigh).
-template <int TH, int TW, int VS, int HS, int T>
-__global__ void __launch_bounds__(T) k_atrous_rec(const float* __restrict__ xlow, const float* __restrict__ yhigh,
-                                                   float* __restrict__ out, ImgStrides st3, int L, int m, LowThresh lt,
-                                                   float sx, float sy) {
+// Two inputs, one output: out = FA (x) (sa * keep(xa, th)) + FB (x) (sb * xb).
+//   forward reconstruction level: FA = g0, FB = g1
+//   adjoint of a decomposition level (ADJ): FA = h0, FB = h1 on the
+//   zero-padded gradients, output on the extended grid.
+template <int FA, int FB, bool ADJ, int TH, int TW, int VS, int HS, int T>
+__global__ void __launch_bounds__(T) k_pyr_two2one(const float* __restrict__ xa, const float* __restrict__ xb,
+                                                    float* __restrict__ out, ImgStrides st3, int L, int m,
+                                                    LowThresh lt, float sa, float sb) {
+  using A = Filt<FA>;
+  using B = Filt<FB>;
   constexpr int W = TW + 2 * HALO;
   constexpr int WP = W | 1;
-  constexpr int RGN = mf::G0_RANK, RHN = mf::G1_RANK;
-  static_assert(mf::G0_LEN == 19 && mf::G1_LEN == 13, "maxflat g0/g1 sizes");
   __shared__ float tile[2][(TH + 2 * HALO) * W];
-  __shared__ float tv[RGN + RHN][TH * WP];
+  __shared__ float tv[A::RANK + B::RANK][TH * WP];
   const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
   const int pr = ph / m, pc = ph - pr * m;
-  xlow += (size_t)img * st3.a;
-  yhigh += (size_t)img * st3.b;
+  xa += (size_t)img * st3.a;
+  xb += (size_t)img * st3.b;
   out += (size_t)img * st3.c;
-  const float th_x = lt.mode == 0 ? -1.0f : (lt.mode == 1 ? lt.v : lt.dev[(size_t)img * lt.stride]);
-  const int nsub = L / m;
+  const float th = lt.mode == 0 ? -1.0f : (lt.mode == 1 ? lt.v : lt.dev[(size_t)img * lt.stride]);
+  const int E = out_side(L, m, ADJ);
+  const int nout = E / m;
   const int R0 = blockIdx.y * TH, C0 = blockIdx.x * TW;
-  load_tile<TH, TW, T>(tile[0], xlow, L, m, pr, pc, R0, C0, th_x, sx);
-  load_tile<TH, TW, T>(tile[1], yhigh, L, m, pr, pc, R0, C0, -1.0f, sy);
+  const int shift = ADJ ? HALO : 0;
+  load_tile<TH, TW, T, ADJ>(tile[0], xa, L, m, pr, pc, R0 - shift, C0 - shift, th, sa);
+  load_tile<TH, TW, T, ADJ>(tile[1], xb, L, m, pr, pc, R0 - shift, C0 - shift, -1.0f, sb);
   __syncthreads();
-  for (int it = threadIdx.x; it < 2 * W * (TH / VS); it += T) {
-    const int which = it / (W * (TH / VS));
-    const int rem = it - which * W * (TH / VS);
-    const int cc = rem % W, st = rem / W;
+  constexpr int NIT = W * (TH / VS);
+  // the two vertical passes run on disjoint item ranges of one loop each
+  for (int it = threadIdx.x; it < 2 * NIT; it += T) {
+    const int which = it / NIT;
+    const int rel = it - which * NIT;
+    const int cc = rel % W, st = rel / W;
     float col[VS + 2 * HALO];
 #pragma unroll
     for (int i = 0; i < VS + 2 * HALO; ++i) col[i] = tile[which][(st * VS + i) * W + cc];
     float t[VS];
     if (which == 0) {
 #pragma unroll
-      for (int k = 0; k < RGN; ++k) {
-        vpass<19, VS>(col, mf::G0_V[k], t);
+      for (int k = 0; k < A::RANK; ++k) {
+        vpass<A::LEN, VS>(col, A::V[k], t);
 #pragma unroll
         for (int i = 0; i < VS; ++i) tv[k][(st * VS + i) * WP + cc] = t[i];
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < RHN; ++k) {
-        vpass<13, VS>(col, mf::G1_V[k], t);
+      for (int k = 0; k < B::RANK; ++k) {
+        vpass<B::LEN, VS>(col, B::V[k], t);
 #pragma unroll
-        for (int i = 0; i < VS; ++i) tv[RGN + k][(st * VS + i) * WP + cc] = t[i];
+        for (int i = 0; i < VS; ++i) tv[A::RANK + k][(st * VS + i) * WP + cc] = t[i];
       }
     }
   }
@@ -208,19 +268,44 @@ __global__ void __launch_bounds__(T) k_atrous_rec(const float* __restrict__ xlow
 #pragma unroll
     for (int j = 0; j < HS; ++j) a[j] = 0.f;
 #pragma unroll
-    for (int k = 0; k < RGN; ++k) hpass<19, HS>(&tv[k][r * WP + st * HS], mf::G0_W[k], a);
+    for (int k = 0; k < A::RANK; ++k) hpass<A::LEN, HS>(&tv[k][r * WP + st * HS], A::W[k], a);
 #pragma unroll
-    for (int k = 0; k < RHN; ++k) hpass<13, HS>(&tv[RGN + k][r * WP + st * HS], mf::G1_W[k], a);
+    for (int k = 0; k < B::RANK; ++k) hpass<B::LEN, HS>(&tv[A::RANK + k][r * WP + st * HS], B::W[k], a);
     const int R = R0 + r;
-    if (R < nsub) {
-      const size_t rowoff = (size_t)(pr + m * R) * L;
+    if (R < nout) {
+      const size_t rowoff = (size_t)(pr + m * R) * E;
 #pragma unroll
       for (int j = 0; j < HS; ++j) {
-        const int C = C0 + st * HS + j;
-        if (C < nsub) out[rowoff + pc + (size_t)m * C] = a[j];
+        const int Cc = C0 + st * HS + j;
+        if (Cc < nout) out[rowoff + pc + (size_t)m * Cc] = a[j];
       }
     }
   }
+}
+
+// Transpose of the symmetric extension: out[s] = sum of Z[t'] over the
+// extended grid points t' = t + H (H = HALO*m) with fold(t) = s, per axis.
+__global__ void k_fold_add(const float* __restrict__ Z, float* __restrict__ out, int L, int H, long long zs,
+                           long long os) {
+  const int img = blockIdx.z;
+  const int sc = blockIdx.x * blockDim.x + threadIdx.x, sr = blockIdx.y;
+  if (sc >= L) return;
+  const int E = L + 2 * H;
+  const float* z = Z + (size_t)img * zs;
+  int pre_r[8], pre_c[8], nr = 0, nc = 0;
+  // preimages of s in [-H, L+H): t = s + 2Lk or t = -s-1 + 2Lk
+  for (int k = -(H / (2 * L)) - 1; k <= H / (2 * L) + 1; ++k) {
+    const int t1 = sr + 2 * L * k, t2 = -sr - 1 + 2 * L * k;
+    if (t1 >= -H && t1 < L + H && nr < 8) pre_r[nr++] = t1 + H;
+    if (t2 >= -H && t2 < L + H && nr < 8) pre_r[nr++] = t2 + H;
+    const int u1 = sc + 2 * L * k, u2 = -sc - 1 + 2 * L * k;
+    if (u1 >= -H && u1 < L + H && nc < 8) pre_c[nc++] = u1 + H;
+    if (u2 >= -H && u2 < L + H && nc < 8) pre_c[nc++] = u2 + H;
+  }
+  float acc = 0.f;
+  for (int i = 0; i < nr; ++i)
+    for (int j = 0; j < nc; ++j) acc += z[(size_t)pre_r[i] * E + pre_c[j]];
+  out[(size_t)img * os + (size_t)sr * L + sc] = acc;
 }
 
 // Tile variants (sub-lattice tile TH x TW, vertical/horizontal strips VS/HS)
@@ -240,48 +325,82 @@ struct PyrCfg<2> {
   static constexpr int TH = 32, TW = 64, VS = 8, HS = 8, T = 256;
 };
 
-template <int V>
-static hipError_t dec_v(const float* x, float* ylow, float* yhigh, const ImgStrides& st, int nimg, int L, int m,
-                        hipStream_t s) {
+template <int FA, int FB, bool ADJ, int V>
+static hipError_t one2two_v(const float* x, float* ya, float* yb, const ImgStrides& st, int nimg, int L, int m,
+                            hipStream_t s) {
   using C = PyrCfg<V>;
-  const int nsub = L / m;
-  dim3 grid((nsub + C::TW - 1) / C::TW, (nsub + C::TH - 1) / C::TH, m * m * nimg);
-  hipLaunchKernelGGL((k_atrous_dec<C::TH, C::TW, C::VS, C::HS, C::T>), grid, dim3(C::T), 0, s, x, ylow, yhigh, st, L,
-                     m);
+  const int nout = (ADJ ? L + 2 * HALO * m : L) / m;
+  dim3 grid((nout + C::TW - 1) / C::TW, (nout + C::TH - 1) / C::TH, m * m * nimg);
+  hipLaunchKernelGGL((k_pyr_one2two<FA, FB, ADJ, C::TH, C::TW, C::VS, C::HS, C::T>), grid, dim3(C::T), 0, s, x, ya,
+                     yb, st, L, m);
   return hipGetLastError();
 }
 
-template <int V>
-static hipError_t rec_v(const float* xlow, const float* yhigh, float* out, const ImgStrides& st, int nimg, int L,
-                        int m, const LowThresh& lt, float sx, float sy, hipStream_t s) {
+template <int FA, int FB, bool ADJ, int V>
+static hipError_t two2one_v(const float* xa, const float* xb, float* out, const ImgStrides& st, int nimg, int L,
+                            int m, const LowThresh& lt, float sa, float sb, hipStream_t s) {
   using C = PyrCfg<V>;
-  const int nsub = L / m;
-  dim3 grid((nsub + C::TW - 1) / C::TW, (nsub + C::TH - 1) / C::TH, m * m * nimg);
-  hipLaunchKernelGGL((k_atrous_rec<C::TH, C::TW, C::VS, C::HS, C::T>), grid, dim3(C::T), 0, s, xlow, yhigh, out, st,
-                     L, m, lt, sx, sy);
+  const int nout = (ADJ ? L + 2 * HALO * m : L) / m;
+  dim3 grid((nout + C::TW - 1) / C::TW, (nout + C::TH - 1) / C::TH, m * m * nimg);
+  hipLaunchKernelGGL((k_pyr_two2one<FA, FB, ADJ, C::TH, C::TW, C::VS, C::HS, C::T>), grid, dim3(C::T), 0, s, xa, xb,
+                     out, st, L, m, lt, sa, sb);
   return hipGetLastError();
 }
+
+static bool level_ok(int L, int m, int nimg) { return m >= 1 && L % m == 0 && nimg >= 1 && m * m * nimg <= 65535; }
 
 hipError_t launch_atrous_dec(const float* x, float* ylow, float* yhigh, const ImgStrides& st, int nimg, int L, int m,
                              hipStream_t s, int variant) {
-  if (m < 1 || L % m != 0 || nimg < 1 || m * m * nimg > 65535) return hipErrorInvalidValue;
+  if (!level_ok(L, m, nimg)) return hipErrorInvalidValue;
   switch (variant) {
-    case 0: return dec_v<0>(x, ylow, yhigh, st, nimg, L, m, s);
-    case 1: return dec_v<1>(x, ylow, yhigh, st, nimg, L, m, s);
-    case 2: return dec_v<2>(x, ylow, yhigh, st, nimg, L, m, s);
+    case 0: return one2two_v<0, 1, false, 0>(x, ylow, yhigh, st, nimg, L, m, s);
+    case 1: return one2two_v<0, 1, false, 1>(x, ylow, yhigh, st, nimg, L, m, s);
+    case 2: return one2two_v<0, 1, false, 2>(x, ylow, yhigh, st, nimg, L, m, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_atrous_rec(const float* xlow, const float* yhigh, float* out, const ImgStrides& st, int nimg, int L,
                              int m, const LowThresh& lt, float sx, float sy, hipStream_t s, int variant) {
-  if (m < 1 || L % m != 0 || nimg < 1 || m * m * nimg > 65535) return hipErrorInvalidValue;
+  if (!level_ok(L, m, nimg)) return hipErrorInvalidValue;
   switch (variant) {
-    case 0: return rec_v<0>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
-    case 1: return rec_v<1>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
-    case 2: return rec_v<2>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
+    case 0: return two2one_v<2, 3, false, 0>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
+    case 1: return two2one_v<2, 3, false, 1>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
+    case 2: return two2one_v<2, 3, false, 2>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+int adjoint_ext_side(int L, int m) { return L + 2 * HALO * m; }
+
+hipError_t launch_atrous_dec_adjoint(const float* glow, const float* ghigh, float* gx, float* zext,
+                                     const ImgStrides& st, long long zstride, int nimg, int L, int m, hipStream_t s) {
+  if (!level_ok(L, m, nimg)) return hipErrorInvalidValue;
+  // Z = C_h0^T glow + C_h1^T ghigh on the extended grid, then fold-add
+  ImgStrides sz{st.a, st.b, zstride};
+  hipError_t e = two2one_v<0, 1, true, 0>(glow, ghigh, zext, sz, nimg, L, m, LowThresh{0, 0.f, nullptr, 0}, 1.0f,
+                                           1.0f, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fold_add, dim3((L + 255) / 256, L, nimg), dim3(256), 0, s, zext, gx, L, HALO * m, zstride,
+                     st.c);
+  return hipGetLastError();
+}
+
+hipError_t launch_atrous_rec_adjoint(const float* gout, float* glow, float* ghigh, float* zext, const ImgStrides& st,
+                                     long long zstride, int nimg, int L, int m, hipStream_t s) {
+  if (!level_ok(L, m, nimg)) return hipErrorInvalidValue;
+  // Za = C_g0^T gout, Zb = C_g1^T gout on the extended grid, then fold-add each
+  const long long E = adjoint_ext_side(L, m);
+  float* za = zext;
+  float* zb = zext + E * E;
+  ImgStrides sz{st.a, zstride, zstride};
+  hipError_t e = one2two_v<2, 3, true, 0>(gout, za, zb, sz, nimg, L, m, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fold_add, dim3((L + 255) / 256, L, nimg), dim3(256), 0, s, za, glow, L, HALO * m, zstride,
+                     st.b);
+  hipLaunchKernelGGL(k_fold_add, dim3((L + 255) / 256, L, nimg), dim3(256), 0, s, zb, ghigh, L, HALO * m, zstride,
+                     st.c);
+  return hipGetLastError();
 }
 
 }  // namespace shx
