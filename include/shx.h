@@ -199,6 +199,19 @@ int shx_mrdwt_adjoint(const float* yl_bar, const float* yh_bar, int m, int n, co
 int shx_mirdwt_adjoint(const float* x_bar, int m, int n, const double* h, int lh, int L, float* yl_bar,
                        float* yh_bar, float* work, int nimg, void* stream);
 
+/* Soft threshold with one threshold per band (th_dev, device FP32) over a
+ * batched layout: element e of each image (img_elems elements) belongs to
+ * band band_off + ((e mod rowlen) / seglen).  NSST stacks: rowlen =
+ * img_elems, seglen = L*L; wavelet yh: rowlen = 3nL, seglen = n, band_off 1.
+ * y = sign(x) max(|x| - t, 0) (shearcuda/SoftThresh.m:11-13, per band).
+ * Backward: gx = gy * 1[|x| > t] (gx may be NULL) and
+ * gth[band] += sum -sign(x) 1[|x| > t] gy (gth not cleared).  No
+ * counterpart in the reference (learned thresholds). */
+int shx_soft_bands(const float* x, float* y, int nimg, int64_t img_elems, int64_t rowlen, int64_t seglen,
+                   int band_off, const float* th_dev, void* stream);
+int shx_soft_bands_backward(const float* x, const float* gy, float* gx, float* gth_dev, int nimg, int64_t img_elems,
+                            int64_t rowlen, int64_t seglen, int band_off, const float* th_dev, void* stream);
+
 /* Sum |x|^p over n elements, result (p-th root not taken) in *out (host).
  * norm_cuda's reduction (shearcuda/src_cuda/norm_cuda.cpp:172-215). */
 int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream);

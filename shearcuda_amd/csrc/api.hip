@@ -133,6 +133,54 @@ __global__ void k_norm_pow(const float* x, long long n, float p, double* out) {
   if ((threadIdx.x & 63) == 0) atomicAdd(out, d);
 }
 
+// Per-band soft threshold over batched coefficient layouts: element e of
+// image i has band band_off + ((e mod rowlen) / seglen) (planes: rowlen =
+// image elements, seglen = plane; wavelet yh rows: rowlen = 3nL, seglen = n).
+struct BandLayout {
+  long long img_elems, rowlen, seglen;
+  int band_off;
+};
+
+__device__ __forceinline__ int band_of(const BandLayout& b, long long e) {
+  const long long r = e % b.img_elems;
+  return b.band_off + (int)((r % b.rowlen) / b.seglen);
+}
+
+__global__ void k_soft_bands(const float* __restrict__ x, float* __restrict__ y, long long n, BandLayout bl,
+                             const float* __restrict__ th) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    const float v = x[e];
+    const float a = fabsf(v) - th[band_of(bl, e)];
+    y[e] = a > 0.f ? copysignf(a, v) : 0.f;
+  }
+}
+
+// gx = gy * 1[|x| > t];  gth[band] += sum -sign(x) * 1[|x| > t] * gy
+__global__ void k_soft_bands_bwd(const float* __restrict__ x, const float* __restrict__ gy, float* __restrict__ gx,
+                                 float* __restrict__ gth, long long n, BandLayout bl, const float* __restrict__ th) {
+  for (long long e0 = (long long)blockIdx.x * blockDim.x; e0 < n; e0 += (long long)gridDim.x * blockDim.x) {
+    const long long e = e0 + threadIdx.x;
+    int band = -1;
+    float c = 0.f;
+    if (e < n) {
+      band = band_of(bl, e);
+      const float v = x[e], g = gy[e];
+      const bool keep = fabsf(v) > th[band];
+      if (gx) gx[e] = keep ? g : 0.f;
+      c = keep ? (v > 0.f ? -g : g) : 0.f;
+    }
+    // one atomic per wave when every active lane is in the same band
+    const int b0 = __shfl(band, 0);
+    const bool uniform = __all(band == b0 || band < 0);
+    if (uniform) {
+      for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+      if ((threadIdx.x & 63) == 0 && b0 >= 0) atomicAdd(&gth[b0], c);
+    } else if (band >= 0 && c != 0.f) {
+      atomicAdd(&gth[band], c);
+    }
+  }
+}
+
 hipError_t pyr_dec(const float* x, int L, int nlev, float* const* y, float* tA, float* tB, hipStream_t s) {
   const float* in = x;
   for (int lev = 0; lev < nlev; ++lev) {
@@ -1113,6 +1161,32 @@ int shx_re_weight_band(const float* x, int L, const double* coeff, int ncoeff, f
 }
 
 }  // extern "C"
+
+int shx_soft_bands(const float* x, float* y, int nimg, int64_t img_elems, int64_t rowlen, int64_t seglen,
+                   int band_off, const float* th_dev, void* stream) {
+  if (!x || !y || !th_dev) return fail(SHX_ERR_ARG, "null input, output or thresholds");
+  if (nimg < 1 || img_elems < 1 || rowlen < 1 || seglen < 1 || img_elems % rowlen || rowlen % seglen)
+    return fail(SHX_ERR_ARG, "invalid band layout");
+  const long long n = (long long)nimg * img_elems;
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(8192, (n + 255) / 256));
+  hipLaunchKernelGGL(k_soft_bands, dim3(blocks), dim3(256), 0, (hipStream_t)stream, x, y, n,
+                     BandLayout{img_elems, rowlen, seglen, band_off}, th_dev);
+  SHX_HIP(hipGetLastError());
+  return SHX_OK;
+}
+
+int shx_soft_bands_backward(const float* x, const float* gy, float* gx, float* gth_dev, int nimg, int64_t img_elems,
+                            int64_t rowlen, int64_t seglen, int band_off, const float* th_dev, void* stream) {
+  if (!x || !gy || !gth_dev || !th_dev) return fail(SHX_ERR_ARG, "null input, gradient or thresholds");
+  if (nimg < 1 || img_elems < 1 || rowlen < 1 || seglen < 1 || img_elems % rowlen || rowlen % seglen)
+    return fail(SHX_ERR_ARG, "invalid band layout");
+  const long long n = (long long)nimg * img_elems;
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(8192, (n + 255) / 256));
+  hipLaunchKernelGGL(k_soft_bands_bwd, dim3(blocks), dim3(256), 0, (hipStream_t)stream, x, gy, gx, gth_dev, n,
+                     BandLayout{img_elems, rowlen, seglen, band_off}, th_dev);
+  SHX_HIP(hipGetLastError());
+  return SHX_OK;
+}
 
 // ---- undecimated wavelet transform ---------------------------------------------
 namespace {
