@@ -165,6 +165,106 @@ def bench_batched(steps, warmup, mix=BATCH_MIX):
             "schedules": {str(p["L"]): p["sched"] for p in plans}}
 
 
+def bench_adjoint(iters=10, nimg=8, L=512, reps=3):
+    """Unrolled separation, 8 x 512^2 images: ms per unrolled iteration of the
+    forward pass and of forward + backward (adjoint kernels)."""
+    import numpy as np
+    import torch
+
+    from shearcuda_amd import api, separation
+    from shearcuda_amd.synth import separation_input, white_noise
+    from shearcuda_amd.wavelets import MakeONFilter
+    sigma, levels = 10.0, 3
+    h = MakeONFilter("Daubechies", 4)
+    shear = api.make_dictionary(L)
+    plan = api.Batch(shear, nimg)
+    api.autotune(shear, plan)
+    noise = torch.from_numpy(white_noise(L).astype(np.float32)).cuda()
+    s_sh = api.band_thresholds(shear, 1.0, api.nsst_scalars_cuda(shear, noise), (1.0, 1.0, 1.0))
+    s_w = separation.wavelet_scalars(noise, h, levels)
+    data = [separation_input(L, seed=300 + i, sigma=sigma) for i in range(nimg)]
+    x = torch.from_numpy(np.stack([d[2] for d in data]).astype(np.float32)).cuda()
+    gt = torch.from_numpy(np.stack([d[1] for d in data]).astype(np.float32)).cuda()
+    th_sh = torch.tensor(3 * sigma * s_sh, dtype=torch.float32, device="cuda", requires_grad=True)
+    th_w = torch.tensor(3 * sigma * s_w, dtype=torch.float32, device="cuda", requires_grad=True)
+
+    def fwd():
+        with torch.no_grad():
+            separation.unrolled_separation(x, th_sh, th_w, plan, h, levels, iters)
+
+    def fwd_bwd():
+        xr = x.clone().requires_grad_(True)
+        c, _ = separation.unrolled_separation(xr, th_sh, th_w, plan, h, levels, iters)
+        torch.mean((c - gt) ** 2).backward()
+
+    t_f = _timed(fwd, reps, 1)
+    t_fb = _timed(fwd_bwd, reps, 1)
+    plan.close()
+    shear.close()
+    return {"workload": "unrolled shearlet/wavelet soft-threshold separation, %d x %dx%d FP32, %d iterations, "
+                        "NSST [8,8,16,16] + 3-level D4, per-band learnable thresholds" % (nimg, L, L, iters),
+            "forward_ms_per_iter": round(t_f * 1e3 / reps / iters, 3),
+            "fwd_bwd_ms_per_iter": round(t_fb * 1e3 / reps / iters, 3)}
+
+
+VOL_DIRS = (8, 8, 4)
+
+
+def vol_bytes_model(sides, band_first):
+    """Band-stack byte model of one 3D volume: input and output once, every
+    band written once and read once (4 B per voxel each)."""
+    n = sides[0]
+    b = 8.0 * n ** 3
+    for l in range(len(band_first) - 1):
+        b += 8.0 * (band_first[l + 1] - band_first[l]) * sides[l] ** 3
+    return b
+
+
+def bench_3d(steps=5, warmup=1, nvol=4, n=128, sigma=25.0, group=8):
+    """3D shearlet volume denoise of a batch of 4 x 128^3 FP32 volumes
+    (8 random filled ellipsoids + N(0, 25^2) each) as one graph launch:
+    Fourier band-pass, 3 levels (Pyr_mode 1), 3 cones with 8x8 / 8x8 / 4x4
+    directions, hard threshold 3 sigma bandEst per band, reconstruction."""
+    import numpy as np
+    import torch
+
+    from shearcuda_amd import api, shearlet3d
+    from shearcuda_amd.synth import noisy_ellipsoids, psnr, white_noise_volume
+    t0 = time.perf_counter()
+    sh = shearlet3d.Shear3D(n, VOL_DIRS, 1)
+    prep_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    est, _ = shearlet3d.nsst_scalars_3d(sh, torch.tensor(white_noise_volume(n), dtype=torch.float32, device="cuda"))
+    scalars_s = time.perf_counter() - t0
+    pairs = [noisy_ellipsoids(n, sigma=sigma, seed=700 + i) for i in range(nvol)]
+    x = torch.tensor(np.stack([p[1] for p in pairs]), dtype=torch.float32, device="cuda")
+    th = shearlet3d.thresholds_3d(sh, est, [sigma] * nvol, (3.0, 3.0, 3.0))
+    plan = shearlet3d.Plan3D(sh, nvol, group=group)
+    out = torch.empty_like(x)
+    g = api.capture(lambda: plan.denoise(x, th, out))
+    el = _timed(g.launch, steps, warmup)
+    ms = el * 1e3 / steps
+    res = out.cpu().numpy()
+    vox = nvol * n ** 3
+    byt = vol_bytes_model(sh.sides, sh.band_first) * nvol
+    info = {"workload": "3D shearlet volume denoise, %d x %d^3 FP32 per launch, Fourier band-pass 3 levels "
+                        "(Pyr_mode 1: %s), 3 cones x %s directions (%d bands of 24^3 filters), FFT grids %s, "
+                        "T=3 sigma bandEst, sigma=%g, one hipGraph launch" % (
+                            nvol, n, sh.sides, "/".join("%dx%d" % (d, d) for d in VOL_DIRS), sh.nbands,
+                            sh.fft_sides, sigma),
+            "ms_per_launch": round(ms, 3), "Mvoxel_s": round(vox / (ms * 1e-3) / 1e6, 2),
+            "bytes_model_per_launch": byt,
+            "path_roofline_frac": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "psnr_in": [round(psnr(p[0], p[1]), 3) for p in pairs],
+            "psnr_out": [round(psnr(p[0], res[i]), 3) for i, p in enumerate(pairs)],
+            "band_group": group, "dict_bytes": sh.nbytes(), "plan_bytes": plan.nbytes(),
+            "dict_prep_s": round(prep_s, 2), "noise_scalars_s": round(scalars_s, 2)}
+    g.close()
+    plan.close()
+    sh.close()
+    return info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +274,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle work (0 = skip)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-plain-1024", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -219,6 +321,10 @@ def main():
                              "path_roofline_frac": round(400.0 * 1024 * 1024 / (ms1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if not args.no_batched:
         res["batched"] = bench_batched(max(5, args.steps // 20), 2)
+    if not args.no_adjoint:
+        res["adjoint"] = bench_adjoint()
+    if not args.no_3d:
+        res["volume"] = bench_3d()
     if args.cpu_budget > 0:
         res["cpu_baseline"] = cpu_baseline(L, sigma, args.cpu_budget)
     print(json.dumps(res), flush=True)

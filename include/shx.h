@@ -226,6 +226,70 @@ double shx_maxflat_factor_error(void);
  * len*len doubles, len returned); out may be NULL to query len. */
 int shx_maxflat_filter(int which, double* out, int* len);
 
+/* ---- 3D shearlet volume transform -------------------------------------------
+ * Volumes are n^3 FP32 arrays on the device; a volume and the filters applied
+ * to it share one memory order (the transform is the same for any order).
+ * Batches of nvol volumes are contiguous.
+ *
+ * Subband block ("subs", per volume shx3_subs_elems() floats): the band-pass
+ * subbands of the Fourier-domain pyramid, finest first, each side_l^3, then
+ * the lowpass side_L^3 (the cell L of DoPyrDec,
+ * shearcuda3d/3DShearTrans/DoPyrDec.m:1-6 -> 3DBP/PyrNDDec_mm.m:48-109).
+ * Band block ("bands", per volume shx3_bands_elems() floats): the shearlet
+ * bands of every level, level-major; within a level cone-major (cones
+ * 1, 2, 3 of GetMeyerBasedFilter's output), then the MATLAB linear
+ * (column-major) index over the (l2, l1) direction cell; each side_l^3.
+ * Thresholds th: device FP32 [nvol][nbands] in band order, a band kept where
+ * |c| > th (shearcuda3d/DenoiseDemo/ThresholdShCoeff.m:7 zeroes |c| <= th).
+ */
+#define SHX3_MAX_LEVELS 6
+typedef struct shx3_dict shx3_dict;
+typedef struct shx3_plan shx3_plan;
+
+/* Dictionary for volumes of side n: nlevels band-pass levels with
+ * Pyr_mode 1, 1.5 or 2 (3DBP/PyrNDDec_mm.m:52-70), ndir[l] directions per
+ * axis at level l (3 * ndir[l]^2 bands), and the spatial cone filters of each
+ * level (host FP32, 3*ndir[l]^2 filters of fsize^3 in band order), the output
+ * of shearcuda3d/3DShearTrans/GetMeyerBasedFilter.m:12-151.  Levels passing
+ * the same filter pointer with the same direction count share spectra.
+ * Supported: band-pass sides in {8..128} (powers of two), fsize a multiple of
+ * 8 up to 64.  Replaces the per-call filter padding and R2C FFT of
+ * ShDecThRec_cuda.cpp:282-296. */
+int shx3_prepare(int n, int nlevels, double pyr_mode, const int* ndir, int fsize, const float* const* filters,
+                 shx3_dict** out);
+int shx3_destroy(shx3_dict* d);
+/* sides[nlevels+1] (subband sides, lowpass last), fft_sides[nlevels] (FFT
+ * grid of each level's directional filtering), band_first[nlevels+1]. */
+int shx3_info(const shx3_dict* d, int* sides, int* fft_sides, int* band_first);
+long long shx3_subs_elems(const shx3_dict* d);
+long long shx3_bands_elems(const shx3_dict* d);
+size_t shx3_dict_bytes(const shx3_dict* d);
+
+/* Workspace for batches of nvol volumes; group = bands per directional launch. */
+int shx3_plan_create(const shx3_dict* d, int nvol, int group, shx3_plan** out);
+int shx3_plan_destroy(shx3_plan* p);
+size_t shx3_plan_bytes(const shx3_plan* p);
+
+/* DoPyrDec / DoPyrRec on the GPU (3DBP/PyrNDDec_mm.m:48-109,
+ * PyrNDRec_mm.m:46-95): x (nvol volumes) <-> subs. */
+int shx3_pyr_dec(shx3_plan* p, const float* x, float* subs, void* stream);
+int shx3_pyr_rec(shx3_plan* p, const float* subs, float* x, void* stream);
+/* ShDecThRec_cuda(F, BP, level, ..., nsstScalars, sigma, T)
+ * (shearcuda3d/3DShearTrans/src_cuda/ShDecThRec_cuda.cpp:74-174,178-335):
+ * for every level, the sum over cones and directions of the hard-thresholded
+ * 'same' convolutions of subs level l, written to the level slots of rec
+ * (subband layout; the lowpass slot is not touched). */
+int shx3_dec_thresh_rec(shx3_plan* p, const float* subs, const float* th, float* rec, void* stream);
+/* ShDec for every cone (3DShearTrans/ShDec.m:18-34): all bands. */
+int shx3_analysis(shx3_plan* p, const float* subs, float* bands, void* stream);
+/* Transpose of shx3_analysis: level slots of subs_bar = sum over the level's
+ * bands of the correlation with the band filter.  No counterpart in the
+ * reference (gradients and adjoint checks). */
+int shx3_analysis_adjoint(shx3_plan* p, const float* bands, float* subs_bar, void* stream);
+/* DenoiseDemo/main.m:100-150: DoPyrDec, ShDecThRec, lowpass passed through,
+ * DoPyrRec, on nvol volumes. */
+int shx3_denoise(shx3_plan* p, const float* x, const float* th, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

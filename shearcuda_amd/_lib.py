@@ -77,6 +77,21 @@ _SIGNATURES = [
     ("shx_softth", c_int, [c_void_p, c_int64, c_double, c_void_p]),
     ("shx_maxflat_factor_error", c_double, []),
     ("shx_maxflat_filter", c_int, [c_int, P(c_double), P(c_int)]),
+    ("shx3_prepare", c_int, [c_int, c_int, c_double, P(c_int), c_int, P(c_void_p), P(c_void_p)]),
+    ("shx3_destroy", c_int, [c_void_p]),
+    ("shx3_info", c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),
+    ("shx3_subs_elems", ctypes.c_longlong, [c_void_p]),
+    ("shx3_bands_elems", ctypes.c_longlong, [c_void_p]),
+    ("shx3_dict_bytes", c_size_t, [c_void_p]),
+    ("shx3_plan_create", c_int, [c_void_p, c_int, c_int, P(c_void_p)]),
+    ("shx3_plan_destroy", c_int, [c_void_p]),
+    ("shx3_plan_bytes", c_size_t, [c_void_p]),
+    ("shx3_pyr_dec", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx3_pyr_rec", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx3_dec_thresh_rec", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx3_analysis", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx3_analysis_adjoint", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx3_denoise", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 ]
 
 EXPORTED = [s[0] for s in _SIGNATURES]

@@ -17,7 +17,8 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libshx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["api.hip", "pyramid.hip", "spectral.hip", "rdwt.hip", "maxflat.cpp", "dict_host.cpp"]
+SOURCES = ["api.hip", "api3d.hip", "pyramid.hip", "spectral.hip", "spectral3d.hip", "rdwt.hip", "maxflat.cpp",
+           "dict_host.cpp"]
 GEN = os.path.join(BUILD, "gen")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + GEN, "-I" + os.path.join(ROOT, "include")]

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/shx.h"
+#include "api_err.h"
 #include "dict_host.h"
 #include "pyramid.h"
 #include "rdwt.h"
@@ -24,8 +25,9 @@ using shx::PlaneSet;
 // errors
 namespace {
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char* fmt, ...) {
+int shx::fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -35,11 +37,8 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define SHX_HIP(expr)                                                                            \
-  do {                                                                                           \
-    hipError_t e_ = (expr);                                                                      \
-    if (e_ != hipSuccess) return fail(SHX_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
+namespace {
+using shx::fail;
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 }  // namespace

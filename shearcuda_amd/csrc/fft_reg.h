@@ -11,7 +11,7 @@
 // a transform split as (R2, R1).  A forward transform, a pointwise product and
 // an inverse transform therefore chain in registers.
 //
-// The short DFTs (2, 4, 8, 16, 32) are fully unrolled with compile-time
+// The short DFTs (2, 3, 4, 5, 8, 9, 16, 32) are fully unrolled with compile-time
 // twiddles (computed below by constexpr series in FP64, rounded once to FP32);
 // the pass twiddles w_N^m come from a table computed in FP64 on the host.
 #pragma once
@@ -102,9 +102,43 @@ __device__ __forceinline__ void dft_split(float2* v) {
   }
 }
 
+// length 3: X1,2 = a - (b+c)/2 -/+ DIR-signed i (sqrt3/2)(b - c)
+template <int DIR>
+__device__ __forceinline__ void dft3(float2* v) {
+  const float h = 0.86602540378443864676f;
+  const float2 a = v[0], s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+  const float2 m = make_float2(a.x - 0.5f * s.x, a.y - 0.5f * s.y);
+  // DIR * i * h * d
+  const float2 r = make_float2(-DIR * h * d.y, DIR * h * d.x);
+  v[0] = cadd(a, s);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
+// direct DFT of a short odd length with compile-time twiddles
+template <int R, int DIR>
+__device__ __forceinline__ void dft_direct(float2* v) {
+  float2 x[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    float2 a = v[0];
+#pragma unroll
+    for (int n = 1; n < R; ++n) a = cadd(a, ctw_mul<R, DIR>(v[n], n * k));
+    x[k] = a;
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = x[k];
+}
+
 template <int R, int DIR>
 __device__ __forceinline__ void dft(float2* v) {
   if constexpr (R == 1) {
+  } else if constexpr (R == 3) {
+    dft3<DIR>(v);
+  } else if constexpr (R == 5) {
+    dft_direct<5, DIR>(v);
+  } else if constexpr (R == 9) {
+    dft_split<3, 3, DIR>(v);
   } else if constexpr (R == 2) {
     dft2<DIR>(v[0], v[1]);
   } else if constexpr (R == 4) {
@@ -124,6 +158,11 @@ __device__ __forceinline__ void dft(float2* v) {
 // N = R1 * R2 split used by the two-pass transforms (R1 <= R2 <= 32).
 template <int N>
 struct Split;
+template <> struct Split<8> { static constexpr int R1 = 2, R2 = 4; };
+template <> struct Split<24> { static constexpr int R1 = 3, R2 = 8; };
+template <> struct Split<48> { static constexpr int R1 = 3, R2 = 16; };
+template <> struct Split<80> { static constexpr int R1 = 5, R2 = 16; };
+template <> struct Split<144> { static constexpr int R1 = 9, R2 = 16; };
 template <> struct Split<16> { static constexpr int R1 = 4, R2 = 4; };
 template <> struct Split<32> { static constexpr int R1 = 4, R2 = 8; };
 template <> struct Split<64> { static constexpr int R1 = 8, R2 = 8; };

@@ -74,3 +74,46 @@ def separation_input(L, seed=0, sigma=10.0):
     curves = bezier_curves(L, n=20, seed=seed + 7)
     rng = np.random.default_rng(seed + 2_000_003)
     return cartoon, curves, cartoon + curves + sigma * rng.standard_normal((L, L))
+
+
+def _rotation(rng):
+    """Uniformly random 3D rotation (QR of a Gaussian matrix, sign-fixed)."""
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def ellipsoids(n, count=8, seed=0, axes=(8.0, 40.0), intensity=(0.0, 255.0)):
+    """n^3 volume of filled ellipsoids: centres uniform in the cube, semi-axes
+    uniform in ``axes`` voxels, random orientation, intensities uniform in
+    ``intensity``, summed where they overlap."""
+    rng = np.random.default_rng(seed)
+    g = np.arange(n, dtype=np.float64)
+    vol = np.zeros((n, n, n))
+    for _ in range(count):
+        c = rng.uniform(0, n, size=3)
+        a = rng.uniform(axes[0], axes[1], size=3)
+        R = _rotation(rng)
+        val = rng.uniform(intensity[0], intensity[1])
+        d = [g - c[0], g - c[1], g - c[2]]
+        # coordinates in the ellipsoid frame, u_k = sum_i R[i, k] d_i
+        q = np.zeros((n, n, n))
+        for k in range(3):
+            u = (R[0, k] * d[0][:, None, None] + R[1, k] * d[1][None, :, None] + R[2, k] * d[2][None, None, :])
+            q += (u / a[k]) ** 2
+        vol += val * (q <= 1.0)
+    return vol
+
+
+def noisy_ellipsoids(n, sigma=25.0, seed=0, count=8):
+    """(clean, noisy) volume pair: ellipsoids plus i.i.d. N(0, sigma^2) noise."""
+    clean = ellipsoids(n, count=count, seed=seed)
+    rng = np.random.default_rng(seed + 1_000_003)
+    return clean, clean + sigma * rng.standard_normal((n, n, n))
+
+
+def white_noise_volume(n, seed=12345):
+    """Unit-variance white-noise volume for the per-band noise scalars."""
+    return np.random.default_rng(seed).standard_normal((n, n, n))
