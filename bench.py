@@ -276,6 +276,10 @@ def main():
     ap.add_argument("--no-plain-1024", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--only", choices=["3d", "adjoint", "batched"], default=None,
+                    help="run one secondary workload alone (profiling); prints its result object")
+    ap.add_argument("--vol-group", type=int, default=8, help="bands per launch of the 3D path")
+    ap.add_argument("--vol-steps", type=int, default=5)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (HBM counter calibration)")
     args = ap.parse_args()
@@ -293,6 +297,12 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.only:
+        r = {"3d": lambda: bench_3d(steps=args.vol_steps, group=args.vol_group),
+             "adjoint": bench_adjoint,
+             "batched": lambda: bench_batched(max(5, args.steps // 20), 2)}[args.only]()
+        print(json.dumps(r), flush=True)
+        return
     L, sigma = args.size, 20.0
     ms, stages, sched = bench_plain(L, sigma, args.steps, args.warmup)
     value = L * L / (ms * 1e-3) / 1e6
@@ -324,7 +334,7 @@ def main():
     if not args.no_adjoint:
         res["adjoint"] = bench_adjoint()
     if not args.no_3d:
-        res["volume"] = bench_3d()
+        res["volume"] = bench_3d(steps=args.vol_steps, group=args.vol_group)
     if args.cpu_budget > 0:
         res["cpu_baseline"] = cpu_baseline(L, sigma, args.cpu_budget)
     print(json.dumps(res), flush=True)

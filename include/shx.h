@@ -219,6 +219,32 @@ int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream)
 /* In-place soft threshold sign(v)*max(|v|-t, 0) (softth_cuda.cpp:49-112). */
 int shx_softth(float* x, int64_t n, double t, void* stream);
 
+/* Sum |a - b|^p over n elements into *out (host): the reduction of
+ * Shearcuda::getNormErr (demo_denoise/shearcuda/shearcuda.cpp:672-717). */
+int shx_norm_pow_diff(const float* a, const float* b, int64_t n, double p, double* out, void* stream);
+
+/* x += N(0, sigma^2) noise from a counter-based generator (splitmix64 of
+ * (seed, index), Box-Muller): Shearcuda::addNoise
+ * (demo_denoise/shearcuda/shearcuda.cpp:140-163), which draws from cuRAND
+ * with seed 1234; the streams differ, the distribution is the same. */
+int shx_add_noise(float* x, int64_t n, double sigma, uint64_t seed, void* stream);
+
+/* The directional halves of the transform on given pyramid levels, as
+ * Shearcuda::convolution / Shearcuda::applyfilters
+ * (demo_denoise/shearcuda/shearcuda.cpp:549-624,719-800).  y: S+1 device
+ * planes, y[0] the lowpass, y[j] (1..S, coarse to fine) the band-pass level
+ * of scale j.  convolution: coeff cells = directional bands of y (coeff[0]
+ * receives a copy of y[0]).  applyfilters: y[1..S] = the conjugate-filtered
+ * band sums of coeff (the input of the pyramid reconstruction). */
+int shx_convolution(shx_dict* d, const float* const* y, float* const* coeff, void* stream);
+/* Device image conversion between FP32 and 8-bit (GpuImage::convert,
+ * demo_denoise/shearcuda/gpuimage.cpp:121-140 with shear_cuda.cu:2039-2064):
+ * float -> uint8 saturates to [0, 255] and truncates. */
+#define SHX_TYPE_FLOAT 0
+#define SHX_TYPE_UINT8 5
+int shx_convert(void* dst, int dst_type, const void* src, int src_type, int64_t n, void* stream);
+int shx_applyfilters(shx_dict* d, const float* const* coeff, float* const* y, void* stream);
+
 /* Largest |h - sum_k v_k w_k^T| of the separable factors the pyramid kernels
  * use for the four maxflat filters (FP64; runs on the host, no GPU needed). */
 double shx_maxflat_factor_error(void);

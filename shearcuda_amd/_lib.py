@@ -77,6 +77,11 @@ _SIGNATURES = [
     ("shx_softth", c_int, [c_void_p, c_int64, c_double, c_void_p]),
     ("shx_maxflat_factor_error", c_double, []),
     ("shx_maxflat_filter", c_int, [c_int, P(c_double), P(c_int)]),
+    ("shx_norm_pow_diff", c_int, [c_void_p, c_void_p, c_int64, c_double, P(c_double), c_void_p]),
+    ("shx_add_noise", c_int, [c_void_p, c_int64, c_double, ctypes.c_uint64, c_void_p]),
+    ("shx_convolution", c_int, [c_void_p, P(c_void_p), P(c_void_p), c_void_p]),
+    ("shx_applyfilters", c_int, [c_void_p, P(c_void_p), P(c_void_p), c_void_p]),
+    ("shx_convert", c_int, [c_void_p, c_int, c_void_p, c_int, c_int64, c_void_p]),
     ("shx3_prepare", c_int, [c_int, c_int, c_double, P(c_int), c_int, P(c_void_p), P(c_void_p)]),
     ("shx3_destroy", c_int, [c_void_p]),
     ("shx3_info", c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),

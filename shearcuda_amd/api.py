@@ -18,7 +18,7 @@ import torch
 
 from . import _lib
 from ._lib import check, lib, ptr_array
-from .filters import DEFAULT_DCOMP, DEFAULT_DSIZE, DEFAULT_TSCALARS, dictionary_filters
+from .filters import DEFAULT_DCOMP, DEFAULT_DSIZE, DEFAULT_TSCALARS, dictionary_filters, shearing_filters_myer
 
 SHIFT_NONE = 0
 SHIFT_CENTER = 1
@@ -278,6 +278,74 @@ def atrousrec_cuda(y, pfilt):
     check(lib.shx_atrousrec(ptr_array([t.data_ptr() for t in y]), y[0].shape[0], len(y) - 1,
                             ctypes.c_void_p(out.data_ptr()), None, _stream()))
     return out
+
+
+def convolution_cuda(y, shear, Ct=None):
+    """Directional bands of given pyramid levels y = [lowpass, coarse ... fine]
+    (Shearcuda::convolution, demo_denoise/shearcuda/shearcuda.cpp:549-624);
+    Ct[0] is a copy of y[0]."""
+    if len(y) != shear.nscales + 1:
+        raise ValueError("y must hold the lowpass and one level per scale")
+    for t in y:
+        _require_gpu(t, "y")
+    Ct = shear.alloc_coeffs() if Ct is None else Ct
+    check(lib.shx_convolution(shear.handle, ptr_array([t.data_ptr() for t in y]),
+                              ptr_array([c.data_ptr() for c in Ct]), _stream()))
+    return Ct
+
+
+def applyfilters_cuda(Ct, shear):
+    """Conjugate-filtered band sums per scale (Shearcuda::applyfilters,
+    demo_denoise/shearcuda/shearcuda.cpp:719-800): [None, level 1 ... S]."""
+    for c in Ct:
+        _require_gpu(c, "Ct")
+    y = [None] + [torch.empty((shear.L, shear.L), device="cuda") for _ in range(shear.nscales)]
+    check(lib.shx_applyfilters(shear.handle, ptr_array([c.data_ptr() for c in Ct]),
+                               ptr_array([0] + [t.data_ptr() for t in y[1:]]), _stream()))
+    return y
+
+
+def add_noise_cuda(x, sigma, seed=1234):
+    """x += N(0, sigma^2) in place (Shearcuda::addNoise,
+    demo_denoise/shearcuda/shearcuda.cpp:140-163; counter-based generator)."""
+    _require_gpu(x)
+    check(lib.shx_add_noise(ctypes.c_void_p(x.data_ptr()), x.numel(), float(sigma), int(seed), _stream()))
+    return x
+
+
+def maxflat_filters():
+    """The four maxflat a trous filters (h0, h1, g0, g1) of the library, FP64
+    (shearlet_toolbox_1/atrousfilters.m:106-175)."""
+    out = []
+    for which in range(4):
+        n = ctypes.c_int()
+        check(lib.shx_maxflat_filter(which, None, ctypes.byref(n)))
+        h = np.empty((n.value, n.value))
+        check(lib.shx_maxflat_filter(which, h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n)))
+        out.append(h)
+    return out
+
+
+def save_shearing_filters(path, dsize=DEFAULT_DSIZE, dcomp=DEFAULT_DCOMP, dataType="single"):
+    """Write the SHFM0002 filter file of
+    demo_denoise/makeFilters/save_shearing_filters.m:43-85 (read by the C++
+    demo's ShearConfig): header, scale count, data type (0 single, 2 double),
+    per scale size, direction count and the column-major Meyer filters, then
+    the four maxflat filters."""
+    if dataType not in ("single", "double"):
+        raise ValueError("dataType must be 'single' or 'double'")
+    dt = np.float32 if dataType == "single" else np.float64
+    with open(path, "wb") as f:
+        f.write(b"SHFM0002")
+        f.write(np.array([len(dcomp), 0 if dataType == "single" else 2], dtype=np.uint16).tobytes())
+        for n1, lev in zip(dsize, dcomp):
+            w = shearing_filters_myer(n1, lev)
+            f.write(np.array([n1, w.shape[0]], dtype=np.uint16).tobytes())
+            for k in range(w.shape[0]):
+                f.write(np.ascontiguousarray(w[k].T, dtype=dt).tobytes())
+        for h in maxflat_filters():
+            f.write(np.array([h.shape[0]], dtype=np.uint16).tobytes())
+            f.write(np.ascontiguousarray(h.T, dtype=dt).tobytes())
 
 
 def re_weight_band_cuda(x, coeff, shear=None):

@@ -65,9 +65,26 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    _build_demo()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", DEMO)
     return LIB
+
+
+DEMO = os.path.join(HERE, "demo", "demo_denoise")
+
+
+def _build_demo():
+    """The C++ denoise demo over include/shearcuda_amd.hpp, linked to libshx.so
+    (found next to the package at run time through its rpath)."""
+    src = os.path.join(HERE, "demo", "demo_denoise.cpp")
+    tmp = DEMO + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src, "-o", tmp, "-L" + HERE, "-lshx",
+           "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("demo build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, DEMO)
 
 
 if __name__ == "__main__":

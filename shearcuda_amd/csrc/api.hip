@@ -132,6 +132,50 @@ __global__ void k_norm_pow(const float* x, long long n, float p, double* out) {
   if ((threadIdx.x & 63) == 0) atomicAdd(out, d);
 }
 
+__global__ void k_norm_pow_diff(const float* a, const float* b, long long n, float p, double* out) {
+  float s = 0.f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = fabsf(a[i] - b[i]);
+    s += (p == 2.f) ? v * v : (p == 1.f ? v : powf(v, p));
+  }
+  double d = s;
+  for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, d);
+}
+
+// Counter-based Gaussian noise: element i draws two uniforms from a
+// splitmix64 hash of (seed, i) and uses the Box-Muller cosine branch.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_add_noise(float* x, long long n, float sigma, unsigned long long seed) {
+  const unsigned long long key = mix64(seed);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long h = mix64(key ^ (unsigned long long)i);
+    const float u1 = ((float)(h >> 40) + 1.0f) * (1.0f / 16777216.0f);  // (0, 1]
+    const float u2 = (float)((h >> 16) & 0xFFFFFF) * (1.0f / 16777216.0f);  // [0, 1)
+    x[i] += sigma * sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+  }
+}
+
+// Image type conversion of the demo path: float -> uint8 saturates to
+// [0, 255] and truncates toward zero; uint8 -> float is exact.
+__global__ void k_f32_to_u8(const float* src, unsigned char* dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = src[i];
+    dst[i] = v > 255.f ? (unsigned char)255 : (v < 0.f ? (unsigned char)0 : (unsigned char)v);
+  }
+}
+
+__global__ void k_u8_to_f32(const unsigned char* src, float* dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
 // Per-band soft threshold over batched coefficient layouts: element e of
 // image i has band band_off + ((e mod rowlen) / seglen) (planes: rowlen =
 // image elements, seglen = plane; wavelet yh rows: rowlen = 3nL, seglen = n).
@@ -1285,6 +1329,100 @@ int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream)
   SHX_HIP(hipMemcpyAsync(out, acc, sizeof(double), hipMemcpyDeviceToHost, s));
   SHX_HIP(hipStreamSynchronize(s));
   hipFree(acc);
+  return SHX_OK;
+}
+
+int shx_norm_pow_diff(const float* a, const float* b, int64_t n, double p, double* out, void* stream) {
+  if (!a || !b || !out) return fail(SHX_ERR_ARG, "null input or output");
+  hipStream_t s = (hipStream_t)stream;
+  double* acc = nullptr;
+  SHX_HIP(hipMalloc(&acc, sizeof(double)));
+  SHX_HIP(hipMemsetAsync(acc, 0, sizeof(double), s));
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(2048, (n + 255) / 256));
+  if (n > 0) hipLaunchKernelGGL(k_norm_pow_diff, dim3(blocks), dim3(256), 0, s, a, b, (long long)n, (float)p, acc);
+  SHX_HIP(hipGetLastError());
+  SHX_HIP(hipMemcpyAsync(out, acc, sizeof(double), hipMemcpyDeviceToHost, s));
+  SHX_HIP(hipStreamSynchronize(s));
+  hipFree(acc);
+  return SHX_OK;
+}
+
+int shx_add_noise(float* x, int64_t n, double sigma, uint64_t seed, void* stream) {
+  if (!x) return fail(SHX_ERR_ARG, "null input");
+  if (n < 0) return fail(SHX_ERR_ARG, "negative length");
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(4096, (n + 255) / 256));
+  if (n > 0)
+    hipLaunchKernelGGL(k_add_noise, dim3(blocks), dim3(256), 0, (hipStream_t)stream, x, (long long)n, (float)sigma,
+                       (unsigned long long)seed);
+  SHX_HIP(hipGetLastError());
+  return SHX_OK;
+}
+
+int shx_convolution(shx_dict* d, const float* const* y, float* const* coeff, void* stream) {
+  if (int r = check_dict(d)) return r;
+  if (!y) return fail(SHX_ERR_ARG, "null pyramid cells");
+  for (int j = 0; j <= d->S; ++j)
+    if (!y[j]) return fail(SHX_ERR_ARG, "pyramid cell %d is null", j);
+  if (int r = check_cells(d, (const void* const*)coeff)) return r;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t pl = sizeof(float) * d->plane();
+  const Work& w = d->w;
+  for (int j = 0; j < d->S; ++j)
+    SHX_HIP(hipMemcpyAsync(w.planes + (size_t)j * d->plane(), y[j + 1], pl, hipMemcpyDeviceToDevice, s));
+  if (coeff[0] != y[0]) SHX_HIP(hipMemcpyAsync(coeff[0], y[0], pl, hipMemcpyDeviceToDevice, s));
+  std::vector<Stage> st;
+  stages_spec_dec(d, w, 1, s, st);
+  const int L = d->L, nb = d->nbands;
+  PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
+  PlaneSet out = cell_coeffs(d, coeff).bands;
+  st.push_back({"rows_c2r_bands", 8.0 * nb * L * L,
+                [=] { return shx::launch_rows_c2r(L, bandset, out, nb, 1, nullptr, d->tw, s); }});
+  return run_stages(st);
+}
+
+int shx_applyfilters(shx_dict* d, const float* const* coeff, float* const* y, void* stream) {
+  if (int r = check_dict(d)) return r;
+  if (!y) return fail(SHX_ERR_ARG, "null pyramid cells");
+  for (int j = 1; j <= d->S; ++j)
+    if (!y[j]) return fail(SHX_ERR_ARG, "pyramid cell %d is null", j);
+  if (int r = check_cells(d, (const void* const*)coeff)) return r;
+  hipStream_t s = (hipStream_t)stream;
+  const Work& w = d->w;
+  const int L = d->L, nb = d->nbands;
+  PlaneSet in = cell_coeffs(d, const_cast<float* const*>(coeff)).bands;
+  PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
+  shx::ThreshSpec none{};
+  std::vector<Stage> st;
+  st.push_back({"rows_r2c_bands", 8.0 * nb * L * L,
+                [=] { return shx::launch_rows_r2c(L, in, bandset, nb, 1, none, d->tw, s); }});
+  stages_spec_rec_tail(d, w, 1, s, st);
+  if (int r = run_stages(st)) return r;
+  const size_t pl = sizeof(float) * d->plane();
+  for (int j = 0; j < d->S; ++j)
+    SHX_HIP(hipMemcpyAsync(y[j + 1], w.planes + (size_t)j * d->plane(), pl, hipMemcpyDeviceToDevice, s));
+  return SHX_OK;
+}
+
+int shx_convert(void* dst, int dst_type, const void* src, int src_type, int64_t n, void* stream) {
+  if (!dst || !src) return fail(SHX_ERR_ARG, "null buffer");
+  if (n < 0) return fail(SHX_ERR_ARG, "negative length");
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(4096, (n + 255) / 256));
+  hipStream_t s = (hipStream_t)stream;
+  if (src_type == SHX_TYPE_FLOAT && dst_type == SHX_TYPE_UINT8) {
+    if (n > 0)
+      hipLaunchKernelGGL(k_f32_to_u8, dim3(blocks), dim3(256), 0, s, (const float*)src, (unsigned char*)dst,
+                         (long long)n);
+  } else if (src_type == SHX_TYPE_UINT8 && dst_type == SHX_TYPE_FLOAT) {
+    if (n > 0)
+      hipLaunchKernelGGL(k_u8_to_f32, dim3(blocks), dim3(256), 0, s, (const unsigned char*)src, (float*)dst,
+                         (long long)n);
+  } else if (src_type == dst_type && (src_type == SHX_TYPE_FLOAT || src_type == SHX_TYPE_UINT8)) {
+    const size_t es = src_type == SHX_TYPE_FLOAT ? sizeof(float) : 1;
+    if (n > 0 && dst != src) SHX_HIP(hipMemcpyAsync(dst, src, es * (size_t)n, hipMemcpyDeviceToDevice, s));
+  } else {
+    return fail(SHX_ERR_UNSUPPORTED, "conversion %d -> %d not supported", src_type, dst_type);
+  }
+  SHX_HIP(hipGetLastError());
   return SHX_OK;
 }
 
