@@ -132,15 +132,18 @@ int shx_inverse_shear_adjoint(shx_dict* dict, const float* x_bar, float* const* 
 int shx_batch_shear_trans_adjoint(shx_batch* batch, const float* coeff_bar, float* x_bar, void* stream);
 int shx_batch_inverse_shear_adjoint(shx_batch* batch, const float* x_bar, float* coeff_bar, void* stream);
 
-/* Autotuner: times the pyramid tile shapes and the reconstruction's
- * directions-per-block grouping on the plan's own workspace (the dictionary's
- * one-image plan when batch is NULL) and keeps the fastest.  chosen (may be
- * NULL) receives {pyramid variant, group}.  Allocates scratch and
- * synchronises: call it outside graph capture.  No counterpart in the
+/* Autotuner: times the pyramid tile shapes, the reconstruction's
+ * directions-per-block grouping and the column tile width of the
+ * decomposition and reconstruction column passes on the plan's own workspace
+ * (the dictionary's one-image plan when batch is NULL) and keeps the fastest.
+ * chosen (may be NULL, else 4 ints) receives {pyramid variant, group,
+ * decomposition column tile, reconstruction column tile}.  Allocates scratch
+ * and synchronises: call it outside graph capture.  No counterpart in the
  * reference. */
 int shx_autotune(shx_dict* dict, shx_batch* batch, int reps, int* chosen, void* stream);
-/* Sets a schedule explicitly (pyramid variant 0..2, group 2/4/8). */
-int shx_set_schedule(shx_dict* dict, shx_batch* batch, int pyr_variant, int group);
+/* Sets a schedule explicitly (pyramid variant 0..2, group 2/4/8, column tiles
+ * 8 or 16 (16 only for L <= 512), 0 = default). */
+int shx_set_schedule(shx_dict* dict, shx_batch* batch, int pyr_variant, int group, int ct_dec, int ct_rec);
 
 /* ---- HIP graphs --------------------------------------------------------------
  * Any sequence of shx calls that neither allocates nor synchronises

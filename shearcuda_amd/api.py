@@ -556,17 +556,17 @@ def batch_thresholds(shear, sigmas, scalars, Tscalars=DEFAULT_TSCALARS):
 
 
 def autotune(shear, batch=None, reps=5):
-    """Pick the pyramid tile shape and the reconstruction's directions per
-    column block for a plan by timing them (outside any graph capture).
-    Returns {'pyramid_variant': v, 'group': g}."""
-    chosen = (ctypes.c_int * 2)()
+    """Pick the pyramid tile shape, the reconstruction's directions per column
+    block and the column tile widths for a plan by timing them (outside any
+    graph capture).  Returns the chosen schedule as a dict."""
+    chosen = (ctypes.c_int * 4)()
     check(lib.shx_autotune(shear.handle, batch._h if batch is not None else None, int(reps), chosen, _stream()))
-    return {"pyramid_variant": chosen[0], "group": chosen[1]}
+    return {"pyramid_variant": chosen[0], "group": chosen[1], "col_tile_dec": chosen[2], "col_tile_rec": chosen[3]}
 
 
-def set_schedule(shear, pyramid_variant, group, batch=None):
+def set_schedule(shear, pyramid_variant, group, batch=None, col_tile_dec=0, col_tile_rec=0):
     check(lib.shx_set_schedule(shear.handle, batch._h if batch is not None else None, int(pyramid_variant),
-                               int(group)))
+                               int(group), int(col_tile_dec), int(col_tile_rec)))
 
 
 # ---- undecimated wavelet transform -----------------------------------------------

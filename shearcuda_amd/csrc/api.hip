@@ -55,6 +55,8 @@ struct Work {
   // schedule picked by the autotuner (defaults until shx_autotune runs)
   int pyr_variant = 0;  // pyramid tile shape
   int group = 4;        // directions per reconstruction column block
+  int ct_dec = 0;       // column tile width of the decomposition / reconstruction
+  int ct_rec = 0;       // column passes (0 = default for L)
   BandMap map{};        // the dictionary's band map with this plan's groups
 };
 
@@ -198,30 +200,41 @@ __global__ void k_soft_bands(const float* __restrict__ x, float* __restrict__ y,
   }
 }
 
-// gx = gy * 1[|x| > t];  gth[band] += sum -sign(x) * 1[|x| > t] * gy
-__global__ void k_soft_bands_bwd(const float* __restrict__ x, const float* __restrict__ gy, float* __restrict__ gx,
-                                 float* __restrict__ gth, long long n, BandLayout bl, const float* __restrict__ th) {
-  for (long long e0 = (long long)blockIdx.x * blockDim.x; e0 < n; e0 += (long long)gridDim.x * blockDim.x) {
+// gx = gy * 1[|x| > t];  gth[band] += sum -sign(x) * 1[|x| > t] * gy.
+// A block owns a contiguous chunk; the threshold gradient is summed per band in
+// LDS (one LDS atomic per wave when the wave's lanes share a band) and flushed
+// with one global atomic per band touched by the block.
+__global__ void __launch_bounds__(256) k_soft_bands_bwd(const float* __restrict__ x, const float* __restrict__ gy,
+                                                        float* __restrict__ gx, float* __restrict__ gth, long long n,
+                                                        BandLayout bl, const float* __restrict__ th, long long chunk,
+                                                        int nbands) {
+  __shared__ float acc[SHX_MAX_BANDS];
+  for (int b = threadIdx.x; b < nbands; b += blockDim.x) acc[b] = 0.f;
+  __syncthreads();
+  const long long start = (long long)blockIdx.x * chunk;
+  const long long end = start + chunk < n ? start + chunk : n;
+  for (long long e0 = start; e0 < end; e0 += blockDim.x) {
     const long long e = e0 + threadIdx.x;
     int band = -1;
     float c = 0.f;
-    if (e < n) {
+    if (e < end) {
       band = band_of(bl, e);
       const float v = x[e], g = gy[e];
       const bool keep = fabsf(v) > th[band];
       if (gx) gx[e] = keep ? g : 0.f;
       c = keep ? (v > 0.f ? -g : g) : 0.f;
     }
-    // one atomic per wave when every active lane is in the same band
     const int b0 = __shfl(band, 0);
-    const bool uniform = __all(band == b0 || band < 0);
-    if (uniform) {
+    if (__all(band == b0 || band < 0)) {
       for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-      if ((threadIdx.x & 63) == 0 && b0 >= 0) atomicAdd(&gth[b0], c);
+      if ((threadIdx.x & 63) == 0 && b0 >= 0) atomicAdd(&acc[b0], c);
     } else if (band >= 0 && c != 0.f) {
-      atomicAdd(&gth[band], c);
+      atomicAdd(&acc[band], c);
     }
   }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbands; b += blockDim.x)
+    if (acc[b] != 0.f) atomicAdd(&gth[b], acc[b]);
 }
 
 hipError_t pyr_dec(const float* x, int L, int nlev, float* const* y, float* tA, float* tB, hipStream_t s) {
@@ -384,7 +397,7 @@ void stages_spec_dec(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::v
                 [=] { return shx::launch_rows_r2c(L, yset, specset, S, nimg, none, d->tw, s); }});
   st.push_back({"cols_dec", (S + 2.0 * nb) * P, [=] {
                   return shx::launch_cols_dec(L, w.spec, w.s_spec / 2, w.band, w.s_band / 2, d->Hmain, d->Hbeta,
-                                              w.map, nimg, d->tw, s);
+                                              w.map, nimg, d->tw, s, w.ct_dec);
                 }});
 }
 
@@ -406,7 +419,7 @@ void stages_spec_rec_tail(shx_dict* d, const Work& w, int nimg, hipStream_t s, s
   const double P = 4.0 * L * L * nimg;
   st.push_back({"cols_rec", (2.0 * nb + ng) * P, [=] {
                   return shx::launch_cols_rec(L, w.band, w.s_band / 2, w.q, w.s_q / 2, d->Hmain, d->Hbeta, w.map,
-                                              nimg, d->tw, s);
+                                              nimg, d->tw, s, w.ct_rec);
                 }});
   PlaneSet qset = shx::single_cell(reinterpret_cast<float*>(w.q), ng, w.s_q);
   PlaneSet yset = shx::single_cell(w.planes, S, w.s_planes);
@@ -688,26 +701,46 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t s, int* 
   for (int v = 1; v < shx::PYR_VARIANTS; ++v)
     if (tp[v] < tp[bestv]) bestv = v;
   w.pyr_variant = bestv;
-  const int cand[3] = {2, 4, 8};
-  double tg[3] = {1e30, 1e30, 1e30};
+  // reconstruction: directions per block x column tile; decomposition: tile
+  const int gcand[3] = {2, 4, 8};
+  const int ccand[2] = {8, 16};
+  const int nct = (d->L / 2 >= 16 && d->L <= 512) ? 2 : 1;
+  double tg[3][2];
+  for (auto& r : tg) r[0] = r[1] = 1e30;
   for (int round = 0; round < 3; ++round)
-    for (int i = 0; i < 3; ++i) {
-      set_group(d, w, cand[i]);
+    for (int i = 0; i < 3; ++i)
+      for (int c = 0; c < nct; ++c) {
+        set_group(d, w, gcand[i]);
+        w.ct_rec = ccand[c];
+        std::vector<Stage> st;
+        stages_spec_rec_tail(d, w, nimg, s, st);
+        const double t = time_stages(st, s, reps);
+        if (t > 0 && t < tg[i][c]) tg[i][c] = t;
+      }
+  int bi = 0, bc = 0;
+  for (int i = 0; i < 3; ++i)
+    for (int c = 0; c < nct; ++c)
+      if (tg[i][c] < tg[bi][bc]) bi = i, bc = c;
+  set_group(d, w, gcand[bi]);
+  w.ct_rec = ccand[bc];
+  double td[2] = {1e30, 1e30};
+  for (int round = 0; round < 3; ++round)
+    for (int c = 0; c < nct; ++c) {
+      w.ct_dec = ccand[c];
       std::vector<Stage> st;
-      stages_spec_rec_tail(d, w, nimg, s, st);
+      stages_spec_dec(d, w, nimg, s, st);
       const double t = time_stages(st, s, reps);
-      if (t > 0 && t < tg[i]) tg[i] = t;
+      if (t > 0 && t < td[c]) td[c] = t;
     }
-  int bi = 0;
-  for (int i = 1; i < 3; ++i)
-    if (tg[i] < tg[bi]) bi = i;
-  set_group(d, w, cand[bi]);
+  w.ct_dec = (nct == 2 && td[1] < td[0]) ? 16 : 8;
   hipStreamSynchronize(s);
   hipFree(img);
   hipFree(coeff);
   if (chosen) {
     chosen[0] = w.pyr_variant;
     chosen[1] = w.group;
+    chosen[2] = w.ct_dec;
+    chosen[3] = w.ct_rec;
   }
   return SHX_OK;
 }
@@ -965,13 +998,18 @@ int shx_autotune(shx_dict* d, shx_batch* b, int reps, int* chosen, void* stream)
   return autotune_work(d, w, b ? b->nimg : 1, reps, (hipStream_t)stream, chosen);
 }
 
-int shx_set_schedule(shx_dict* d, shx_batch* b, int pyr_variant, int group) {
+int shx_set_schedule(shx_dict* d, shx_batch* b, int pyr_variant, int group, int ct_dec, int ct_rec) {
   if (!d) return fail(SHX_ERR_ARG, "null dictionary");
   if (pyr_variant < 0 || pyr_variant >= shx::PYR_VARIANTS) return fail(SHX_ERR_ARG, "no pyramid variant %d", pyr_variant);
   if (group != 2 && group != 4 && group != 8) return fail(SHX_ERR_ARG, "direction group must be 2, 4 or 8");
+  for (int ct : {ct_dec, ct_rec})
+    if (ct != 0 && ct != 8 && !(ct == 16 && d->L / 2 >= 16 && d->L <= 512))
+      return fail(SHX_ERR_ARG, "column tile %d not available for L = %d (0, 8 or 16)", ct, d->L);
   Work& w = b ? b->w : d->w;
   w.pyr_variant = pyr_variant;
   set_group(d, w, group);
+  w.ct_dec = ct_dec;
+  w.ct_rec = ct_rec;
   return SHX_OK;
 }
 
@@ -1223,10 +1261,15 @@ int shx_soft_bands_backward(const float* x, const float* gy, float* gx, float* g
   if (!x || !gy || !gth_dev || !th_dev) return fail(SHX_ERR_ARG, "null input, gradient or thresholds");
   if (nimg < 1 || img_elems < 1 || rowlen < 1 || seglen < 1 || img_elems % rowlen || rowlen % seglen)
     return fail(SHX_ERR_ARG, "invalid band layout");
+  const long long nbands = band_off + rowlen / seglen;
+  if (band_off < 0 || nbands > SHX_MAX_BANDS) return fail(SHX_ERR_ARG, "%lld bands (at most %d)", nbands, SHX_MAX_BANDS);
   const long long n = (long long)nimg * img_elems;
-  const int blocks = (int)std::max<long long>(1, std::min<long long>(8192, (n + 255) / 256));
+  // chunks of whole waves, about 2048 blocks
+  long long chunk = std::max<long long>(4096, (n + 2047) / 2048);
+  chunk = (chunk + 255) / 256 * 256;
+  const int blocks = (int)((n + chunk - 1) / chunk);
   hipLaunchKernelGGL(k_soft_bands_bwd, dim3(blocks), dim3(256), 0, (hipStream_t)stream, x, gy, gx, gth_dev, n,
-                     BandLayout{img_elems, rowlen, seglen, band_off}, th_dev);
+                     BandLayout{img_elems, rowlen, seglen, band_off}, th_dev, chunk, (int)nbands);
   SHX_HIP(hipGetLastError());
   return SHX_OK;
 }

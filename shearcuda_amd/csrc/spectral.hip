@@ -86,11 +86,11 @@ __device__ __forceinline__ float2 c2r_pre(float2 x, float2 xm, float2 w) {
 // Row passes.  Rows are transforms of N = L/2 complex points (the real row
 // as (even, odd) pairs), split N = R1 x R2.  A block holds ROWS rows and
 // T = ROWS * R2 threads; pass A uses ROWS * R1 of them.
-template <int L>
+template <int L, int ROWS_ = 0>
 struct RowCfg {
   static constexpr int N = L / 2;
   static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2;
-  static constexpr int ROWS = (256 / R2) < L ? (256 / R2) : L;
+  static constexpr int ROWS = ROWS_ ? ROWS_ : ((256 / R2) < L ? (256 / R2) : L);
   static constexpr int T = ROWS * R2;
   static constexpr int PADSH = ilog2(R1);
   static constexpr int LDS = lds_size<ROWS * N, PADSH>();
@@ -132,20 +132,34 @@ __device__ __forceinline__ void row_fwd_from_A(float2 (&v)[R2], bool validA, int
   }
 }
 
+// P[k] and P[N-k] of one packed row for k = n1 + R1 n2, all loads issued
+// before any is used.
+template <int N, int R1, int R2>
+__device__ __forceinline__ void gather_row(const float2* __restrict__ row, int n1, float2 (&x)[R2],
+                                           float2 (&xm)[R2]) {
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    const int k = n1 + R1 * n2;
+    x[n2] = row[k];
+    xm[n2] = row[(N - k) & (N - 1)];
+  }
+}
+
 // Inverse of a packed row: thread (r, n1) gathers P[k], P[N-k] for
-// k = n1 + R1 n2 (via ld) and returns, in pass-B layout, z[k2 + R2 k1]
-// (= L * (x[2k], x[2k+1])).  All threads of the block must call it.
-template <int L, int R1, int R2, int PADSH, class LD>
+// k = n1 + R1 n2 (gather(r, n1, x, xm) fills both arrays) and returns, in
+// pass-B layout, z[k2 + R2 k1] (= L * (x[2k], x[2k+1])).  All threads of the
+// block must call it.
+template <int L, int R1, int R2, int PADSH, class GATHER>
 __device__ __forceinline__ void row_inv_to_B(float2 (&u)[R1], bool validA, int r, int n1, float2* buf, bool validB,
-                                             int rB, int k2, const float2* __restrict__ tw, LD ld) {
+                                             int rB, int k2, const float2* __restrict__ tw, GATHER gather) {
   constexpr int N = L / 2;
   if (validA) {
-    float2 v[R2];
+    float2 v[R2], xs[R2], xms[R2];
+    gather(r, n1, xs, xms);
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const int k = n1 + R1 * n2;
-      const float2 x = ld(r, k);
-      const float2 xm = ld(r, (N - k) & (N - 1));
+      const float2 x = xs[n2], xm = xms[n2];
       v[n2] = (k == 0) ? make_float2(x.x + x.y, x.x - x.y) : c2r_pre(x, xm, tw[k]);
     }
     passA<R1, R2, +1, 2>(v, n1, tw);
@@ -192,10 +206,17 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_r2c(PlaneSet in, PlaneSet
 
 // Row complex-to-real inverse FFT of a packed spectrum (unnormalised: x L),
 // optionally of the sum of several input planes.
-template <int L, bool SUM>
-__global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r(PlaneSet in, PlaneSet out, SumTable sum,
-                                                            const float2* __restrict__ tw) {
-  using C = RowCfg<L>;
+// The summing form runs on a few planes (one per scale), so it takes fewer
+// rows per block (SumRows) to spread them over more blocks.
+template <int L>
+struct SumRows {
+  static constexpr int v = RowCfg<L>::ROWS >= 4 ? 4 : RowCfg<L>::ROWS;
+};
+
+template <int L, bool SUM, int ROWS = (SUM ? SumRows<L>::v : RowCfg<L>::ROWS)>
+__global__ void __launch_bounds__(ROWS * Split<L / 2>::R2) k_rows_c2r(PlaneSet in, PlaneSet out, SumTable sum,
+                                                                  const float2* __restrict__ tw) {
+  using C = RowCfg<L, ROWS>;
   constexpr int N = C::N, R1 = C::R1, R2 = C::R2;
   __shared__ float2 buf[C::LDS];
   const int tid = threadIdx.x;
@@ -207,17 +228,28 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r(PlaneSet in, PlaneSet
   float2 u[R1];
   if constexpr (SUM) {
     const int q0 = sum.first[p], q1 = sum.first[p + 1];
-    auto ld = [&](int rr, int k) {
-      float2 a = make_float2(0.f, 0.f);
-      for (int q = q0; q < q1; ++q)
-        a = cadd(a, reinterpret_cast<const float2*>(plane_ptr(in, q, pe, b))[(size_t)(row0 + rr) * N + k]);
-      return a;
+    // groups outermost: each group's R2 pairs of loads are in flight together
+    auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) x[n2] = xm[n2] = make_float2(0.f, 0.f);
+      for (int q = q0; q < q1; ++q) {
+        float2 a[R2], am[R2];
+        gather_row<N, R1, R2>(reinterpret_cast<const float2*>(plane_ptr(in, q, pe, b)) + (size_t)(row0 + rr) * N, nn,
+                              a, am);
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+          x[n2] = cadd(x[n2], a[n2]);
+          xm[n2] = cadd(xm[n2], am[n2]);
+        }
+      }
     };
-    row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, ld);
+    row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
   } else {
     const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe, b)) + (size_t)row0 * N;
-    auto ld = [&](int rr, int k) { return src[(size_t)rr * N + k]; };
-    row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, ld);
+    auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
+      gather_row<N, R1, R2>(src + (size_t)rr * N, nn, x, xm);
+    };
+    row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
   }
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) dst[(size_t)rB * N + k2 + R2 * k1] = u[k1];
@@ -244,8 +276,10 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
   const float t = use_th ? thresh_of(th, p, b) : 0.0f;
   const int r = tid / R1, n1 = tid % R1, rB = tid / R2, k2 = tid % R2;
   float2 u[R1];
-  auto ld = [&](int rr, int k) { return src[(size_t)rr * N + k]; };
-  row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, ld);
+  auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
+    gather_row<N, R1, R2>(src + (size_t)rr * N, nn, x, xm);
+  };
+  row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) {
     const float2 c = u[k1];
@@ -262,10 +296,17 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
 // stride L/2 complex); each column is a transform of N = L points split
 // R1 x R2.  T = CT * R2 threads; element (u, c) of the LDS tile is linear
 // element u*CT + c.
+// Column tile widths: 8, and 16 where the tile fits the LDS (L <= 512).
 template <int L>
+struct ColTile {
+  static constexpr bool wide = L / 2 >= 16 && L <= 512;
+  static constexpr int dflt = (L >= 1024) ? 8 : ((L / 2) < 16 ? (L / 2) : 16);
+};
+
+template <int L, int CT_ = ColTile<L>::dflt>
 struct ColCfg {
   static constexpr int R1 = Split<L>::R1, R2 = Split<L>::R2;
-  static constexpr int CT = (L >= 1024) ? 8 : ((L / 2) < 16 ? (L / 2) : 16);
+  static constexpr int CT = CT_;
   static constexpr int T = CT * R2;
   static constexpr int PADSH = 7;
   static constexpr int LDS = lds_size<L * CT, PADSH>();
@@ -273,10 +314,10 @@ struct ColCfg {
 
 // Forward column transform: from global (row stride L/2) to pass-B layout
 // registers u[k1] = X[k2 + R2 k1] of thread (c, k2).
-template <int L>
-__device__ __forceinline__ void col_fwd_to_B(float2 (&u)[ColCfg<L>::R1], const float2* __restrict__ src,
+template <int L, int CT_>
+__device__ __forceinline__ void col_fwd_to_B(float2 (&u)[Split<L>::R1], const float2* __restrict__ src,
                                              float2* buf, const float2* __restrict__ tw) {
-  using C = ColCfg<L>;
+  using C = ColCfg<L, CT_>;
   constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
   const int tid = threadIdx.x;
   const int c = tid % CT, n1 = tid / CT, k2 = tid / CT;
@@ -297,10 +338,10 @@ __device__ __forceinline__ void col_fwd_to_B(float2 (&u)[ColCfg<L>::R1], const f
 // Inverse column transform from pass-B layout registers (thread (c, k2)
 // holding Q[k2 + R2 k1]) to global (row stride L/2).  Starts with a barrier
 // so buf may still be in use by the caller.
-template <int L>
-__device__ __forceinline__ void col_inv_from_B(float2 (&q)[ColCfg<L>::R1], float2* __restrict__ dst, float2* buf,
+template <int L, int CT_>
+__device__ __forceinline__ void col_inv_from_B(float2 (&q)[Split<L>::R1], float2* __restrict__ dst, float2* buf,
                                                const float2* __restrict__ tw) {
-  using C = ColCfg<L>;
+  using C = ColCfg<L, CT_>;
   constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
   const int tid = threadIdx.x;
   const int c = tid % CT, k2 = tid / CT;
@@ -323,10 +364,10 @@ __device__ __forceinline__ void col_inv_from_B(float2 (&q)[ColCfg<L>::R1], float
 // Filter pass-B registers by band spectrum H (CONJ: its conjugate).  In the
 // block holding column 0 the packed (DC, Nyquist) column needs X(-u): it is
 // exchanged through LDS (block-uniform branch).
-template <int L, bool CONJ, bool ACC>
-__device__ __forceinline__ void col_filter(float2 (&u)[ColCfg<L>::R1], float2 (&out)[ColCfg<L>::R1], int c0,
+template <int L, int CT_, bool CONJ, bool ACC>
+__device__ __forceinline__ void col_filter(float2 (&u)[Split<L>::R1], float2 (&out)[Split<L>::R1], int c0,
                                            const float2* __restrict__ H, const float2* __restrict__ Hb, float2* buf) {
-  using C = ColCfg<L>;
+  using C = ColCfg<L, CT_>;
   constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
   const int tid = threadIdx.x;
   const int c = tid % CT, k2 = tid / CT;
@@ -358,22 +399,22 @@ __device__ __forceinline__ void col_filter(float2 (&u)[ColCfg<L>::R1], float2 (&
 }
 
 // Column pass of the decomposition: one block per (column tile, band, image).
-template <int L>
-__global__ void __launch_bounds__(ColCfg<L>::T) k_cols_dec(const float2* __restrict__ Y1, long long y1_stride,
+template <int L, int CT_>
+__global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_dec(const float2* __restrict__ Y1, long long y1_stride,
                                                             float2* __restrict__ Tout, long long t_stride,
                                                             const float2* __restrict__ Hmain,
                                                             const float2* __restrict__ Hbeta, BandMap map,
                                                             const float2* __restrict__ tw) {
-  using C = ColCfg<L>;
+  using C = ColCfg<L, CT_>;
   constexpr int NH = L / 2;
   __shared__ float2 buf[C::LDS];
   const int c0 = blockIdx.x * C::CT;
   const int p = blockIdx.y, b = blockIdx.z;
   const int j = map.scale_of[p];
   float2 u[C::R1], q[C::R1];
-  col_fwd_to_B<L>(u, Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0, buf, tw);
-  col_filter<L, false, false>(u, q, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
-  col_inv_from_B<L>(q, Tout + (size_t)b * t_stride + (size_t)p * L * NH + c0, buf, tw);
+  col_fwd_to_B<L, CT_>(u, Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0, buf, tw);
+  col_filter<L, CT_, false, false>(u, q, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
+  col_inv_from_B<L, CT_>(q, Tout + (size_t)b * t_stride + (size_t)p * L * NH + c0, buf, tw);
 }
 
 // Column pass of the reconstruction: one block per (column tile, direction
@@ -381,22 +422,22 @@ __global__ void __launch_bounds__(ColCfg<L>::T) k_cols_dec(const float2* __restr
 // the row pass that follows sums the groups of a scale.  For L >= 512 the
 // running sum lives in a second LDS tile (each thread owns its elements), which
 // keeps the register footprint of the 32-point passes below the spill point.
-template <int L>
+template <int L, int CT_>
 struct ColRecAcc {
   static constexpr bool LDS = L >= 512;
-  static constexpr int SIZE = LDS ? ColCfg<L>::LDS : 1;
+  static constexpr int SIZE = LDS ? ColCfg<L, CT_>::LDS : 1;
 };
 
-template <int L>
-__global__ void __launch_bounds__(ColCfg<L>::T) k_cols_rec(const float2* __restrict__ U, long long u_stride,
+template <int L, int CT_>
+__global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* __restrict__ U, long long u_stride,
                                                             float2* __restrict__ Q, long long q_stride,
                                                             const float2* __restrict__ Hmain,
                                                             const float2* __restrict__ Hbeta, BandMap map,
                                                             const float2* __restrict__ tw) {
-  using C = ColCfg<L>;
+  using C = ColCfg<L, CT_>;
   constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
   __shared__ float2 buf[C::LDS];
-  __shared__ float2 accl[ColRecAcc<L>::SIZE];
+  __shared__ float2 accl[ColRecAcc<L, CT_>::SIZE];
   const int c0 = blockIdx.x * CT;
   const int g = blockIdx.y, b = blockIdx.z;
   const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
@@ -407,24 +448,24 @@ __global__ void __launch_bounds__(ColCfg<L>::T) k_cols_rec(const float2* __restr
   for (int p = p0; p < p1; ++p) {
     float2 u[R1];
     __syncthreads();  // previous direction's reads of buf are complete
-    col_fwd_to_B<L>(u, U + (size_t)b * u_stride + (size_t)p * L * NH + c0, buf, tw);
-    if constexpr (ColRecAcc<L>::LDS) {
+    col_fwd_to_B<L, CT_>(u, U + (size_t)b * u_stride + (size_t)p * L * NH + c0, buf, tw);
+    if constexpr (ColRecAcc<L, CT_>::LDS) {
       float2 q[R1];
-      col_filter<L, true, false>(u, q, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
+      col_filter<L, CT_, true, false>(u, q, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) {
         const int e = lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c);
         accl[e] = (p == p0) ? q[k1] : cadd(accl[e], q[k1]);
       }
     } else {
-      col_filter<L, true, true>(u, acc, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
+      col_filter<L, CT_, true, true>(u, acc, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
     }
   }
-  if constexpr (ColRecAcc<L>::LDS) {
+  if constexpr (ColRecAcc<L, CT_>::LDS) {
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) acc[k1] = accl[lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c)];
   }
-  col_inv_from_B<L>(acc, Q + (size_t)b * q_stride + (size_t)g * L * NH + c0, buf, tw);
+  col_inv_from_B<L, CT_>(acc, Q + (size_t)b * q_stride + (size_t)g * L * NH + c0, buf, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,7 +473,6 @@ __global__ void __launch_bounds__(ColCfg<L>::T) k_cols_rec(const float2* __restr
 template <int L>
 struct Launch {
   using R = RowCfg<L>;
-  using C = ColCfg<L>;
   static hipError_t rows_r2c(const PlaneSet& in, const PlaneSet& out, int np, int nimg, const ThreshSpec& th,
                              const float2* tw, hipStream_t s) {
     hipLaunchKernelGGL((k_rows_r2c<L>), dim3(L / R::ROWS, np, nimg), dim3(R::T), 0, s, in, out, th, tw);
@@ -442,8 +482,9 @@ struct Launch {
                              const float2* tw, hipStream_t s) {
     SumTable st{};
     if (sum_first) {
+      using RS = RowCfg<L, SumRows<L>::v>;
       for (int i = 0; i <= np; ++i) st.first[i] = sum_first[i];
-      hipLaunchKernelGGL((k_rows_c2r<L, true>), dim3(L / R::ROWS, np, nimg), dim3(R::T), 0, s, in, out, st, tw);
+      hipLaunchKernelGGL((k_rows_c2r<L, true>), dim3(L / RS::ROWS, np, nimg), dim3(RS::T), 0, s, in, out, st, tw);
     } else {
       hipLaunchKernelGGL((k_rows_c2r<L, false>), dim3(L / R::ROWS, np, nimg), dim3(R::T), 0, s, in, out, st, tw);
     }
@@ -454,17 +495,34 @@ struct Launch {
     hipLaunchKernelGGL((k_rows_c2r_r2c<L>), dim3(L / R::ROWS, np, nimg), dim3(R::T), 0, s, Tin, band, U, th, tw);
     return hipGetLastError();
   }
-  static hipError_t cols_dec(const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
-                             const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_cols_dec<L>), dim3(L / 2 / C::CT, map.nbands, nimg), dim3(C::T), 0, s, Y1, ys, To, ts, Hm,
-                       Hb, map, tw);
+  template <int CT>
+  static hipError_t cols_dec_ct(const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
+                                const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
+    hipLaunchKernelGGL((k_cols_dec<L, CT>), dim3(L / 2 / CT, map.nbands, nimg), dim3(ColCfg<L, CT>::T), 0, s, Y1, ys,
+                       To, ts, Hm, Hb, map, tw);
     return hipGetLastError();
   }
-  static hipError_t cols_rec(const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
-                             const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_cols_rec<L>), dim3(L / 2 / C::CT, map.ngroups, nimg), dim3(C::T), 0, s, U, us, Q, qs, Hm,
-                       Hb, map, tw);
+  template <int CT>
+  static hipError_t cols_rec_ct(const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
+                                const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
+    hipLaunchKernelGGL((k_cols_rec<L, CT>), dim3(L / 2 / CT, map.ngroups, nimg), dim3(ColCfg<L, CT>::T), 0, s, U, us,
+                       Q, qs, Hm, Hb, map, tw);
     return hipGetLastError();
+  }
+  // ct: column tile width (8 or 16; 0 = default for L)
+  static hipError_t cols_dec(const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
+                             const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct) {
+    if (ct == 0) ct = ColTile<L>::dflt;
+    if (ct == 8) return cols_dec_ct<8>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s);
+    if (ct == 16 && ColTile<L>::wide) return cols_dec_ct<(ColTile<L>::wide ? 16 : 8)>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s);
+    return hipErrorInvalidValue;
+  }
+  static hipError_t cols_rec(const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
+                             const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct) {
+    if (ct == 0) ct = ColTile<L>::dflt;
+    if (ct == 8) return cols_rec_ct<8>(U, us, Q, qs, Hm, Hb, map, nimg, tw, s);
+    if (ct == 16 && ColTile<L>::wide) return cols_rec_ct<(ColTile<L>::wide ? 16 : 8)>(U, us, Q, qs, Hm, Hb, map, nimg, tw, s);
+    return hipErrorInvalidValue;
   }
 };
 
@@ -496,13 +554,13 @@ hipError_t launch_rows_c2r_r2c(int L, const PlaneSet& Tin, const PlaneSet& band,
 }
 
 hipError_t launch_cols_dec(int L, const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
-                           const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
-  SHX_BY_L(L, LT::cols_dec(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s))
+                           const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct) {
+  SHX_BY_L(L, LT::cols_dec(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s, ct))
 }
 
 hipError_t launch_cols_rec(int L, const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
-                           const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
-  SHX_BY_L(L, LT::cols_rec(U, us, Q, qs, Hm, Hb, map, nimg, tw, s))
+                           const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct) {
+  SHX_BY_L(L, LT::cols_rec(U, us, Q, qs, Hm, Hb, map, nimg, tw, s, ct))
 }
 
 }  // namespace shx
