@@ -298,6 +298,11 @@ size_t shx3_dict_bytes(const shx3_dict* d);
 int shx3_plan_create(const shx3_dict* d, int nvol, int group, shx3_plan** out);
 int shx3_plan_destroy(shx3_plan* p);
 size_t shx3_plan_bytes(const shx3_plan* p);
+/* Schedule of the directional loop: bands per launch (1 .. the group the plan
+ * was created with) and the fused plane kernel's output columns per block
+ * (0 automatic, 32 or 64; -1, the default, runs the row and column passes as
+ * two kernels). */
+int shx3_set_schedule(shx3_plan* p, int group, int plane_cols);
 
 /* DoPyrDec / DoPyrRec on the GPU (3DBP/PyrNDDec_mm.m:48-109,
  * PyrNDRec_mm.m:46-95): x (nvol volumes) <-> subs. */

@@ -91,6 +91,7 @@ _SIGNATURES = [
     ("shx3_plan_create", c_int, [c_void_p, c_int, c_int, P(c_void_p)]),
     ("shx3_plan_destroy", c_int, [c_void_p]),
     ("shx3_plan_bytes", c_size_t, [c_void_p]),
+    ("shx3_set_schedule", c_int, [c_void_p, c_int, c_int]),
     ("shx3_pyr_dec", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("shx3_pyr_rec", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("shx3_dec_thresh_rec", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

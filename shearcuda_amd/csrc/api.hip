@@ -58,6 +58,11 @@ struct Work {
   int ct_dec = 0;       // column tile width of the decomposition / reconstruction
   int ct_rec = 0;       // column passes (0 = default for L)
   BandMap map{};        // the dictionary's band map with this plan's groups
+  // side streams (one per scale) and events of the concurrent denoise: each
+  // scale's spectral chain overlaps the pyramid levels
+  hipStream_t lanes[SHX_MAX_SCALES] = {};
+  hipEvent_t evs[4 * SHX_MAX_SCALES + 4] = {};
+  int nlanes = 0;
 };
 
 struct shx_dict {
@@ -307,10 +312,23 @@ int alloc_work(const shx_dict* d, int nimg, Work& w) {
       (e = hipMalloc(&w.ext, sizeof(float) * w.s_ext * n)) != hipSuccess)
     return fail(SHX_ERR_ALLOC, "workspace allocation (%d images of %d^2) failed: %s", nimg, d->L, hipGetErrorString(e));
   w.bytes = sizeof(float) * n * (w.s_planes + w.s_spec + w.s_band + w.s_q + w.s_ext);
+  for (int j = 0; j < d->S; ++j)
+    if ((e = hipStreamCreateWithFlags(&w.lanes[j], hipStreamNonBlocking)) != hipSuccess)
+      return fail(SHX_ERR_HIP, "stream creation failed: %s", hipGetErrorString(e));
+  w.nlanes = d->S;
+  for (hipEvent_t& ev : w.evs)
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+      return fail(SHX_ERR_HIP, "event creation failed: %s", hipGetErrorString(e));
   return SHX_OK;
 }
 
 void free_work(Work& w) {
+  for (int j = 0; j < w.nlanes; ++j)
+    if (w.lanes[j]) hipStreamDestroy(w.lanes[j]);
+  for (hipEvent_t& ev : w.evs)
+    if (ev) hipEventDestroy(ev);
+  for (hipEvent_t& ev : w.evs) ev = nullptr;
+  w.nlanes = 0;
   hipFree(w.planes);
   hipFree(w.spec);
   hipFree(w.band);
@@ -356,6 +374,11 @@ struct Stage {
   std::string name;
   double bytes;
   std::function<hipError_t()> run;
+  // concurrent plans (run_stage_dag): the stream run() launches on, earlier
+  // stages to wait for, and whether later stages wait for this one
+  hipStream_t stream = nullptr;
+  std::vector<int> waits;
+  bool record = false;
 };
 
 // Thresholds of a reconstruction: none, host values (one image, kernel
@@ -499,6 +522,103 @@ void stages_denoise(shx_dict* d, const Work& w, int nimg, const float* x, long l
   stages_pyr_rec(d, w, nimg, c, t, out, os, s, st);
 }
 
+// Planes [first, first + count) of a set whose range lies in one cell.
+PlaneSet sub_planes(const PlaneSet& s, int first, int count, long long plane_elems) {
+  int j = 0;
+  while (j + 1 < s.ncells && first >= s.off[j + 1]) ++j;
+  return shx::single_cell(s.cell[j] + (first - s.off[j]) * plane_elems, count, s.img_stride);
+}
+
+// Concurrent denoise (batched form): the pyramid decomposition and reconstruction run on s0;
+// scale j's spectral chain (row R2C, column filter, fused row C2R/threshold/
+// R2C, column reconstruction, summing row C2R) runs on side stream j as soon
+// as the decomposition has produced its band-pass plane, and the
+// reconstruction level of scale j waits for that chain.  Same kernels and
+// results as stages_denoise; the per-scale launches overlap.
+void stages_denoise_dag(shx_dict* d, Work& w, int nimg, const float* x, long long xs, const Coeffs& c, const Thr& t,
+                        float* out, long long os, hipStream_t s0, std::vector<Stage>& st) {
+  const int L = d->L, S = d->S, NH = L / 2;
+  const long long pl = (long long)L * L;
+  const double P = 4.0 * pl * nimg;
+  std::vector<int> made(S), done(S);
+  {
+    std::vector<Stage> tmp;
+    stages_pyr_dec(d, w, nimg, x, xs, c, s0, tmp);
+    for (int lev = 0; lev < S; ++lev) {
+      tmp[lev].stream = s0;
+      tmp[lev].record = true;
+      made[S - 1 - lev] = (int)st.size();
+      st.push_back(tmp[lev]);
+    }
+  }
+  const shx::ThreshSpec thall = band_thresh(d, t);
+  for (int j = 0; j < S; ++j) {
+    hipStream_t sj = w.lanes[j];
+    const int f = d->map.first[j], nb = d->ndir[j];
+    const int g0 = w.map.group_first[j], ng = w.map.group_first[j + 1] - g0;
+    // sub-map of scale j: bands 0..nb-1, its groups with band ranges relative to f
+    BandMap m{};
+    m.nscales = 1;
+    m.nbands = nb;
+    m.first[0] = 0;
+    m.first[1] = nb;
+    m.ngroups = ng;
+    m.group_first[0] = 0;
+    m.group_first[1] = ng;
+    for (int g = 0; g <= ng; ++g) m.group_band0[g] = (unsigned short)(w.map.group_band0[g0 + g] - f);
+    shx::ThreshSpec th = thall;
+    if (th.mode == 1)
+      for (int p = 0; p < nb; ++p) th.arg[p] = thall.arg[f + p];
+    if (th.mode == 2) th.dev_off += f;
+    PlaneSet ypl = shx::single_cell(w.planes + (size_t)j * pl, 1, w.s_planes);
+    PlaneSet spc = shx::single_cell(reinterpret_cast<float*>(w.spec) + (size_t)j * pl, 1, w.s_spec);
+    PlaneSet bnd = shx::single_cell(reinterpret_cast<float*>(w.band) + (size_t)f * pl, nb, w.s_band);
+    PlaneSet cb = sub_planes(c.bands, f, nb, pl);
+    PlaneSet qset = shx::single_cell(reinterpret_cast<float*>(w.q) + (size_t)g0 * pl, ng, w.s_q);
+    const float2* Y1 = w.spec + (size_t)j * L * NH;
+    float2* T = w.band + (size_t)f * L * NH;
+    float2* Q = w.q + (size_t)g0 * L * NH;
+    const float2* Hm = d->Hmain + (size_t)f * L * NH;
+    const float2* Hb = d->Hbeta + (size_t)f * L;
+    shx::ThreshSpec none{};
+    const std::string sfx = "_s" + std::to_string(j);
+    Stage a{"rows_r2c" + sfx, 2 * P, [=] { return shx::launch_rows_r2c(L, ypl, spc, 1, nimg, none, d->tw, sj); }};
+    a.stream = sj;
+    a.waits = {made[j]};
+    st.push_back(a);
+    Stage b{"cols_dec" + sfx, (1 + 2.0 * nb) * P, [=] {
+              return shx::launch_cols_dec(L, Y1, w.s_spec / 2, T, w.s_band / 2, Hm, Hb, m, nimg, d->tw, sj, w.ct_dec);
+            }};
+    b.stream = sj;
+    st.push_back(b);
+    Stage cc{"rows_c2r_r2c" + sfx, 3.0 * nb * P,
+             [=] { return shx::launch_rows_c2r_r2c(L, bnd, cb, bnd, nb, nimg, th, d->tw, sj); }};
+    cc.stream = sj;
+    st.push_back(cc);
+    Stage e{"cols_rec" + sfx, (2.0 * nb + ng) * P, [=] {
+              return shx::launch_cols_rec(L, T, w.s_band / 2, Q, w.s_q / 2, Hm, Hb, m, nimg, d->tw, sj, w.ct_rec);
+            }};
+    e.stream = sj;
+    st.push_back(e);
+    const int sf[2] = {0, ng};
+    Stage h{"rows_c2r_sum" + sfx, (ng + 1.0) * P,
+            [=] { return shx::launch_rows_c2r(L, qset, ypl, 1, nimg, sf, d->tw, sj); }};
+    h.stream = sj;
+    h.record = true;
+    done[j] = (int)st.size();
+    st.push_back(h);
+  }
+  {
+    std::vector<Stage> tmp;
+    stages_pyr_rec(d, w, nimg, c, t, out, os, s0, tmp);
+    for (int k = 0; k < S; ++k) {
+      tmp[k].stream = s0;
+      tmp[k].waits = {done[k]};
+      st.push_back(tmp[k]);
+    }
+  }
+}
+
 // Adjoint of the pyramid decomposition: gradients of the lowpass (c.low) and
 // of the pyramid bands (workspace ypyr planes) -> gradient of the image gx.
 void stages_pyr_dec_adjoint(shx_dict* d, const Work& w, int nimg, const Coeffs& c, float* gx, long long gxs,
@@ -578,6 +698,30 @@ int run_stages(const std::vector<Stage>& st) {
   for (const Stage& g : st) {
     hipError_t e = g.run();
     if (e != hipSuccess) return fail(SHX_ERR_HIP, "%s launch failed: %s", g.name.c_str(), hipGetErrorString(e));
+  }
+  return SHX_OK;
+}
+
+// Runs stages that carry their own streams: every wait becomes
+// hipStreamWaitEvent on an event recorded after the awaited stage (under
+// graph capture these are the graph's edges).
+int run_stage_dag(const std::vector<Stage>& st, const Work& w) {
+  std::vector<int> ev_of(st.size(), -1);
+  int nev = 0;
+  const int maxev = (int)(sizeof(w.evs) / sizeof(w.evs[0]));
+  for (size_t i = 0; i < st.size(); ++i) {
+    const Stage& g = st[i];
+    for (int j : g.waits) {
+      if (j < 0 || j >= (int)i || ev_of[j] < 0) return fail(SHX_ERR_ARG, "stage %s waits for an unrecorded stage", g.name.c_str());
+      SHX_HIP(hipStreamWaitEvent(g.stream, w.evs[ev_of[j]], 0));
+    }
+    hipError_t e = g.run();
+    if (e != hipSuccess) return fail(SHX_ERR_HIP, "%s launch failed: %s", g.name.c_str(), hipGetErrorString(e));
+    if (g.record) {
+      if (nev >= maxev) return fail(SHX_ERR_ARG, "too many concurrent stages");
+      ev_of[i] = nev;
+      SHX_HIP(hipEventRecord(w.evs[nev++], g.stream));
+    }
   }
   return SHX_OK;
 }
@@ -901,6 +1045,8 @@ int shx_denoise(shx_dict* d, const float* x, float* const* coeff, double sigma, 
   Thr t;
   t.mode = 1;
   t.host = th.data();
+  // one image: the kernels are short, so the single-stream chain beats the
+  // concurrent form (whose cross-stream edges cost more than the overlap wins)
   std::vector<Stage> st;
   stages_denoise(d, d->w, 1, x, 0, cell_coeffs(d, coeff), t, out, 0, (hipStream_t)stream, st);
   return run_stages(st);
@@ -986,8 +1132,8 @@ int shx_batch_denoise(shx_batch* b, const float* x, float* coeff, const float* t
   t.dev = th_dev;
   std::vector<Stage> st;
   const long long pl = (long long)d->L * d->L;
-  stages_denoise(d, b->w, b->nimg, x, pl, stack_coeffs(d, coeff), t, out, pl, (hipStream_t)stream, st);
-  return run_stages(st);
+  stages_denoise_dag(d, b->w, b->nimg, x, pl, stack_coeffs(d, coeff), t, out, pl, (hipStream_t)stream, st);
+  return run_stage_dag(st, b->w);
 }
 
 int shx_autotune(shx_dict* d, shx_batch* b, int reps, int* chosen, void* stream) {

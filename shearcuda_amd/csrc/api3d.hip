@@ -106,7 +106,11 @@ struct shx3_dict {
 
 struct shx3_plan {
   const shx3_dict* d = nullptr;
-  int nvol = 0, G = 1;
+  int nvol = 0, G = 1, Gmax = 1;
+  // fused plane kernel columns per block (0 auto, 32, 64) or -1: separate row
+  // and column passes, the default (measured faster at 128^3: the fused form
+  // recomputes row FFTs per column chunk and runs at 2 waves per SIMD)
+  int plane_cols = -1;
   float2 *X = nullptr, *Hs = nullptr, *Lp0 = nullptr, *Lp1 = nullptr, *S = nullptr, *tA = nullptr, *tB = nullptr;
   long long sX = 0, sS = 0, sA = 0, sB = 0;
   float *subs = nullptr, *rec = nullptr;
@@ -210,11 +214,19 @@ int directional(shx3_plan* p, const float* subs, const float* th, float* bands, 
     const long long fstr = spec_elems(N), n3 = (long long)n * n * n;
     for (int g0 = 0; g0 < v.nb; g0 += p->G) {
       const int G = std::min(p->G, v.nb - g0);
-      shx::Rows3Mul rm{p->S, p->sS, v.F + (size_t)g0 * fstr, fstr, (float)(1.0 / ((double)N * N * N)),
-                       p->tB, n, (long long)H * N * n, p->sB, n, o, H * N, G};
-      SHX_LAUNCH(shx::launch3_rows_mul(N, rm, nv, tw, s));
-      shx::Cols3 c{p->tB, (long long)N * n, p->sB, N, 0, p->tA, (long long)n * n, p->sA, n, o, n, G * H};
-      SHX_LAUNCH(shx::launch3_cols(N, +1, c, nv, tw, s));
+      const float scale = (float)(1.0 / ((double)N * N * N));
+      if (p->plane_cols >= 0 && shx::plane3_supported(N, n)) {
+        // rows and cols of each plane in one kernel, straight into tA
+        shx::Plane3Inv pi{p->S, p->sS, v.F + (size_t)g0 * fstr, fstr, scale, p->tA, (long long)H * n * n, p->sA,
+                          n, o, G, H};
+        SHX_LAUNCH(shx::launch3_plane_inv(N, pi, nv, p->plane_cols, tw, s));
+      } else {
+        shx::Rows3Mul rm{p->S, p->sS, v.F + (size_t)g0 * fstr, fstr, scale, p->tB, n, (long long)H * N * n, p->sB, n,
+                         o, H * N, G};
+        SHX_LAUNCH(shx::launch3_rows_mul(N, rm, nv, tw, s));
+        shx::Cols3 c{p->tB, (long long)N * n, p->sB, N, 0, p->tA, (long long)n * n, p->sA, n, o, n, G * H};
+        SHX_LAUNCH(shx::launch3_cols(N, +1, c, nv, tw, s));
+      }
       shx::Pencil3Inv pi{};
       pi.in = p->tA;
       pi.in_gs = (long long)H * n * n;
@@ -458,7 +470,7 @@ int shx3_plan_create(const shx3_dict* d, int nvol, int group, shx3_plan** out) {
   auto p = new shx3_plan();
   p->d = d;
   p->nvol = nvol;
-  p->G = group;
+  p->G = p->Gmax = group;
   long long sA = spec_elems(d->n), sB = spec_elems(d->n), sS = 0;
   for (int l = 0; l < d->L; ++l) {
     const Level3& v = d->lv[l];
@@ -504,6 +516,17 @@ int shx3_plan_destroy(shx3_plan* p) {
 }
 
 size_t shx3_plan_bytes(const shx3_plan* p) { return p ? p->bytes : 0; }
+
+int shx3_set_schedule(shx3_plan* p, int group, int plane_cols) {
+  if (!p) return fail(SHX_ERR_ARG, "null 3D plan");
+  if (group < 1 || group > p->Gmax)
+    return fail(SHX_ERR_ARG, "band group %d not in [1, %d] (the plan's workspace size)", group, p->Gmax);
+  if (plane_cols != -1 && plane_cols != 0 && plane_cols != 32 && plane_cols != 64)
+    return fail(SHX_ERR_ARG, "plane columns %d not in {-1, 0, 32, 64}", plane_cols);
+  p->G = group;
+  p->plane_cols = plane_cols;
+  return SHX_OK;
+}
 
 int shx3_pyr_dec(shx3_plan* p, const float* x, float* subs, void* stream) {
   if (!p || !x || !subs) return fail(SHX_ERR_ARG, "null argument");

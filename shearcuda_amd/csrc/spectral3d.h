@@ -44,6 +44,22 @@ struct Rows3Mul {
   int nlines, G;
 };
 
+// Fused inverse of one half-spectrum plane for G bands: plane k0 of
+// F_g . S * scale is inverse transformed along axis 2 (rows) and axis 1
+// (cols) with the output window [o_out, o_out + n_out) on both, giving
+// out[g][k0][i1][i2] (i1, i2 < n_out) at out + b*out_bs + g*out_gs +
+// (k0*n_out + i1)*n_out + i2.  S and F planes are N x N complex.
+struct Plane3Inv {
+  const float2* S;
+  long long s_bs;
+  const float2* F;
+  long long f_gs;
+  float scale;
+  float2* out;
+  long long out_gs, out_bs;
+  int n_out, o_out, G, H;
+};
+
 // Forward rows of G band inputs, multiplied by conj(F_g) * scale and summed
 // over the bands into out (lines of N; accumulate = add to out).
 struct Rows3Cmac {
@@ -110,6 +126,9 @@ bool fft3_supported(int N);
 hipError_t launch3_rows(int N, int dir, const Rows3& io, int nb, const float2* tw, hipStream_t s);
 hipError_t launch3_rows_mul(int N, const Rows3Mul& io, int nb, const float2* tw, hipStream_t s);
 hipError_t launch3_rows_cmac(int N, const Rows3Cmac& io, int nb, const float2* tw, hipStream_t s);
+// cw: output columns per block (64 or 32; 0 = default).
+bool plane3_supported(int N, int n_out);
+hipError_t launch3_plane_inv(int N, const Plane3Inv& io, int nb, int cw, const float2* tw, hipStream_t s);
 hipError_t launch3_cols(int N, int dir, const Cols3& io, int nb, const float2* tw, hipStream_t s);
 hipError_t launch3_pencil_fwd(int N, const Pencil3Fwd& io, int G, int nb, const float2* tw, hipStream_t s);
 hipError_t launch3_pencil_inv(int N, const Pencil3Inv& io, int nb, const float2* tw, hipStream_t s);

@@ -169,6 +169,104 @@ __global__ void __launch_bounds__(256) k3_rows_cmac(Rows3Cmac io, const float2* 
 }
 
 // ---------------------------------------------------------------------------
+// Fused plane inverse: rows then cols of one k0 plane for a chunk of CW
+// output columns, through an LDS tile of N rows x CW columns.  Blocks are
+// ordered volume-fastest, then band, chunk, plane, so the blocks that share an
+// S plane or an F plane run together, and each XCD (the low 3 bits of the
+// dispatch order) takes a contiguous range of that order, so the sharing
+// stays in one L2.
+template <int N>
+struct Plane3Cfg {
+  static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2;
+  static constexpr int T = N >= 80 ? 512 : 256;
+  static constexpr int RC = T / R2;  // rows per row pass
+  static constexpr int CC = T / R2;  // columns per column pass
+};
+
+template <int N, int CW>
+__global__ void __launch_bounds__(Plane3Cfg<N>::T) k3_plane_inv(Plane3Inv io, int nchunk, int nblk, int nvol,
+                                                                const float2* __restrict__ tw) {
+  using C = Plane3Cfg<N>;
+  constexpr int R1 = C::R1, R2 = C::R2, RC = C::RC, CC = C::CC, T = C::T;
+  constexpr int CWP = CW + 1;
+  constexpr int SCR = (RC * (N + 1) > N * CC) ? RC * (N + 1) : N * CC;
+  __shared__ float2 tile[N * CWP];
+  __shared__ float2 scr[SCR];
+  // XCD-aware block order
+  const int pid = blockIdx.x;
+  const int per = nblk / 8;
+  const int lid = (nblk % 8 == 0) ? (pid % 8) * per + pid / 8 : pid;
+  int t = lid;
+  const int v = t % nvol;
+  t /= nvol;
+  const int g = t % io.G;
+  t /= io.G;
+  const int ch = t % nchunk;
+  const int k0 = t / nchunk;
+  const int tid = threadIdx.x;
+  const int col0 = ch * CW;
+  const float2* Sp = io.S + (size_t)v * io.s_bs + (size_t)k0 * N * N;
+  const float2* Fp = io.F + (size_t)g * io.f_gs + (size_t)k0 * N * N;
+  // rows: inverse along axis 2, keep columns [o_out + col0, o_out + col0 + CW)
+  for (int r0 = 0; r0 < N; r0 += RC) {
+    const int ra = tid / R1, n1 = tid % R1;
+    if (ra < RC && r0 + ra < N) {
+      const float2* srow = Sp + (size_t)(r0 + ra) * N;
+      const float2* frow = Fp + (size_t)(r0 + ra) * N;
+      float2 vv[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) vv[n2] = cscale(cmul(srow[n1 + R1 * n2], frow[n1 + R1 * n2]), io.scale);
+      passA<R1, R2, +1, 1>(vv, n1, tw);
+#pragma unroll
+      for (int k = 0; k < R2; ++k) scr[ra * (N + 1) + n1 + R1 * k] = vv[k];
+    }
+    __syncthreads();
+    const int rb = tid / R2, k2 = tid % R2;
+    if (r0 + rb < N) {
+      float2 u[R1];
+#pragma unroll
+      for (int a = 0; a < R1; ++a) u[a] = scr[rb * (N + 1) + a + R1 * k2];
+      dft<R1, +1>(u);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const int j = k2 + R2 * k1 - io.o_out - col0;
+        if (j >= 0 && j < CW) tile[(r0 + rb) * CWP + j] = u[k1];
+      }
+    }
+    __syncthreads();
+  }
+  // cols: inverse along axis 1, keep rows [o_out, o_out + n_out)
+  const int n = io.n_out;
+  float2* op = io.out + (size_t)v * io.out_bs + (size_t)g * io.out_gs + (size_t)k0 * n * n + col0;
+  for (int c0 = 0; c0 < CW; c0 += CC) {
+    const int c = tid % CC, n1 = tid / CC;
+    if (n1 < R1 && c0 + c < CW) {
+      float2 vv[R2];
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) vv[n2] = tile[(n1 + R1 * n2) * CWP + c0 + c];
+      passA<R1, R2, +1, 1>(vv, n1, tw);
+#pragma unroll
+      for (int k = 0; k < R2; ++k) scr[(n1 + R1 * k) * CC + c] = vv[k];
+    }
+    __syncthreads();
+    const int k2 = tid / CC;
+    if (c0 + c < CW && col0 + c0 + c < n) {
+      float2 u[R1];
+#pragma unroll
+      for (int a = 0; a < R1; ++a) u[a] = scr[(a + R1 * k2) * CC + c];
+      dft<R1, +1>(u);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const int i1 = k2 + R2 * k1 - io.o_out;
+        if (in_window(i1, n)) op[(size_t)i1 * n + c0 + c] = u[k1];
+      }
+    }
+    __syncthreads();
+  }
+  (void)T;
+}
+
+// ---------------------------------------------------------------------------
 // Cols (axis 1): a tile of CT adjacent columns, T = CT * R2 threads; element
 // (m, c) of the tile at LDS slot m * CT + c.
 template <int N, int CT>
@@ -412,6 +510,21 @@ struct Launch3 {
     hipLaunchKernelGGL((k3_rows_mul<N>), grid, dim3(RC::T), 0, s, io, tw);
     return hipGetLastError();
   }
+  template <int CW>
+  static hipError_t plane_cw(const Plane3Inv& io, int nb, const float2* tw, hipStream_t s) {
+    const int nchunk = (io.n_out + CW - 1) / CW;
+    const int nblk = nchunk * io.H * io.G * nb;
+    hipLaunchKernelGGL((k3_plane_inv<N, CW>), dim3(nblk), dim3(Plane3Cfg<N>::T), 0, s, io, nchunk, nblk, nb, tw);
+    return hipGetLastError();
+  }
+  static hipError_t plane_inv(const Plane3Inv& io, int nb, int cw, const float2* tw, hipStream_t s) {
+    if constexpr (N >= 24) {
+      if (cw == 0) cw = io.n_out >= 64 ? 64 : 32;
+      if (cw == 64) return plane_cw<64>(io, nb, tw, s);
+      if (cw == 32) return plane_cw<32>(io, nb, tw, s);
+    }
+    return hipErrorInvalidValue;
+  }
   static hipError_t rows_cmac(const Rows3Cmac& io, int nb, const float2* tw, hipStream_t s) {
     const dim3 grid((io.nlines + RC::LPB - 1) / RC::LPB, nb);
     hipLaunchKernelGGL((k3_rows_cmac<N>), grid, dim3(RC::T), 0, s, io, tw);
@@ -485,6 +598,12 @@ hipError_t launch3_rows_mul(int N, const Rows3Mul& io, int nb, const float2* tw,
 hipError_t launch3_rows_cmac(int N, const Rows3Cmac& io, int nb, const float2* tw, hipStream_t s) {
   SHX3_BY_N(N, LT::rows_cmac(io, nb, tw, s))
 }
+bool plane3_supported(int N, int n_out) { return N >= 24 && fft3_supported(N) && n_out >= 8; }
+
+hipError_t launch3_plane_inv(int N, const Plane3Inv& io, int nb, int cw, const float2* tw, hipStream_t s) {
+  SHX3_BY_N(N, LT::plane_inv(io, nb, cw, tw, s))
+}
+
 hipError_t launch3_cols(int N, int dir, const Cols3& io, int nb, const float2* tw, hipStream_t s) {
   SHX3_BY_N(N, LT::cols(dir, io, nb, tw, s))
 }

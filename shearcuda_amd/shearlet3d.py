@@ -139,6 +139,12 @@ class Plan3D:
     def nbytes(self):
         return int(lib.shx3_plan_bytes(self.handle))
 
+    def set_schedule(self, group=None, plane_cols=-1):
+        """Bands per launch (at most the plan's group) and the fused plane
+        kernel's columns per block (0 automatic, 32, 64; -1, the default,
+        separate row and column passes)."""
+        check(lib.shx3_set_schedule(self.handle, int(self.group if group is None else group), int(plane_cols)))
+
     def close(self):
         if self._h is not None:
             lib.shx3_plan_destroy(self._h)

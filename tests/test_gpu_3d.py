@@ -189,6 +189,23 @@ def test_denoise_matches_oracle(mods, case32):
     plan.close()
 
 
+@pytest.mark.parametrize("n,cw", [(32, 32), (64, 64), (64, 32)])
+def test_fused_plane_kernel_equals_separate_passes(mods, n, cw):
+    """The fused plane kernel (rows + cols of a plane in one launch) against
+    the separate row and column passes, on every band."""
+    print("case fused plane n=%d cw=%d" % (n, cw), flush=True)
+    sh = _shear(mods, n)
+    plan = mods[1].Plan3D(sh, 2, group=6)
+    x = torch.tensor(_vol(n, 61, nvol=2), dtype=torch.float32, device="cuda")
+    subs = plan.pyr_dec(x)
+    plan.set_schedule(plane_cols=-1)
+    ref = plan.analysis(subs).clone()
+    plan.set_schedule(plane_cols=cw)
+    got = plan.analysis(subs)
+    assert float((got - ref).abs().max() / ref.abs().max()) < 1e-6
+    plan.close()
+
+
 def test_batch_equals_single(mods, case32):
     print("case batch n=32", flush=True)
     sh = case32[0]
