@@ -220,7 +220,39 @@ def vol_bytes_model(sides, band_first):
     return b
 
 
-def bench_3d(steps=5, warmup=1, nvol=4, n=128, sigma=25.0, group=8):
+def cpu_baseline_3d(n, sigma, nbands_sample=6):
+    """FP64 numpy oracle of the volume denoise on one host core, extrapolated
+    from a bounded sample: the band-pass decomposition and reconstruction of
+    one volume plus `nbands_sample` thresholded 'same' convolutions per level,
+    scaled to all bands of the level."""
+    import numpy as np
+
+    from oracle import nsst3d as o3
+    from shearcuda_amd.synth import noisy_ellipsoids
+    _, x = noisy_ellipsoids(n, sigma=sigma, seed=700)
+    F = o3.meyer_filters_3d(list(VOL_DIRS))
+    t0 = time.perf_counter()
+    subs = o3.pyr_nd_dec(x, len(VOL_DIRS), 1)
+    o3.pyr_nd_rec(subs, 1)
+    total = time.perf_counter() - t0
+    sampled = 0
+    for l in range(len(VOL_DIRS)):
+        bands = o3.band_list(F, l)
+        conv = o3._SameConv(subs[l], F[0][l][0][0].shape)
+        acc = np.zeros(subs[l].shape)
+        t0 = time.perf_counter()
+        for f in bands[:nbands_sample]:
+            b = conv(f)
+            acc += np.where(np.abs(b) <= 3 * sigma * 0.05, 0.0, b)
+        total += (time.perf_counter() - t0) * len(bands) / nbands_sample
+        sampled += nbands_sample
+    return {"value": round(n ** 3 / total / 1e6, 5), "unit": "Mvoxel/s", "cores": 1, "kind": "port",
+            "sample": "one %d^3 volume: band-pass dec+rec and %d of %d band convolutions (%d per level) timed, "
+                      "extrapolated to all bands (%.1f s per volume)" % (n, sampled, 3 * sum(d * d for d in VOL_DIRS),
+                                                                         nbands_sample, total)}
+
+
+def bench_3d(steps=5, warmup=1, nvol=4, n=128, sigma=25.0, group=8, cpu=True):
     """3D shearlet volume denoise of a batch of 4 x 128^3 FP32 volumes
     (8 random filled ellipsoids + N(0, 25^2) each) as one graph launch:
     Fourier band-pass, 3 levels (Pyr_mode 1), 3 cones with 8x8 / 8x8 / 4x4
@@ -259,10 +291,26 @@ def bench_3d(steps=5, warmup=1, nvol=4, n=128, sigma=25.0, group=8):
             "psnr_out": [round(psnr(p[0], res[i]), 3) for i, p in enumerate(pairs)],
             "band_group": group, "dict_bytes": sh.nbytes(), "plan_bytes": plan.nbytes(),
             "dict_prep_s": round(prep_s, 2), "noise_scalars_s": round(scalars_s, 2)}
+    if cpu:
+        info["cpu_baseline"] = cpu_baseline_3d(n, sigma)
     g.close()
     plan.close()
     sh.close()
     return info
+
+
+def measured_traffic(stage, L, sched):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
+    counter runs (profiles/hbm_traffic.json, FETCH_SIZE and WRITE_SIZE scaled
+    by a 256 MiB copy), or None when that kernel was not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    names = {"cols_rec": "shx::k_cols_rec<%d, %d>" % (L, sched.get("col_tile_rec", 16)),
+             "cols_dec": "shx::k_cols_dec<%d, %d>" % (L, sched.get("col_tile_dec", 16)),
+             "rows_c2r_r2c_bands": "shx::k_rows_c2r_r2c<%d>" % L}
+    k = json.load(open(path))["kernels"].get(names.get(stage, ""))
+    return round(k["hbm_bytes"]) if k else None
 
 
 def main():
@@ -308,7 +356,8 @@ def main():
     value = L * L / (ms * 1e-3) / 1e6
     dom = max(stages, key=lambda s: s["ms"])
     roof = {"kernel": dom["name"], "bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(dom["name"], L, sched),
             "bytes_per_launch": dom["bytes"], "ms_per_launch": round(dom["ms"], 5)}
     path_bytes = 400.0 * L * L
     path_gbs = path_bytes / (ms * 1e-3) / 1e9
@@ -334,7 +383,7 @@ def main():
     if not args.no_adjoint:
         res["adjoint"] = bench_adjoint()
     if not args.no_3d:
-        res["volume"] = bench_3d(steps=args.vol_steps, group=args.vol_group)
+        res["volume"] = bench_3d(steps=args.vol_steps, group=args.vol_group, cpu=args.cpu_budget > 0)
     if args.cpu_budget > 0:
         res["cpu_baseline"] = cpu_baseline(L, sigma, args.cpu_budget)
     print(json.dumps(res), flush=True)
