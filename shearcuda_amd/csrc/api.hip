@@ -418,9 +418,11 @@ void stages_spec_dec(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::v
   shx::ThreshSpec none{};
   st.push_back({"rows_r2c_scales", 2 * S * P,
                 [=] { return shx::launch_rows_r2c(L, yset, specset, S, nimg, none, d->tw, s); }});
+  st.push_back({"cols_fwd_scales", 2 * S * P,
+                [=] { return shx::launch_cols_fwd(L, w.spec, w.s_spec / 2, S, nimg, d->tw, s, w.ct_dec); }});
   st.push_back({"cols_dec", (S + 2.0 * nb) * P, [=] {
                   return shx::launch_cols_dec(L, w.spec, w.s_spec / 2, w.band, w.s_band / 2, d->Hmain, d->Hbeta,
-                                              w.map, nimg, d->tw, s, w.ct_dec);
+                                              w.map, nimg, d->tw, s, w.ct_dec, true);
                 }});
 }
 
@@ -586,8 +588,14 @@ void stages_denoise_dag(shx_dict* d, Work& w, int nimg, const float* x, long lon
     a.stream = sj;
     a.waits = {made[j]};
     st.push_back(a);
+    float2* Y1w = w.spec + (size_t)j * L * NH;
+    Stage cf{"cols_fwd" + sfx, 2 * P,
+             [=] { return shx::launch_cols_fwd(L, Y1w, w.s_spec / 2, 1, nimg, d->tw, sj, w.ct_dec); }};
+    cf.stream = sj;
+    st.push_back(cf);
     Stage b{"cols_dec" + sfx, (1 + 2.0 * nb) * P, [=] {
-              return shx::launch_cols_dec(L, Y1, w.s_spec / 2, T, w.s_band / 2, Hm, Hb, m, nimg, d->tw, sj, w.ct_dec);
+              return shx::launch_cols_dec(L, Y1, w.s_spec / 2, T, w.s_band / 2, Hm, Hb, m, nimg, d->tw, sj, w.ct_dec,
+                                          true);
             }};
     b.stream = sj;
     st.push_back(b);

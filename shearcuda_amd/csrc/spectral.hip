@@ -398,8 +398,47 @@ __device__ __forceinline__ void col_filter(float2 (&u)[Split<L>::R1], float2 (&o
   }
 }
 
-// Column pass of the decomposition: one block per (column tile, band, image).
+// XCD-aware block order of the column passes: workgroups are dispatched to
+// the 8 XCDs in turn, so consecutive grid ids land in different L2 caches.
+// The flat id is remapped so that each XCD takes a contiguous range of
+// (tile fastest, then image, then plane): neighbouring column tiles, which
+// share the 128-byte lines of narrow tiles, and the images sharing a filter
+// plane then run in the same L2.
+__device__ __forceinline__ void col_block_order(int& tile, int& plane, int& img) {
+  const int nt = gridDim.x, np = gridDim.y, ni = gridDim.z;
+  const int total = nt * np * ni;
+  int lid = blockIdx.x + nt * (blockIdx.y + np * blockIdx.z);
+  if (total % 8 == 0) lid = (lid % 8) * (total / 8) + lid / 8;
+  tile = lid % nt;
+  const int r = lid / nt;
+  img = r % ni;
+  plane = r / ni;
+}
+
+// Forward column FFT of every scale spectrum, in place: the column pass of the
+// decomposition then filters the transformed scale once per band instead of
+// transforming it again for each.  One block per (column tile, scale, image);
+// each block reads its whole tile before writing it back.
 template <int L, int CT_>
+__global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_fwd(float2* __restrict__ Y1, long long y1_stride,
+                                                            const float2* __restrict__ tw) {
+  using C = ColCfg<L, CT_>;
+  constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2;
+  __shared__ float2 buf[C::LDS];
+  int tile, j, b;
+  col_block_order(tile, j, b);
+  const int c0 = tile * C::CT;
+  float2* base = Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0;
+  float2 u[R1];
+  col_fwd_to_B<L, CT_>(u, base, buf, tw);
+  const int c = threadIdx.x % C::CT, k2 = threadIdx.x / C::CT;
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) base[(size_t)(k2 + R2 * k1) * NH + c] = u[k1];
+}
+
+// Column pass of the decomposition: one block per (column tile, band, image).
+// PRE: the scale spectrum is already column-transformed (k_cols_fwd).
+template <int L, int CT_, bool PRE>
 __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_dec(const float2* __restrict__ Y1, long long y1_stride,
                                                             float2* __restrict__ Tout, long long t_stride,
                                                             const float2* __restrict__ Hmain,
@@ -408,11 +447,19 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_dec(const float2* _
   using C = ColCfg<L, CT_>;
   constexpr int NH = L / 2;
   __shared__ float2 buf[C::LDS];
-  const int c0 = blockIdx.x * C::CT;
-  const int p = blockIdx.y, b = blockIdx.z;
+  int tile, p, b;
+  col_block_order(tile, p, b);
+  const int c0 = tile * C::CT;
   const int j = map.scale_of[p];
   float2 u[C::R1], q[C::R1];
-  col_fwd_to_B<L, CT_>(u, Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0, buf, tw);
+  const float2* src = Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0;
+  if constexpr (PRE) {
+    const int c = threadIdx.x % C::CT, k2 = threadIdx.x / C::CT;
+#pragma unroll
+    for (int k1 = 0; k1 < C::R1; ++k1) u[k1] = src[(size_t)(k2 + C::R2 * k1) * NH + c];
+  } else {
+    col_fwd_to_B<L, CT_>(u, src, buf, tw);
+  }
   col_filter<L, CT_, false, false>(u, q, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
   col_inv_from_B<L, CT_>(q, Tout + (size_t)b * t_stride + (size_t)p * L * NH + c0, buf, tw);
 }
@@ -438,8 +485,9 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* _
   constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
   __shared__ float2 buf[C::LDS];
   __shared__ float2 accl[ColRecAcc<L, CT_>::SIZE];
-  const int c0 = blockIdx.x * CT;
-  const int g = blockIdx.y, b = blockIdx.z;
+  int tile, g, b;
+  col_block_order(tile, g, b);
+  const int c0 = tile * CT;
   const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
   float2 acc[R1];
 #pragma unroll
@@ -497,9 +545,20 @@ struct Launch {
   }
   template <int CT>
   static hipError_t cols_dec_ct(const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
-                                const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_cols_dec<L, CT>), dim3(L / 2 / CT, map.nbands, nimg), dim3(ColCfg<L, CT>::T), 0, s, Y1, ys,
-                       To, ts, Hm, Hb, map, tw);
+                                const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s,
+                                bool pre) {
+    if (pre)
+      hipLaunchKernelGGL((k_cols_dec<L, CT, true>), dim3(L / 2 / CT, map.nbands, nimg), dim3(ColCfg<L, CT>::T), 0, s,
+                         Y1, ys, To, ts, Hm, Hb, map, tw);
+    else
+      hipLaunchKernelGGL((k_cols_dec<L, CT, false>), dim3(L / 2 / CT, map.nbands, nimg), dim3(ColCfg<L, CT>::T), 0,
+                         s, Y1, ys, To, ts, Hm, Hb, map, tw);
+    return hipGetLastError();
+  }
+  template <int CT>
+  static hipError_t cols_fwd_ct(float2* Y1, long long ys, int nplanes, int nimg, const float2* tw, hipStream_t s) {
+    hipLaunchKernelGGL((k_cols_fwd<L, CT>), dim3(L / 2 / CT, nplanes, nimg), dim3(ColCfg<L, CT>::T), 0, s, Y1, ys,
+                       tw);
     return hipGetLastError();
   }
   template <int CT>
@@ -511,10 +570,18 @@ struct Launch {
   }
   // ct: column tile width (8 or 16; 0 = default for L)
   static hipError_t cols_dec(const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
-                             const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct) {
+                             const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct,
+                             bool pre) {
     if (ct == 0) ct = ColTile<L>::dflt;
-    if (ct == 8) return cols_dec_ct<8>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s);
-    if (ct == 16 && ColTile<L>::wide) return cols_dec_ct<(ColTile<L>::wide ? 16 : 8)>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s);
+    if (ct == 8) return cols_dec_ct<8>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s, pre);
+    if (ct == 16 && ColTile<L>::wide)
+      return cols_dec_ct<(ColTile<L>::wide ? 16 : 8)>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s, pre);
+    return hipErrorInvalidValue;
+  }
+  static hipError_t cols_fwd(float2* Y1, long long ys, int nplanes, int nimg, const float2* tw, hipStream_t s, int ct) {
+    if (ct == 0) ct = ColTile<L>::dflt;
+    if (ct == 8) return cols_fwd_ct<8>(Y1, ys, nplanes, nimg, tw, s);
+    if (ct == 16 && ColTile<L>::wide) return cols_fwd_ct<(ColTile<L>::wide ? 16 : 8)>(Y1, ys, nplanes, nimg, tw, s);
     return hipErrorInvalidValue;
   }
   static hipError_t cols_rec(const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
@@ -554,8 +621,14 @@ hipError_t launch_rows_c2r_r2c(int L, const PlaneSet& Tin, const PlaneSet& band,
 }
 
 hipError_t launch_cols_dec(int L, const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
-                           const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct) {
-  SHX_BY_L(L, LT::cols_dec(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s, ct))
+                           const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct,
+                           bool pre) {
+  SHX_BY_L(L, LT::cols_dec(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s, ct, pre))
+}
+
+hipError_t launch_cols_fwd(int L, float2* Y1, long long ys, int nplanes, int nimg, const float2* tw, hipStream_t s,
+                           int ct) {
+  SHX_BY_L(L, LT::cols_fwd(Y1, ys, nplanes, nimg, tw, s, ct))
 }
 
 hipError_t launch_cols_rec(int L, const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
