@@ -215,6 +215,15 @@ int shx_soft_bands(const float* x, float* y, int nimg, int64_t img_elems, int64_
 int shx_soft_bands_backward(const float* x, const float* gy, float* gx, float* gth_dev, int nimg, int64_t img_elems,
                             int64_t rowlen, int64_t seglen, int band_off, const float* th_dev, void* stream);
 
+/* Per-band hard threshold y = x * 1[|x| > t] over the same batched layouts
+ * (nsst_HT.m:39 / thresh_cuda's keep rule, one threshold per band), and its
+ * input gradient gx = gy * 1[|x| > t].  The backward has no counterpart in the
+ * reference (gradients through the hard threshold). */
+int shx_hard_bands(const float* x, float* y, int nimg, int64_t img_elems, int64_t rowlen, int64_t seglen,
+                   int band_off, const float* th_dev, void* stream);
+int shx_hard_bands_backward(const float* x, const float* gy, float* gx, int nimg, int64_t img_elems, int64_t rowlen,
+                            int64_t seglen, int band_off, const float* th_dev, void* stream);
+
 /* Sum |x|^p over n elements, result (p-th root not taken) in *out (host).
  * norm_cuda's reduction (shearcuda/src_cuda/norm_cuda.cpp:172-215). */
 int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream);

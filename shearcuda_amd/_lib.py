@@ -73,6 +73,9 @@ _SIGNATURES = [
     ("shx_soft_bands", c_int, [c_void_p, c_void_p, c_int, c_int64, c_int64, c_int64, c_int, c_void_p, c_void_p]),
     ("shx_soft_bands_backward", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int64, c_int64, c_int64,
                                         c_int, c_void_p, c_void_p]),
+    ("shx_hard_bands", c_int, [c_void_p, c_void_p, c_int, c_int64, c_int64, c_int64, c_int, c_void_p, c_void_p]),
+    ("shx_hard_bands_backward", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int64, c_int64, c_int64, c_int,
+                                        c_void_p, c_void_p]),
     ("shx_norm_pow", c_int, [c_void_p, c_int64, c_double, P(c_double), c_void_p]),
     ("shx_softth", c_int, [c_void_p, c_int64, c_double, c_void_p]),
     ("shx_maxflat_factor_error", c_double, []),

@@ -242,6 +242,16 @@ __global__ void __launch_bounds__(256) k_soft_bands_bwd(const float* __restrict_
     if (acc[b] != 0.f) atomicAdd(&gth[b], acc[b]);
 }
 
+// Per-band hard threshold y = x * 1[|x| > t] and its input gradient
+// gx = gy * 1[|x| > t] (the threshold has no gradient almost everywhere).
+__global__ void k_hard_bands(const float* __restrict__ x, const float* __restrict__ gy, float* __restrict__ y,
+                             long long n, BandLayout bl, const float* __restrict__ th) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    const bool keep = fabsf(x[e]) > th[band_of(bl, e)];
+    y[e] = keep ? (gy ? gy[e] : x[e]) : 0.f;
+  }
+}
+
 hipError_t pyr_dec(const float* x, int L, int nlev, float* const* y, float* tA, float* tB, hipStream_t s) {
   const float* in = x;
   for (int lev = 0; lev < nlev; ++lev) {
@@ -1396,6 +1406,32 @@ int shx_re_weight_band(const float* x, int L, const double* coeff, int ncoeff, f
 }
 
 }  // extern "C"
+
+int shx_hard_bands(const float* x, float* y, int nimg, int64_t img_elems, int64_t rowlen, int64_t seglen,
+                   int band_off, const float* th_dev, void* stream) {
+  if (!x || !y || !th_dev) return fail(SHX_ERR_ARG, "null input, output or thresholds");
+  if (nimg < 1 || img_elems < 1 || rowlen < 1 || seglen < 1 || img_elems % rowlen || rowlen % seglen)
+    return fail(SHX_ERR_ARG, "invalid band layout");
+  const long long n = (long long)nimg * img_elems;
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(8192, (n + 255) / 256));
+  hipLaunchKernelGGL(k_hard_bands, dim3(blocks), dim3(256), 0, (hipStream_t)stream, x, nullptr, y, n,
+                     BandLayout{img_elems, rowlen, seglen, band_off}, th_dev);
+  SHX_HIP(hipGetLastError());
+  return SHX_OK;
+}
+
+int shx_hard_bands_backward(const float* x, const float* gy, float* gx, int nimg, int64_t img_elems, int64_t rowlen,
+                            int64_t seglen, int band_off, const float* th_dev, void* stream) {
+  if (!x || !gy || !gx || !th_dev) return fail(SHX_ERR_ARG, "null input, gradient or thresholds");
+  if (nimg < 1 || img_elems < 1 || rowlen < 1 || seglen < 1 || img_elems % rowlen || rowlen % seglen)
+    return fail(SHX_ERR_ARG, "invalid band layout");
+  const long long n = (long long)nimg * img_elems;
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(8192, (n + 255) / 256));
+  hipLaunchKernelGGL(k_hard_bands, dim3(blocks), dim3(256), 0, (hipStream_t)stream, x, gy, gx, n,
+                     BandLayout{img_elems, rowlen, seglen, band_off}, th_dev);
+  SHX_HIP(hipGetLastError());
+  return SHX_OK;
+}
 
 int shx_soft_bands(const float* x, float* y, int nimg, int64_t img_elems, int64_t rowlen, int64_t seglen,
                    int band_off, const float* th_dev, void* stream) {

@@ -42,6 +42,19 @@ def soft_bands_backward(x, gy, th, gth, nimg, img_elems, rowlen, seglen, band_of
     return gx
 
 
+def hard_bands(x, th, nimg, img_elems, rowlen, seglen, band_off=0, out=None):
+    out = torch.empty_like(x) if out is None else out
+    check(lib.shx_hard_bands(_vp(x), _vp(out), nimg, img_elems, rowlen, seglen, band_off, _vp(th), _stream()))
+    return out
+
+
+def hard_bands_backward(x, gy, th, nimg, img_elems, rowlen, seglen, band_off=0):
+    gx = torch.empty_like(x)
+    check(lib.shx_hard_bands_backward(_vp(x), _vp(gy), _vp(gx), nimg, img_elems, rowlen, seglen, band_off, _vp(th),
+                                      _stream()))
+    return gx
+
+
 class ShearShrink(torch.autograd.Function):
     """a (B, L, L) -> S_sh(soft(A_sh(a), th)) with per-band thresholds th."""
 

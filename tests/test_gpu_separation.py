@@ -109,3 +109,19 @@ def test_unrolled_gradients_match_fp64_finite_differences(mods):
         assert abs(gw[k] - fd) <= 2e-3 * np.abs(gw).max() + 1e-7
     plan.close()
     shear.close()
+
+
+def test_hard_bands_forward_backward(gpu):
+    print("case hard_bands", flush=True)
+    from shearcuda_amd import separation as sep
+    rng = np.random.default_rng(4)
+    nimg, nb, plane = 3, 5, 64
+    x = torch.tensor(rng.standard_normal((nimg, nb * plane)), dtype=torch.float32, device="cuda")
+    gy = torch.tensor(rng.standard_normal((nimg, nb * plane)), dtype=torch.float32, device="cuda")
+    th = torch.tensor(np.linspace(0.2, 1.2, nb), dtype=torch.float32, device="cuda")
+    y = sep.hard_bands(x, th, nimg, nb * plane, nb * plane, plane)
+    gx = sep.hard_bands_backward(x, gy, th, nimg, nb * plane, nb * plane, plane)
+    t = th.repeat_interleave(plane)[None, :]
+    keep = x.abs() > t
+    assert torch.equal(y, torch.where(keep, x, torch.zeros_like(x)))
+    assert torch.equal(gx, torch.where(keep, gy, torch.zeros_like(gy)))
