@@ -299,18 +299,21 @@ def bench_3d(steps=5, warmup=1, nvol=4, n=128, sigma=25.0, group=8, cpu=True):
     return info
 
 
-def measured_traffic(stage, L, sched):
+def measured_traffic(stage, L, sched, algo_bytes):
     """HBM bytes per launch of a stage's kernel from the committed rocprofv3
     counter runs (profiles/hbm_traffic.json, FETCH_SIZE and WRITE_SIZE scaled
-    by a 256 MiB copy), or None when that kernel was not measured."""
+    by a 256 MiB copy): the measured launch of the same kernel with the same
+    algorithmic bytes (same direction grouping), or None when not measured."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if not os.path.exists(path):
         return None
     names = {"cols_rec": "shx::k_cols_rec<%d, %d>" % (L, sched.get("col_tile_rec", 16)),
-             "cols_dec": "shx::k_cols_dec<%d, %d>" % (L, sched.get("col_tile_dec", 16)),
+             "cols_dec": "shx::k_cols_dec<%d, %d, true>" % (L, sched.get("col_tile_dec", 16)),
              "rows_c2r_r2c_bands": "shx::k_rows_c2r_r2c<%d>" % L}
-    k = json.load(open(path))["kernels"].get(names.get(stage, ""))
-    return round(k["hbm_bytes"]) if k else None
+    for k in json.load(open(path))["kernels"].get(names.get(stage, ""), []):
+        if k.get("algorithmic_bytes") == int(algo_bytes):
+            return round(k["hbm_bytes"])
+    return None
 
 
 def main():
@@ -357,7 +360,7 @@ def main():
     dom = max(stages, key=lambda s: s["ms"])
     roof = {"kernel": dom["name"], "bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(dom["name"], L, sched),
+            "traffic": measured_traffic(dom["name"], L, sched, dom["bytes"]),
             "bytes_per_launch": dom["bytes"], "ms_per_launch": round(dom["ms"], 5)}
     path_bytes = 400.0 * L * L
     path_gbs = path_bytes / (ms * 1e-3) / 1e9

@@ -176,26 +176,53 @@ template <> struct Split<1024> { static constexpr int R1 = 32, R2 = 32; };
 // (tw[m*TWS] = exp(-2 pi i m / N)).
 // For RB > 8 only w^r (r < 8) and w^(8q) are loaded and w^(8q + r) is one
 // product of the two, which keeps the twiddle registers few.
+// Pass twiddles of one thread: w^r (r < LO) and w^(LO q) (q < NHI).
+template <int RB>
+struct PassTw {
+  static constexpr int LO = RB < 8 ? RB : 8;
+  static constexpr int NHI = RB / LO;
+  float2 lo[LO];
+  float2 hi[NHI];
+};
+
+template <int RB, int DIR, int TWS>
+__device__ __forceinline__ void passA_tw(PassTw<RB>& t, int a, const float2* __restrict__ tw) {
+  constexpr int LO = PassTw<RB>::LO, NHI = PassTw<RB>::NHI;
+#pragma unroll
+  for (int r = 1; r < LO; ++r) t.lo[r] = tw[a * r * TWS];
+#pragma unroll
+  for (int q = 1; q < NHI; ++q) t.hi[q] = tw[a * q * LO * TWS];
+}
+
+template <int RB, int DIR>
+__device__ __forceinline__ void passA_apply(float2 (&v)[RB], PassTw<RB> t) {
+  constexpr int LO = PassTw<RB>::LO, NHI = PassTw<RB>::NHI;
+  if constexpr (DIR > 0) {
+#pragma unroll
+    for (int r = 1; r < LO; ++r) t.lo[r].y = -t.lo[r].y;
+#pragma unroll
+    for (int q = 1; q < NHI; ++q) t.hi[q].y = -t.hi[q].y;
+  }
+  dft<RB, DIR>(v);
+#pragma unroll
+  for (int r = 1; r < LO; ++r) v[r] = cmul(v[r], t.lo[r]);
+#pragma unroll
+  for (int q = 1; q < NHI; ++q) {
+    v[q * LO] = cmul(v[q * LO], t.hi[q]);
+#pragma unroll
+    for (int r = 1; r < LO; ++r) v[q * LO + r] = cmul(v[q * LO + r], cmul(t.hi[q], t.lo[r]));
+  }
+}
+
+// The twiddle loads are issued before the DFT, and a scheduling barrier keeps
+// them (and any loads of v the caller issued just before) ahead of the
+// arithmetic, so all of them are in flight together.
 template <int RA, int RB, int DIR, int TWS>
 __device__ __forceinline__ void passA(float2 (&v)[RB], int a, const float2* __restrict__ tw) {
-  dft<RB, DIR>(v);
-  constexpr int LO = RB < 8 ? RB : 8;
-  float2 lo[LO];
-#pragma unroll
-  for (int r = 1; r < LO; ++r) {
-    lo[r] = tw[a * r * TWS];
-    if constexpr (DIR > 0) lo[r].y = -lo[r].y;
-  }
-#pragma unroll
-  for (int r = 1; r < LO; ++r) v[r] = cmul(v[r], lo[r]);
-#pragma unroll
-  for (int q = 1; q < RB / LO; ++q) {
-    float2 hi = tw[a * q * LO * TWS];
-    if constexpr (DIR > 0) hi.y = -hi.y;
-    v[q * LO] = cmul(v[q * LO], hi);
-#pragma unroll
-    for (int r = 1; r < LO; ++r) v[q * LO + r] = cmul(v[q * LO + r], cmul(hi, lo[r]));
-  }
+  PassTw<RB> t;
+  passA_tw<RB, DIR, TWS>(t, a, tw);
+  __builtin_amdgcn_sched_barrier(0);
+  passA_apply<RB, DIR>(v, t);
 }
 
 }  // namespace shx

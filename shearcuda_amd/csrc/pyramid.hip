@@ -105,27 +105,49 @@ __device__ __forceinline__ void hpass(const float* __restrict__ row, const float
 }
 
 // Load a (TH+2H) x (TW+2H) sub-lattice tile centred on input rows R0.. and
-// columns C0..: symmetric fold (forward) or zero padding (adjoint).
-template <int TH, int TW, int T, bool ZERO>
-__device__ __forceinline__ void load_tile(float* __restrict__ tile, const float* __restrict__ x, int L, int m, int pr,
-                                          int pc, int R0, int C0, float th, float scale) {
-  constexpr int W = TW + 2 * HALO;
-  constexpr int NT = (TH + 2 * HALO) * W;
-  for (int idx = threadIdx.x; idx < NT; idx += T) {
-    const int rr = idx / W, cc = idx - rr * W;
+// columns C0..: symmetric fold (forward) or zero padding (adjoint).  Split in
+// two so that every global load of a block (one or two tiles) is issued
+// before the first is used: tile_fetch loads this thread's NIT elements into
+// registers (addresses clamped into the tile, no branch around a load),
+// tile_store writes them (threshold, scale) to LDS; the LDS tile is padded to
+// NIT * T elements so the stores need no guard either.
+template <int TH, int TW, int T>
+struct TileCfg {
+  static constexpr int W = TW + 2 * HALO;
+  static constexpr int NT = (TH + 2 * HALO) * W;
+  static constexpr int NIT = (NT + T - 1) / T;
+  static constexpr int PADDED = NIT * T;  // LDS tile size: every thread stores NIT elements
+};
+
+template <int TH, int TW, int T, bool ZERO, int NIT>
+__device__ __forceinline__ void tile_fetch(float (&v)[NIT], const float* __restrict__ x, int L, int m, int pr, int pc,
+                                           int R0, int C0) {
+  using TC = TileCfg<TH, TW, T>;
+  static_assert(NIT == TC::NIT, "register count of the tile");
+#pragma unroll
+  for (int i = 0; i < TC::NIT; ++i) {
+    const int idx = min((int)threadIdx.x + i * T, TC::NT - 1);
+    const int rr = idx / TC::W, cc = idx - rr * TC::W;
     int gr = pr + m * (R0 - HALO + rr);
     int gc = pc + m * (C0 - HALO + cc);
-    float v;
     if constexpr (ZERO) {
-      v = (gr >= 0 && gr < L && gc >= 0 && gc < L) ? x[(size_t)gr * L + gc] : 0.0f;
+      const bool in = gr >= 0 && gr < L && gc >= 0 && gc < L;
+      v[i] = x[in ? (size_t)gr * L + gc : 0];
+      if (!in) v[i] = 0.0f;
     } else {
       gr = fold_index(gr, L);
       gc = fold_index(gc, L);
-      v = x[(size_t)gr * L + gc];
-      if (th >= 0.f) v = hard_keep(v, th);
+      v[i] = x[(size_t)gr * L + gc];
     }
-    tile[idx] = v * scale;
   }
+}
+
+template <int TH, int TW, int T, int NIT>
+__device__ __forceinline__ void tile_store(float* __restrict__ tile, const float (&v)[NIT], float th, float scale) {
+  using TC = TileCfg<TH, TW, T>;
+  static_assert(NIT == TC::NIT, "register count of the tile");
+#pragma unroll
+  for (int i = 0; i < TC::NIT; ++i) tile[threadIdx.x + i * T] = (th >= 0.f ? hard_keep(v[i], th) : v[i]) * scale;
 }
 
 // Vertical passes of filter F into tv[k0 .. k0+RANK) for all tile columns.
@@ -166,7 +188,7 @@ __global__ void __launch_bounds__(T) k_pyr_one2two(const float* __restrict__ x, 
   using B = Filt<FB>;
   constexpr int W = TW + 2 * HALO;
   constexpr int WP = W | 1;  // odd row stride for the intermediates
-  __shared__ float tile[(TH + 2 * HALO) * W];
+  __shared__ float tile[TileCfg<TH, TW, T>::PADDED];
   __shared__ float tv[A::RANK + B::RANK][TH * WP];
   const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
   const int pr = ph / m, pc = ph - pr * m;
@@ -177,7 +199,10 @@ __global__ void __launch_bounds__(T) k_pyr_one2two(const float* __restrict__ x, 
   const int nout = E / m;
   const int R0 = blockIdx.y * TH, C0 = blockIdx.x * TW;
   const int shift = ADJ ? HALO : 0;  // extended row R' sits on input row R' - HALO
-  load_tile<TH, TW, T, ADJ>(tile, x, L, m, pr, pc, R0 - shift, C0 - shift, -1.0f, 1.0f);
+  float v[TileCfg<TH, TW, T>::NIT];
+  tile_fetch<TH, TW, T, ADJ>(v, x, L, m, pr, pc, R0 - shift, C0 - shift);
+  __builtin_amdgcn_sched_barrier(0);  // the tile's loads in flight together
+  tile_store<TH, TW, T>(tile, v, -1.0f, 1.0f);
   __syncthreads();
   vpasses<FA, TH, W, WP, VS, T, 0>(tile, tv, 0, 0, W * (TH / VS));
   vpasses<FB, TH, W, WP, VS, T, 0>(tile, tv, A::RANK, 0, W * (TH / VS));
@@ -220,7 +245,7 @@ __global__ void __launch_bounds__(T) k_pyr_two2one(const float* __restrict__ xa,
   using B = Filt<FB>;
   constexpr int W = TW + 2 * HALO;
   constexpr int WP = W | 1;
-  __shared__ float tile[2][(TH + 2 * HALO) * W];
+  __shared__ float tile[2][TileCfg<TH, TW, T>::PADDED];
   __shared__ float tv[A::RANK + B::RANK][TH * WP];
   const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
   const int pr = ph / m, pc = ph - pr * m;
@@ -232,8 +257,12 @@ __global__ void __launch_bounds__(T) k_pyr_two2one(const float* __restrict__ xa,
   const int nout = E / m;
   const int R0 = blockIdx.y * TH, C0 = blockIdx.x * TW;
   const int shift = ADJ ? HALO : 0;
-  load_tile<TH, TW, T, ADJ>(tile[0], xa, L, m, pr, pc, R0 - shift, C0 - shift, th, sa);
-  load_tile<TH, TW, T, ADJ>(tile[1], xb, L, m, pr, pc, R0 - shift, C0 - shift, -1.0f, sb);
+  float va[TileCfg<TH, TW, T>::NIT], vb[TileCfg<TH, TW, T>::NIT];
+  tile_fetch<TH, TW, T, ADJ>(va, xa, L, m, pr, pc, R0 - shift, C0 - shift);
+  tile_fetch<TH, TW, T, ADJ>(vb, xb, L, m, pr, pc, R0 - shift, C0 - shift);
+  __builtin_amdgcn_sched_barrier(0);  // both tiles' loads in flight together
+  tile_store<TH, TW, T>(tile[0], va, th, sa);
+  tile_store<TH, TW, T>(tile[1], vb, -1.0f, sb);
   __syncthreads();
   constexpr int NIT = W * (TH / VS);
   // the two vertical passes run on disjoint item ranges of one loop each

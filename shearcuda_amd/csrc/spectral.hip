@@ -96,16 +96,40 @@ struct RowCfg {
   static constexpr int LDS = lds_size<ROWS * N, PADSH>();
 };
 
+// Twiddles of row_fwd_from_A, loaded by the caller ahead of time: the pass-A
+// twiddles of index n1 and w_L^k2.  (A store issued before a load also counts
+// against the load's wait on this architecture, so loading these late would
+// wait for earlier stores too.)
+template <int L, int R1, int R2>
+struct RowFwdTw {
+  PassTw<R2> pt;
+  float2 w0;
+};
+template <int L, int R1, int R2>
+__device__ __forceinline__ RowFwdTw<L, R1, R2> row_fwd_tw(int n1, int k2, const float2* __restrict__ tw) {
+  RowFwdTw<L, R1, R2> t;
+  passA_tw<R2, -1, 2>(t.pt, n1, tw);
+  t.w0 = tw[k2];
+  return t;
+}
+
 // Forward real-row FFT from pass-A registers (thread (r, n1) holds
 // z[n1 + R1 n2]) to the packed half spectrum, stored to dst (row r).
 // All threads of the block must call it.
 template <int L, int R1, int R2, int PADSH>
 __device__ __forceinline__ void row_fwd_from_A(float2 (&v)[R2], bool validA, int r, int n1, float2* buf,
                                                float2* __restrict__ dst_rows, bool validB, int rB, int k2,
-                                               const float2* __restrict__ tw) {
+                                               const RowFwdTw<L, R1, R2>& t) {
   constexpr int N = L / 2;
   if (validA) {
-    passA<R1, R2, -1, 2>(v, n1, tw);
+    // keep the products of the preloaded twiddles here, after the data:
+    // computed earlier they would make the first wait cover the twiddle loads
+    PassTw<R2> pt = t.pt;
+#pragma unroll
+    for (int i = 1; i < PassTw<R2>::LO; ++i) asm volatile("" : "+v"(pt.lo[i].x), "+v"(pt.lo[i].y));
+#pragma unroll
+    for (int i = 1; i < PassTw<R2>::NHI; ++i) asm volatile("" : "+v"(pt.hi[i].x), "+v"(pt.hi[i].y));
+    passA_apply<R2, -1>(v, pt);
 #pragma unroll
     for (int k = 0; k < R2; ++k) buf[lds_ix<PADSH>(r * N + n1 + R1 * k)] = v[k];
   }
@@ -121,14 +145,17 @@ __device__ __forceinline__ void row_fwd_from_A(float2 (&v)[R2], bool validA, int
   const int lane = threadIdx.x & 63;
   const int src = (lane & ~(R2 - 1)) | ((R2 - k2) & (R2 - 1));
   float2* drow = dst_rows + (size_t)rB * N;
+  // w_L^k for k = k2 + R2 k1: the table value w_L^k2 times the compile-time
+  // w_L^(R2 k1) (one load per thread instead of one per output).  The values
+  // are computed on every lane and only the stores are predicated.
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) {
     const float2 sh = shfl2(u[R1 - 1 - k1], src);
-    if (validB) {
-      const int k = k2 + R2 * k1;
-      const float2 zm = (k2 == 0) ? u[(R1 - k1) & (R1 - 1)] : sh;
-      drow[k] = (k == 0) ? make_float2(u[0].x + u[0].y, u[0].x - u[0].y) : r2c_post(u[k1], zm, tw[k]);
-    }
+    const int k = k2 + R2 * k1;
+    const float2 zm = (k2 == 0) ? u[(R1 - k1) & (R1 - 1)] : sh;
+    const float2 val = (k == 0) ? make_float2(u[0].x + u[0].y, u[0].x - u[0].y)
+                                : r2c_post(u[k1], zm, ctw_mul<L / R2, -1>(t.w0, k1));
+    if (validB) drow[k] = val;
   }
 }
 
@@ -156,13 +183,18 @@ __device__ __forceinline__ void row_inv_to_B(float2 (&u)[R1], bool validA, int r
   if (validA) {
     float2 v[R2], xs[R2], xms[R2];
     gather(r, n1, xs, xms);
+    // w_L^k for k = n1 + R1 n2: table value w_L^n1 times compile-time w_L^(R1 n2)
+    const float2 w0 = tw[n1];
+    PassTw<R2> pt;
+    passA_tw<R2, +1, 2>(pt, n1, tw);
+    __builtin_amdgcn_sched_barrier(0);  // every load above in flight before the arithmetic
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const int k = n1 + R1 * n2;
       const float2 x = xs[n2], xm = xms[n2];
-      v[n2] = (k == 0) ? make_float2(x.x + x.y, x.x - x.y) : c2r_pre(x, xm, tw[k]);
+      v[n2] = (k == 0) ? make_float2(x.x + x.y, x.x - x.y) : c2r_pre(x, xm, ctw_mul<L / R1, -1>(w0, n2));
     }
-    passA<R1, R2, +1, 2>(v, n1, tw);
+    passA_apply<R2, +1>(v, pt);
 #pragma unroll
     for (int k = 0; k < R2; ++k) buf[lds_ix<PADSH>(r * N + n1 + R1 * k)] = v[k];
   }
@@ -195,13 +227,15 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_r2c(PlaneSet in, PlaneSet
   float2 v[R2];
   if (validA) {
 #pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) {
-      float2 x = src[(size_t)r * N + n1 + R1 * n2];
-      if (use_th) x = make_float2(hard_keep(x.x, t), hard_keep(x.y, t));
-      v[n2] = x;
-    }
+    for (int n2 = 0; n2 < R2; ++n2) v[n2] = src[(size_t)r * N + n1 + R1 * n2];
   }
-  row_fwd_from_A<L, R1, R2, C::PADSH>(v, validA, r, n1, buf, dst, true, tid / R2, tid % R2, tw);
+  const auto ftw = row_fwd_tw<L, R1, R2>(n1, tid % R2, tw);
+  __builtin_amdgcn_sched_barrier(0);  // data and twiddle loads in flight together
+  if (validA && use_th) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) v[n2] = make_float2(hard_keep(v[n2].x, t), hard_keep(v[n2].y, t));
+  }
+  row_fwd_from_A<L, R1, R2, C::PADSH>(v, validA, r, n1, buf, dst, true, tid / R2, tid % R2, ftw);
 }
 
 // Row complex-to-real inverse FFT of a packed spectrum (unnormalised: x L),
@@ -236,6 +270,7 @@ __global__ void __launch_bounds__(ROWS * Split<L / 2>::R2) k_rows_c2r(PlaneSet i
         float2 a[R2], am[R2];
         gather_row<N, R1, R2>(reinterpret_cast<const float2*>(plane_ptr(in, q, pe, b)) + (size_t)(row0 + rr) * N, nn,
                               a, am);
+        __builtin_amdgcn_sched_barrier(0);  // the group's loads in flight together
 #pragma unroll
         for (int n2 = 0; n2 < R2; ++n2) {
           x[n2] = cadd(x[n2], a[n2]);
@@ -275,6 +310,9 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
   const bool use_th = th.mode != 0;
   const float t = use_th ? thresh_of(th, p, b) : 0.0f;
   const int r = tid / R1, n1 = tid % R1, rB = tid / R2, k2 = tid % R2;
+  // twiddles of the forward half (split (R2, R1), pass-A index k2), loaded
+  // before any store of this block
+  const auto ftw = row_fwd_tw<L, R2, R1>(k2, tid % R1, tw);
   float2 u[R1];
   auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
     gather_row<N, R1, R2>(src + (size_t)rr * N, nn, x, xm);
@@ -288,7 +326,7 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
   }
   __syncthreads();  // pass-B reads of buf are complete
   // forward transform split (R2, R1): thread k2 is pass-A index n1' = k2
-  row_fwd_from_A<L, R2, R1, C::PADSH>(u, true, rB, k2, buf, udst, tid < C::ROWS * R1, tid / R1, tid % R1, tw);
+  row_fwd_from_A<L, R2, R1, C::PADSH>(u, true, rB, k2, buf, udst, tid < C::ROWS * R1, tid / R1, tid % R1, ftw);
 }
 
 // ---------------------------------------------------------------------------
@@ -313,14 +351,16 @@ struct ColCfg {
 };
 
 // Forward column transform: from global (row stride L/2) to pass-B layout
-// registers u[k1] = X[k2 + R2 k1] of thread (c, k2).
+// registers u[k1] = X[k2 + R2 k1] of thread (c, k2).  Pass A (global loads,
+// first DFT, LDS stores) and pass B (LDS loads, second DFT) are separate so a
+// caller can issue other loads between them; a barrier must separate them.
 template <int L, int CT_>
-__device__ __forceinline__ void col_fwd_to_B(float2 (&u)[Split<L>::R1], const float2* __restrict__ src,
-                                             float2* buf, const float2* __restrict__ tw) {
+__device__ __forceinline__ void col_fwd_A(const float2* __restrict__ src, float2* buf,
+                                          const float2* __restrict__ tw) {
   using C = ColCfg<L, CT_>;
   constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
   const int tid = threadIdx.x;
-  const int c = tid % CT, n1 = tid / CT, k2 = tid / CT;
+  const int c = tid % CT, n1 = tid / CT;
   if (n1 < R1) {
     float2 v[R2];
 #pragma unroll
@@ -329,10 +369,25 @@ __device__ __forceinline__ void col_fwd_to_B(float2 (&u)[Split<L>::R1], const fl
 #pragma unroll
     for (int k = 0; k < R2; ++k) buf[lds_ix<C::PADSH>((n1 + R1 * k) * CT + c)] = v[k];
   }
-  __syncthreads();
+}
+
+template <int L, int CT_>
+__device__ __forceinline__ void col_fwd_B(float2 (&u)[Split<L>::R1], const float2* buf) {
+  using C = ColCfg<L, CT_>;
+  constexpr int R1 = C::R1, CT = C::CT;
+  const int tid = threadIdx.x;
+  const int c = tid % CT, k2 = tid / CT;
 #pragma unroll
   for (int a = 0; a < R1; ++a) u[a] = buf[lds_ix<C::PADSH>((a + R1 * k2) * CT + c)];
   dft<R1, -1>(u);
+}
+
+template <int L, int CT_>
+__device__ __forceinline__ void col_fwd_to_B(float2 (&u)[Split<L>::R1], const float2* __restrict__ src,
+                                             float2* buf, const float2* __restrict__ tw) {
+  col_fwd_A<L, CT_>(src, buf, tw);
+  __syncthreads();
+  col_fwd_B<L, CT_>(u, buf);
 }
 
 // Inverse column transform from pass-B layout registers (thread (c, k2)
@@ -361,18 +416,33 @@ __device__ __forceinline__ void col_inv_from_B(float2 (&q)[Split<L>::R1], float2
   }
 }
 
-// Filter pass-B registers by band spectrum H (CONJ: its conjugate).  In the
-// block holding column 0 the packed (DC, Nyquist) column needs X(-u): it is
-// exchanged through LDS (block-uniform branch).
-template <int L, int CT_, bool CONJ, bool ACC>
-__device__ __forceinline__ void col_filter(float2 (&u)[Split<L>::R1], float2 (&out)[Split<L>::R1], int c0,
-                                           const float2* __restrict__ H, const float2* __restrict__ Hb, float2* buf) {
+// Band spectrum H in pass-B layout: h[k1] = H(k2 + R2 k1, c).  Issued as one
+// batch of independent loads, ahead of the work that hides their latency.
+template <int L, int CT_>
+__device__ __forceinline__ void col_load_h(float2 (&h)[Split<L>::R1], const float2* __restrict__ H) {
   using C = ColCfg<L, CT_>;
   constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
+  const unsigned c = threadIdx.x % CT, k2 = threadIdx.x / CT;
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) h[k1] = H[(k2 + R2 * k1) * NH + c];
+}
+
+// Filter pass-B registers by the band spectrum h (col_load_h; CONJ: its
+// conjugate).  In the block holding column 0 the packed (DC, Nyquist) column
+// needs X(-u): it is exchanged through LDS (block-uniform branch), and the
+// beta term is added on lane column 0 by a select.
+template <int L, int CT_, bool CONJ, bool ACC>
+__device__ __forceinline__ void col_filter(float2 (&u)[Split<L>::R1], float2 (&out)[Split<L>::R1], int c0,
+                                           const float2 (&h)[Split<L>::R1], const float2* __restrict__ Hb,
+                                           float2* buf) {
+  using C = ColCfg<L, CT_>;
+  constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT;
   const int tid = threadIdx.x;
   const int c = tid % CT, k2 = tid / CT;
-  float2 pm[R1];
   if (c0 == 0) {
+    float2 hb[R1];
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) hb[k1] = Hb[k2 + R2 * k1];
     __syncthreads();
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) buf[lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c)] = u[k1];
@@ -380,23 +450,34 @@ __device__ __forceinline__ void col_filter(float2 (&u)[Split<L>::R1], float2 (&o
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       const int uu = k2 + R2 * k1;
-      pm[k1] = cconj(buf[lds_ix<C::PADSH>(((L - uu) & (L - 1)) * CT)]);
+      const float2 pm = cconj(buf[lds_ix<C::PADSH>(((L - uu) & (L - 1)) * CT)]);
+      float2 q = CONJ ? cmulc(u[k1], h[k1]) : cmul(h[k1], u[k1]);
+      const float2 e = CONJ ? cmulc(pm, hb[k1]) : cmul(hb[k1], pm);
+      if (c == 0) q = cadd(q, e);
+      out[k1] = ACC ? cadd(out[k1], q) : q;
     }
-  }
+  } else {
 #pragma unroll
-  for (int k1 = 0; k1 < R1; ++k1) {
-    const int uu = k2 + R2 * k1;
-    const float2 h = H[(size_t)uu * NH + c];
-    float2 q;
-    if (c0 == 0 && c == 0) {
-      const float2 hb = Hb[uu];
-      q = CONJ ? cadd(cmulc(u[k1], h), cmulc(pm[k1], hb)) : cadd(cmul(h, u[k1]), cmul(hb, pm[k1]));
-    } else {
-      q = CONJ ? cmulc(u[k1], h) : cmul(h, u[k1]);
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const float2 q = CONJ ? cmulc(u[k1], h[k1]) : cmul(h[k1], u[k1]);
+      out[k1] = ACC ? cadd(out[k1], q) : q;
     }
-    out[k1] = ACC ? cadd(out[k1], q) : q;
   }
 }
+
+// Wave-uniform reads of the byte and halfword tables of a BandMap kernel
+// argument as aligned dwords, so they become scalar loads (the scalar unit has
+// no sub-dword loads).
+__device__ __forceinline__ int map_u8(const unsigned char* a, int i) {
+  const unsigned w = reinterpret_cast<const unsigned*>(a)[i >> 2];
+  return (w >> (8 * (i & 3))) & 0xffu;
+}
+__device__ __forceinline__ int map_u16(const unsigned short* a, int i) {
+  const unsigned w = reinterpret_cast<const unsigned*>(a)[i >> 1];
+  return (i & 1) ? (w >> 16) : (w & 0xffffu);
+}
+static_assert(offsetof(BandMap, scale_of) % 4 == 0 && offsetof(BandMap, group_band0) % 4 == 0,
+              "BandMap tables must be dword aligned");
 
 // XCD-aware block order of the column passes: workgroups are dispatched to
 // the 8 XCDs in turn, so consecutive grid ids land in different L2 caches.
@@ -409,10 +490,12 @@ __device__ __forceinline__ void col_block_order(int& tile, int& plane, int& img)
   const int total = nt * np * ni;
   int lid = blockIdx.x + nt * (blockIdx.y + np * blockIdx.z);
   if (total % 8 == 0) lid = (lid % 8) * (total / 8) + lid / 8;
-  tile = lid % nt;
+  // the block index arithmetic runs on the vector unit; the results are
+  // wave-uniform, so move them to scalar registers (scalar table lookups)
+  tile = __builtin_amdgcn_readfirstlane(lid % nt);
   const int r = lid / nt;
-  img = r % ni;
-  plane = r / ni;
+  img = __builtin_amdgcn_readfirstlane(r % ni);
+  plane = __builtin_amdgcn_readfirstlane(r / ni);
 }
 
 // Forward column FFT of every scale spectrum, in place: the column pass of the
@@ -450,17 +533,21 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_dec(const float2* _
   int tile, p, b;
   col_block_order(tile, p, b);
   const int c0 = tile * C::CT;
-  const int j = map.scale_of[p];
+  const int j = map_u8(map.scale_of, p);
   float2 u[C::R1], q[C::R1];
   const float2* src = Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0;
+  float2 h[C::R1];
   if constexpr (PRE) {
-    const int c = threadIdx.x % C::CT, k2 = threadIdx.x / C::CT;
+    const unsigned c = threadIdx.x % C::CT, k2 = threadIdx.x / C::CT;
 #pragma unroll
-    for (int k1 = 0; k1 < C::R1; ++k1) u[k1] = src[(size_t)(k2 + C::R2 * k1) * NH + c];
+    for (int k1 = 0; k1 < C::R1; ++k1) u[k1] = src[(k2 + C::R2 * k1) * NH + c];
+    col_load_h<L, CT_>(h, Hmain + (size_t)p * L * NH + c0);
+    __builtin_amdgcn_sched_barrier(0);  // all loads in flight before the arithmetic
   } else {
+    col_load_h<L, CT_>(h, Hmain + (size_t)p * L * NH + c0);
     col_fwd_to_B<L, CT_>(u, src, buf, tw);
   }
-  col_filter<L, CT_, false, false>(u, q, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
+  col_filter<L, CT_, false, false>(u, q, c0, h, Hbeta + (size_t)p * L, buf);
   col_inv_from_B<L, CT_>(q, Tout + (size_t)b * t_stride + (size_t)p * L * NH + c0, buf, tw);
 }
 
@@ -492,21 +579,24 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* _
   float2 acc[R1];
 #pragma unroll
   for (int k = 0; k < R1; ++k) acc[k] = make_float2(0.f, 0.f);
-  const int p0 = map.group_band0[g], p1 = map.group_band0[g + 1];
+  const int p0 = map_u16(map.group_band0, g), p1 = map_u16(map.group_band0, g + 1);
   for (int p = p0; p < p1; ++p) {
-    float2 u[R1];
+    float2 u[R1], h[R1];
     __syncthreads();  // previous direction's reads of buf are complete
-    col_fwd_to_B<L, CT_>(u, U + (size_t)b * u_stride + (size_t)p * L * NH + c0, buf, tw);
+    col_fwd_A<L, CT_>(U + (size_t)b * u_stride + (size_t)p * L * NH + c0, buf, tw);
+    col_load_h<L, CT_>(h, Hmain + (size_t)p * L * NH + c0);  // in flight across pass B
+    __syncthreads();
+    col_fwd_B<L, CT_>(u, buf);
     if constexpr (ColRecAcc<L, CT_>::LDS) {
       float2 q[R1];
-      col_filter<L, CT_, true, false>(u, q, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
+      col_filter<L, CT_, true, false>(u, q, c0, h, Hbeta + (size_t)p * L, buf);
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) {
         const int e = lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c);
         accl[e] = (p == p0) ? q[k1] : cadd(accl[e], q[k1]);
       }
     } else {
-      col_filter<L, CT_, true, true>(u, acc, c0, Hmain + (size_t)p * L * NH + c0, Hbeta + (size_t)p * L, buf);
+      col_filter<L, CT_, true, true>(u, acc, c0, h, Hbeta + (size_t)p * L, buf);
     }
   }
   if constexpr (ColRecAcc<L, CT_>::LDS) {
