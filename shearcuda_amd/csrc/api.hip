@@ -807,33 +807,54 @@ int profile_stages(const std::vector<Stage>& st, hipStream_t s, int reps, int ma
   return SHX_OK;
 }
 
-// Mean duration (ms) of `reps` runs of a stage list.
-double time_stages(const std::vector<Stage>& st, hipStream_t s, int reps) {
-  hipEvent_t a, b;
-  if (hipEventCreate(&a) != hipSuccess) return -1.0;
-  if (hipEventCreate(&b) != hipSuccess) {
-    hipEventDestroy(a);
+// Mean duration (ms) of `reps` replays of a stage list captured as one graph
+// on stream cs: the way plans run in practice (no host launch cost between
+// kernels), so short kernels are compared by their own durations.
+double time_stages_graph(const std::vector<Stage>& st, hipStream_t cs, int reps) {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return -1.0;
+  const int rc = run_stages(st);
+  if (hipStreamEndCapture(cs, &g) != hipSuccess || rc != SHX_OK) {
+    if (g) hipGraphDestroy(g);
     return -1.0;
   }
   double ms = -1.0;
-  if (run_stages(st) == SHX_OK && hipEventRecord(a, s) == hipSuccess) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess && hipEventCreate(&a) == hipSuccess &&
+      hipEventCreate(&b) == hipSuccess && hipGraphLaunch(ex, cs) == hipSuccess &&
+      hipEventRecord(a, cs) == hipSuccess) {
     bool ok = true;
-    for (int r = 0; r < reps && ok; ++r) ok = run_stages(st) == SHX_OK;
+    for (int r = 0; r < reps && ok; ++r) ok = hipGraphLaunch(ex, cs) == hipSuccess;
     float t = 0.f;
-    if (ok && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+    if (ok && hipEventRecord(b, cs) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
         hipEventElapsedTime(&t, a, b) == hipSuccess)
       ms = t / reps;
   }
-  hipEventDestroy(a);
-  hipEventDestroy(b);
+  hipStreamSynchronize(cs);
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  if (ex) hipGraphExecDestroy(ex);
+  hipGraphDestroy(g);
   return ms;
 }
 
 // Autotune one plan: pyramid tile shape, then reconstruction direction
-// groups, each by timing the kernels it affects on the plan's own workspace
-// and scratch images.
-int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t s, int* chosen) {
+// groups, each by timing graph replays of the kernels it affects on the
+// plan's own workspace and scratch images.  The timing runs on a private
+// stream after the caller's stream has drained.
+int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t caller, int* chosen) {
   const long long pl = (long long)d->L * d->L;
+  SHX_HIP(hipStreamSynchronize(caller));
+  hipStream_t s = nullptr;
+  SHX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() {
+      hipStreamSynchronize(s);
+      hipStreamDestroy(s);
+    }
+  } guard{s};
   float* img = nullptr;
   if (hipMalloc(&img, sizeof(float) * 2 * pl * nimg) != hipSuccess)
     return fail(SHX_ERR_ALLOC, "autotune scratch allocation failed");
@@ -856,7 +877,7 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t s, int* 
       std::vector<Stage> st;
       stages_pyr_dec(d, w, nimg, img, pl, c, s, st);
       stages_pyr_rec(d, w, nimg, c, Thr{}, img + pl * nimg, pl, s, st);
-      const double t = time_stages(st, s, reps);
+      const double t = time_stages_graph(st, s, reps);
       if (t > 0 && t < tp[v]) tp[v] = t;
     }
   int bestv = 0;
@@ -876,7 +897,7 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t s, int* 
         w.ct_rec = ccand[c];
         std::vector<Stage> st;
         stages_spec_rec_tail(d, w, nimg, s, st);
-        const double t = time_stages(st, s, reps);
+        const double t = time_stages_graph(st, s, reps);
         if (t > 0 && t < tg[i][c]) tg[i][c] = t;
       }
   int bi = 0, bc = 0;
@@ -891,7 +912,7 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t s, int* 
       w.ct_dec = ccand[c];
       std::vector<Stage> st;
       stages_spec_dec(d, w, nimg, s, st);
-      const double t = time_stages(st, s, reps);
+      const double t = time_stages_graph(st, s, reps);
       if (t > 0 && t < td[c]) td[c] = t;
     }
   w.ct_dec = (nct == 2 && td[1] < td[0]) ? 16 : 8;

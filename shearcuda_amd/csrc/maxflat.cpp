@@ -2,11 +2,20 @@
 //
 // The quadrant tables are numeric data that must equal the reference's filter
 // design (shearlet_toolbox_1/atrousfilters.m:109-164); they are mirrored to
-// 13x13 (h0, g1) and 19x19 (h1, g0) as atrousfilters.m:167-175 does.  Each full
-// filter is symmetric, so h = sum_k lambda_k u_k u_k^T; a cyclic Jacobi
-// eigensolver in FP64 finds the nonzero terms (h0: 2, h1: 4, g0: 3, g1: 3)
-// and the kernels use v_k = u_k, w_k = lambda_k u_k (written as FP32
-// compile-time constants by gen_factors.cpp).
+// 13x13 (h0, g1) and 19x19 (h1, g0) as atrousfilters.m:167-175 does.
+//
+// A pyramid level applies a filter pair: (h0, h1) in the decomposition,
+// (g0, g1) in the reconstruction.  Zero-padded to 19x19, the columns of
+// [h0 | h1 - delta] span only a 3-dimensional space, and so do those of
+// [g0 | g1 - delta] (delta: the unit impulse at the centre).  With an
+// orthonormal basis u_1..u_3 of that space (pivoted modified Gram-Schmidt in
+// FP64), each filter F of the pair (the second one minus delta) is
+//   F = sum_i u_i (F u_i)^T,
+// exactly, and since every maxflat filter is symmetric (F = F^T) also
+//   F = sum_i (F u_i) u_i^T.
+// A level therefore needs three separable passes shared by both filters, plus
+// the identity term (written as FP32 compile-time constants by
+// gen_factors.cpp).
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -83,85 +92,99 @@ Full mirror(const double* q, int n) {
   return f;
 }
 
-// Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major).
-void jacobi(int n, std::vector<double> a, std::vector<double>& eval, std::vector<double>& evec) {
-  evec.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) evec[(size_t)i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int p = 0; p < n; ++p)
-      for (int q = 0; q < n; ++q) {
-        const double v = a[(size_t)p * n + q] * a[(size_t)p * n + q];
-        tot += v;
-        if (p != q) off += v;
-      }
-    if (off <= 1e-32 * tot) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = a[(size_t)p * n + q];
-        if (std::fabs(apq) < 1e-300) continue;
-        const double theta = (a[(size_t)q * n + q] - a[(size_t)p * n + p]) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < n; ++k) {  // columns p, q
-          const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
-          a[(size_t)k * n + p] = c * akp - s * akq;
-          a[(size_t)k * n + q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {  // rows p, q
-          const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
-          a[(size_t)p * n + k] = c * apk - s * aqk;
-          a[(size_t)q * n + k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double vkp = evec[(size_t)k * n + p], vkq = evec[(size_t)k * n + q];
-          evec[(size_t)k * n + p] = c * vkp - s * vkq;
-          evec[(size_t)k * n + q] = s * vkp + c * vkq;
-        }
-      }
-  }
-  eval.resize(n);
-  for (int i = 0; i < n; ++i) eval[i] = a[(size_t)i * n + i];
+Full pad_to(const Full& f, int n) {
+  Full g;
+  g.len = n;
+  g.h.assign((size_t)n * n, 0.0);
+  const int o = (n - f.len) / 2;
+  for (int r = 0; r < f.len; ++r)
+    for (int c = 0; c < f.len; ++c) g.h[(size_t)(r + o) * n + c + o] = f.h[(size_t)r * f.len + c];
+  return g;
 }
 
-struct Factors {
+struct PairFactors {
   int len = 0, rank = 0;
-  std::vector<double> v, w;  // rank x len
+  std::vector<double> u, a, b;  // rank x len: basis, A u_i, (B - delta) u_i
   double err = 0.0;
 };
 
-Factors factorize(const Full& f) {
-  const int n = f.len;
-  std::vector<double> ev, V;
-  jacobi(n, f.h, ev, V);
-  std::vector<int> idx(n);
-  for (int i = 0; i < n; ++i) idx[i] = i;
-  std::sort(idx.begin(), idx.end(), [&](int x, int y) { return std::fabs(ev[x]) > std::fabs(ev[y]); });
-  const double top = std::fabs(ev[idx[0]]);
-  Factors F;
-  F.len = n;
-  for (int i : idx) {
-    if (std::fabs(ev[i]) <= 1e-12 * top) break;
-    ++F.rank;
-    for (int a = 0; a < n; ++a) {
-      double u = V[(size_t)a * n + i];
-      if (a == 1 || a == n - 2) u = 0.0;  // exact zero rows of every maxflat filter
-      F.v.push_back(u);
-    }
-    for (int a = 0; a < n; ++a) F.w.push_back(ev[i] * F.v[(size_t)(F.rank - 1) * n + a]);
-  }
-  for (int r = 0; r < n; ++r)
+// Joint factors of (A, B - delta) on a common odd side n (see the header).
+PairFactors pair_factorize(const Full& A0, const Full& B0, int n) {
+  const Full A = pad_to(A0, n);
+  const Full B = [&] {
+    Full t = pad_to(B0, n);
+    t.h[(size_t)(n / 2) * n + n / 2] -= 1.0;
+    return t;
+  }();
+  // the 2n columns of [A | B]
+  std::vector<std::vector<double>> cols;
+  for (const Full* f : {&A, &B})
     for (int c = 0; c < n; ++c) {
-      double s = 0.0;
-      for (int k = 0; k < F.rank; ++k) s += F.v[(size_t)k * n + r] * F.w[(size_t)k * n + c];
-      F.err = std::max(F.err, std::fabs(s - f.h[(size_t)r * n + c]));
+      std::vector<double> v(n);
+      for (int r = 0; r < n; ++r) v[r] = f->h[(size_t)r * n + c];
+      cols.push_back(v);
     }
-  return F;
+  auto norm = [](const std::vector<double>& v) {
+    double s = 0.0;
+    for (double x : v) s += x * x;
+    return std::sqrt(s);
+  };
+  double top = 0.0;
+  for (const auto& v : cols) top = std::max(top, norm(v));
+  PairFactors P;
+  P.len = n;
+  std::vector<std::vector<double>> basis;
+  while ((int)basis.size() < n) {
+    int best = -1;
+    double bn = 0.0;
+    for (size_t j = 0; j < cols.size(); ++j) {
+      const double t = norm(cols[j]);
+      if (t > bn) bn = t, best = (int)j;
+    }
+    if (best < 0 || bn <= 1e-13 * top) break;
+    std::vector<double> q = cols[best];
+    for (double& x : q) x /= bn;
+    for (int pass = 0; pass < 2; ++pass)  // re-orthogonalise against the earlier vectors
+      for (const auto& e : basis) {
+        double d = 0.0;
+        for (int r = 0; r < n; ++r) d += e[r] * q[r];
+        for (int r = 0; r < n; ++r) q[r] -= d * e[r];
+      }
+    const double qn = norm(q);
+    for (double& x : q) x /= qn;
+    for (auto& v : cols) {
+      double d = 0.0;
+      for (int r = 0; r < n; ++r) d += q[r] * v[r];
+      for (int r = 0; r < n; ++r) v[r] -= d * q[r];
+    }
+    basis.push_back(q);
+  }
+  P.rank = (int)basis.size();
+  for (const auto& q : basis) P.u.insert(P.u.end(), q.begin(), q.end());
+  for (const Full* f : {&A, &B}) {
+    std::vector<double>& out = (f == &A) ? P.a : P.b;
+    for (const auto& q : basis)
+      for (int c = 0; c < n; ++c) {  // (F^T q)[c]; F is symmetric
+        double s = 0.0;
+        for (int r = 0; r < n; ++r) s += f->h[(size_t)r * n + c] * q[r];
+        out.push_back(s);
+      }
+  }
+  for (const Full* f : {&A, &B}) {
+    const std::vector<double>& w = (f == &A) ? P.a : P.b;
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < n; ++c) {
+        double s = 0.0;
+        for (int k = 0; k < P.rank; ++k) s += P.u[(size_t)k * n + r] * w[(size_t)k * n + c];
+        P.err = std::max(P.err, std::fabs(s - f->h[(size_t)r * n + c]));
+      }
+  }
+  return P;
 }
 
 struct Maxflat {
   Full full[4];
-  Factors fac[4];
+  PairFactors pair[2];  // (h0, h1), (g0, g1)
   bool ok = false;
 };
 
@@ -173,11 +196,10 @@ const Maxflat& maxflat() {
     M.full[1] = mirror(&kH1Q[0][0], 10);
     M.full[2] = mirror(&kG0Q[0][0], 10);
     M.full[3] = mirror(&kG1Q[0][0], 7);
-    for (int i = 0; i < 4; ++i) M.fac[i] = factorize(M.full[i]);
-    // ranks the pyramid kernels are specialised for
-    const int want[4] = {2, 4, 3, 3}, len[4] = {13, 19, 19, 13};
-    M.ok = true;
-    for (int i = 0; i < 4; ++i) M.ok = M.ok && M.fac[i].rank == want[i] && M.fac[i].len == len[i];
+    M.pair[0] = pair_factorize(M.full[0], M.full[1], 19);
+    M.pair[1] = pair_factorize(M.full[2], M.full[3], 19);
+    // the rank the pyramid kernels are specialised for
+    M.ok = M.pair[0].rank == 3 && M.pair[1].rank == 3;
   });
   return M;
 }
@@ -193,22 +215,21 @@ const double* maxflat_filter(int which, int* len) {
   return f.h.data();
 }
 
-bool maxflat_factors_fp64(int which, int* len, int* rank, const double** v, const double** w) {
-  if (which < 0 || which > 3) return false;
-  const Factors& F = maxflat().fac[which];
-  *len = F.len;
-  *rank = F.rank;
-  *v = F.v.data();
-  *w = F.w.data();
+bool maxflat_pair_fp64(int pair, int* len, int* rank, const double** u, const double** a, const double** b) {
+  if (pair < 0 || pair > 1) return false;
+  const PairFactors& P = maxflat().pair[pair];
+  *len = P.len;
+  *rank = P.rank;
+  *u = P.u.data();
+  *a = P.a.data();
+  *b = P.b.data();
   return true;
 }
 
 double maxflat_factor_error() {
   const Maxflat& M = maxflat();
   if (!M.ok) return INFINITY;
-  double e = 0.0;
-  for (int i = 0; i < 4; ++i) e = std::max(e, M.fac[i].err);
-  return e;
+  return std::max(M.pair[0].err, M.pair[1].err);
 }
 
 }  // namespace shx

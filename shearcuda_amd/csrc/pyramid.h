@@ -7,11 +7,13 @@ namespace shx {
 
 // Full FP64 filters, row-major LEN x LEN: which = 0 h0, 1 h1, 2 g0, 3 g1.
 const double* maxflat_filter(int which, int* len);
-// Largest |h - sum_k v_k w_k^T| over the four filters (FP64 factors).
+// Largest |F - sum_i u_i f_i^T| over both filter pairs (FP64 factors).
 double maxflat_factor_error();
-// FP64 separable factors of filter `which` (v, w: rank x len, row-major).
-bool maxflat_factors_fp64(int which, int* len, int* rank, const double** v, const double** w);
-// True when every filter has the rank the kernels are specialised for.
+// FP64 joint factors of pair 0 (h0, h1) or 1 (g0, g1) on a 19 x 19 grid:
+// u (shared basis), a (first filter), b (second filter minus the centre
+// impulse), each rank x len, row-major; F(r, c) = sum_i u_i[r] f_i[c].
+bool maxflat_pair_fp64(int pair, int* len, int* rank, const double** u, const double** a, const double** b);
+// True when both pairs have the rank the kernels are specialised for.
 bool maxflat_ok();
 
 // Per-image element strides of the three planes a level touches

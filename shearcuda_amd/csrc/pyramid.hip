@@ -20,11 +20,19 @@
 // a fold-add of the extended grid back onto [0, L)^2 (the transpose of the
 // symmetric extension).
 //
-// The maxflat filters have exact low rank (h0: 2, h1: 4, g0: 3, g1: 3), so
-// h = sum_k v_k (x) w_k is applied as RANK vertical 1D passes into LDS
-// followed by RANK horizontal 1D passes, skipping the taps 1 and LEN-2 that
-// are zero in every maxflat filter.  The factors are compile-time constants
-// (maxflat_factors.h), so every tap is an FMA with a literal operand.
+// A level applies a filter pair, (h0, h1) or (g0, g1).  On a common 19 x 19
+// support the pair has joint rank 3 once the centre impulse is taken out of
+// the second filter (maxflat.cpp), with shared basis vectors u_i:
+//   F1 = sum_i u_i (x) a_i,   F2 = delta + sum_i u_i (x) b_i
+// (vertical (x) horizontal), and, the filters being symmetric, also
+//   F1 = sum_i a_i (x) u_i,   F2 = delta + sum_i b_i (x) u_i.
+// One input, two outputs (decomposition): three vertical passes with u_i into
+// LDS, then each output takes its three horizontal passes from the same
+// loaded rows.  Two inputs, one output (reconstruction): three vertical
+// passes a_i * x1 + b_i * x2 into LDS, then three horizontal passes with u_i.
+// The identity term is read from the staged input tile.  The factors are
+// compile-time constants (maxflat_factors.h): every tap is an FMA with a
+// literal operand, and taps that are exactly zero are skipped at compile time.
 #include "maxflat_factors.h"  // generated at build time by gen_factors.cpp
 #include "pyramid.h"
 
@@ -38,68 +46,46 @@ __device__ __forceinline__ int fold_index(int s, int n) {
   return t < n ? t : 2 * n - 1 - t;
 }
 
-template <int LEN>
-__device__ __forceinline__ constexpr bool zero_tap(int a) {
-  return a == 1 || a == LEN - 2;
-}
-
-// Filter traits: 0 = h0, 1 = h1, 2 = g0, 3 = g1 (maxflat_factors.h).
-template <int F>
-struct Filt;
+// Joint factors of the pairs: 0 = (h0, h1), 1 = (g0, g1).
+constexpr int PLEN = mf::PAIR_LEN, PRANK = mf::PAIR_RANK;
+static_assert(PLEN == 2 * HALO + 1, "the pair factors span the halo");
+template <int P>
+struct Pair;
 template <>
-struct Filt<0> {
-  static constexpr int LEN = mf::H0_LEN, RANK = mf::H0_RANK;
-  static constexpr const float (&V)[RANK][LEN] = mf::H0_V;
-  static constexpr const float (&W)[RANK][LEN] = mf::H0_W;
+struct Pair<0> {
+  static constexpr const float (&U)[PRANK][PLEN] = mf::PH_U;
+  static constexpr const float (&A)[PRANK][PLEN] = mf::PH_A;
+  static constexpr const float (&B)[PRANK][PLEN] = mf::PH_B;
 };
 template <>
-struct Filt<1> {
-  static constexpr int LEN = mf::H1_LEN, RANK = mf::H1_RANK;
-  static constexpr const float (&V)[RANK][LEN] = mf::H1_V;
-  static constexpr const float (&W)[RANK][LEN] = mf::H1_W;
-};
-template <>
-struct Filt<2> {
-  static constexpr int LEN = mf::G0_LEN, RANK = mf::G0_RANK;
-  static constexpr const float (&V)[RANK][LEN] = mf::G0_V;
-  static constexpr const float (&W)[RANK][LEN] = mf::G0_W;
-};
-template <>
-struct Filt<3> {
-  static constexpr int LEN = mf::G1_LEN, RANK = mf::G1_RANK;
-  static constexpr const float (&V)[RANK][LEN] = mf::G1_V;
-  static constexpr const float (&W)[RANK][LEN] = mf::G1_W;
+struct Pair<1> {
+  static constexpr const float (&U)[PRANK][PLEN] = mf::PG_U;
+  static constexpr const float (&A)[PRANK][PLEN] = mf::PG_A;
+  static constexpr const float (&B)[PRANK][PLEN] = mf::PG_B;
 };
 
-// Vertical pass of one filter component over a strip of VS rows.
-template <int LEN, int VS>
-__device__ __forceinline__ void vpass(const float (&col)[VS + 2 * HALO], const float (&v)[LEN], float (&t)[VS]) {
-  constexpr int off = HALO - (LEN - 1) / 2;
+// t[i] += sum_a v[a] * col[i + a], i < VS (taps that are zero skipped).
+template <int VS>
+__device__ __forceinline__ void vacc(const float (&col)[VS + 2 * HALO], const float (&v)[PLEN], float (&t)[VS]) {
 #pragma unroll
   for (int i = 0; i < VS; ++i) {
-    float s = 0.f;
 #pragma unroll
-    for (int a = 0; a < LEN; ++a) {
-      if (zero_tap<LEN>(a)) continue;
-      s = fmaf(v[a], col[i + a + off], s);
+    for (int a = 0; a < PLEN; ++a) {
+      if (v[a] == 0.0f) continue;
+      t[i] = fmaf(v[a], col[i + a], t[i]);
     }
-    t[i] = s;
   }
 }
 
-// Horizontal pass: acc[j] += sum_b w[b] * row[j + b + off]
-template <int LEN, int HS>
-__device__ __forceinline__ void hpass(const float* __restrict__ row, const float (&w)[LEN], float (&acc)[HS]) {
-  constexpr int off = HALO - (LEN - 1) / 2;
-  float r[HS + LEN - 1];
-#pragma unroll
-  for (int i = 0; i < HS + LEN - 1; ++i) r[i] = row[i + off];
+// acc[j] += sum_b w[b] * row[j + b], j < HS.
+template <int HS>
+__device__ __forceinline__ void hacc(const float (&row)[HS + 2 * HALO], const float (&w)[PLEN], float (&acc)[HS]) {
 #pragma unroll
   for (int j = 0; j < HS; ++j) {
 #pragma unroll
-    for (int b = 0; b < LEN; ++b) {
-      if (zero_tap<LEN>(b)) continue;
-      acc[j] = fmaf(w[b], r[j + b], acc[j]);
+    for (int b = 0; b < PLEN; ++b) {
+      if (w[b] == 0.0f) continue;
+      acc[j] = fmaf(w[b], row[j + b], acc[j]);
     }
   }
 }
@@ -150,46 +136,24 @@ __device__ __forceinline__ void tile_store(float* __restrict__ tile, const float
   for (int i = 0; i < TC::NIT; ++i) tile[threadIdx.x + i * T] = (th >= 0.f ? hard_keep(v[i], th) : v[i]) * scale;
 }
 
-// Vertical passes of filter F into tv[k0 .. k0+RANK) for all tile columns.
-template <int F, int TH, int W, int WP, int VS, int T, int NTV>
-__device__ __forceinline__ void vpasses(const float* __restrict__ tile, float (*tv)[TH * WP], int k0, int it0,
-                                        int nit) {
-  using Fi = Filt<F>;
-  for (int it = threadIdx.x + it0; it < it0 + nit; it += T) {
-    const int rel = it - it0;
-    const int cc = rel % W, st = rel / W;
-    float col[VS + 2 * HALO];
-#pragma unroll
-    for (int i = 0; i < VS + 2 * HALO; ++i) col[i] = tile[(st * VS + i) * W + cc];
-    float t[VS];
-#pragma unroll
-    for (int k = 0; k < Fi::RANK; ++k) {
-      vpass<Fi::LEN, VS>(col, Fi::V[k], t);
-#pragma unroll
-      for (int i = 0; i < VS; ++i) tv[k0 + k][(st * VS + i) * WP + cc] = t[i];
-    }
-  }
-}
-
 // Output grid of a level: the image (forward) or the extended grid of the
 // adjoint (EXT), E = L + 2*HALO*m per side, in sub-lattice rows of `m`.
 __device__ __forceinline__ int out_side(int L, int m, bool ext) { return ext ? L + 2 * HALO * m : L; }
 
 // ---------------------------------------------------------------------------
-// One input, two outputs: ya = FA (x) x, yb = FB (x) x.
-//   forward decomposition level: FA = h0, FB = h1
-//   adjoint of a reconstruction level (ADJ): FA = g0, FB = g1 on the
+// One input, two outputs with pair P: ya = F1 (x) x, yb = F2 (x) x.
+//   forward decomposition level: P = 0 (h0, h1)
+//   adjoint of a reconstruction level (ADJ): P = 1 (g0, g1) on the
 //   zero-padded gradient, outputs on the extended grid.
 // grid: (ceil(nout/TW), ceil(nout/TH), m*m*nimg), nout = out_side/m.
-template <int FA, int FB, bool ADJ, int TH, int TW, int VS, int HS, int T>
+template <int P, bool ADJ, int TH, int TW, int VS, int HS, int T>
 __global__ void __launch_bounds__(T) k_pyr_one2two(const float* __restrict__ x, float* __restrict__ ya,
                                                     float* __restrict__ yb, ImgStrides st3, int L, int m) {
-  using A = Filt<FA>;
-  using B = Filt<FB>;
+  using F = Pair<P>;
   constexpr int W = TW + 2 * HALO;
   constexpr int WP = W | 1;  // odd row stride for the intermediates
   __shared__ float tile[TileCfg<TH, TW, T>::PADDED];
-  __shared__ float tv[A::RANK + B::RANK][TH * WP];
+  __shared__ float tv[PRANK][TH * WP];
   const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
   const int pr = ph / m, pc = ph - pr * m;
   x += (size_t)img * st3.a;
@@ -204,18 +168,39 @@ __global__ void __launch_bounds__(T) k_pyr_one2two(const float* __restrict__ x, 
   __builtin_amdgcn_sched_barrier(0);  // the tile's loads in flight together
   tile_store<TH, TW, T>(tile, v, -1.0f, 1.0f);
   __syncthreads();
-  vpasses<FA, TH, W, WP, VS, T, 0>(tile, tv, 0, 0, W * (TH / VS));
-  vpasses<FB, TH, W, WP, VS, T, 0>(tile, tv, A::RANK, 0, W * (TH / VS));
+  // shared vertical passes: tv[i] = u_i (x) tile, every tile column
+  for (int it = threadIdx.x; it < W * (TH / VS); it += T) {
+    const int cc = it % W, st = it / W;
+    float col[VS + 2 * HALO];
+#pragma unroll
+    for (int i = 0; i < VS + 2 * HALO; ++i) col[i] = tile[(st * VS + i) * W + cc];
+#pragma unroll
+    for (int k = 0; k < PRANK; ++k) {
+      float t[VS];
+#pragma unroll
+      for (int i = 0; i < VS; ++i) t[i] = 0.f;
+      vacc<VS>(col, F::U[k], t);
+#pragma unroll
+      for (int i = 0; i < VS; ++i) tv[k][(st * VS + i) * WP + cc] = t[i];
+    }
+  }
   __syncthreads();
   for (int it = threadIdx.x; it < TH * (TW / HS); it += T) {
     const int r = it % TH, st = it / TH;
     float a0[HS], a1[HS];
 #pragma unroll
-    for (int j = 0; j < HS; ++j) a0[j] = a1[j] = 0.f;
+    for (int j = 0; j < HS; ++j) {
+      a0[j] = 0.f;
+      a1[j] = tile[(r + HALO) * W + st * HS + j + HALO];  // identity term of F2
+    }
 #pragma unroll
-    for (int k = 0; k < A::RANK; ++k) hpass<A::LEN, HS>(&tv[k][r * WP + st * HS], A::W[k], a0);
+    for (int k = 0; k < PRANK; ++k) {
+      float row[HS + 2 * HALO];
 #pragma unroll
-    for (int k = 0; k < B::RANK; ++k) hpass<B::LEN, HS>(&tv[A::RANK + k][r * WP + st * HS], B::W[k], a1);
+      for (int i = 0; i < HS + 2 * HALO; ++i) row[i] = tv[k][r * WP + st * HS + i];
+      hacc<HS>(row, F::A[k], a0);
+      hacc<HS>(row, F::B[k], a1);
+    }
     const int R = R0 + r;
     if (R < nout) {
       const size_t rowoff = (size_t)(pr + m * R) * E;
@@ -233,20 +218,20 @@ __global__ void __launch_bounds__(T) k_pyr_one2two(const float* __restrict__ x, 
 }
 
 // ---------------------------------------------------------------------------
-// Two inputs, one output: out = FA (x) (sa * keep(xa, th)) + FB (x) (sb * xb).
-//   forward reconstruction level: FA = g0, FB = g1
-//   adjoint of a decomposition level (ADJ): FA = h0, FB = h1 on the
+// Two inputs, one output with pair P:
+//   out = F1 (x) (sa * keep(xa, th)) + F2 (x) (sb * xb).
+//   forward reconstruction level: P = 1 (g0, g1)
+//   adjoint of a decomposition level (ADJ): P = 0 (h0, h1) on the
 //   zero-padded gradients, output on the extended grid.
-template <int FA, int FB, bool ADJ, int TH, int TW, int VS, int HS, int T>
+template <int P, bool ADJ, int TH, int TW, int VS, int HS, int T>
 __global__ void __launch_bounds__(T) k_pyr_two2one(const float* __restrict__ xa, const float* __restrict__ xb,
                                                     float* __restrict__ out, ImgStrides st3, int L, int m,
                                                     LowThresh lt, float sa, float sb) {
-  using A = Filt<FA>;
-  using B = Filt<FB>;
+  using F = Pair<P>;
   constexpr int W = TW + 2 * HALO;
   constexpr int WP = W | 1;
   __shared__ float tile[2][TileCfg<TH, TW, T>::PADDED];
-  __shared__ float tv[A::RANK + B::RANK][TH * WP];
+  __shared__ float tv[PRANK][TH * WP];
   const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
   const int pr = ph / m, pc = ph - pr * m;
   xa += (size_t)img * st3.a;
@@ -264,30 +249,25 @@ __global__ void __launch_bounds__(T) k_pyr_two2one(const float* __restrict__ xa,
   tile_store<TH, TW, T>(tile[0], va, th, sa);
   tile_store<TH, TW, T>(tile[1], vb, -1.0f, sb);
   __syncthreads();
-  constexpr int NIT = W * (TH / VS);
-  // the two vertical passes run on disjoint item ranges of one loop each
-  for (int it = threadIdx.x; it < 2 * NIT; it += T) {
-    const int which = it / NIT;
-    const int rel = it - which * NIT;
-    const int cc = rel % W, st = rel / W;
-    float col[VS + 2 * HALO];
+  // vertical passes of both inputs into the three shared intermediates:
+  // tv[i] = a_i (x) tile0 + b_i (x) tile1
+  for (int it = threadIdx.x; it < W * (TH / VS); it += T) {
+    const int cc = it % W, st = it / W;
+    float c0[VS + 2 * HALO], c1[VS + 2 * HALO];
 #pragma unroll
-    for (int i = 0; i < VS + 2 * HALO; ++i) col[i] = tile[which][(st * VS + i) * W + cc];
-    float t[VS];
-    if (which == 0) {
+    for (int i = 0; i < VS + 2 * HALO; ++i) {
+      c0[i] = tile[0][(st * VS + i) * W + cc];
+      c1[i] = tile[1][(st * VS + i) * W + cc];
+    }
 #pragma unroll
-      for (int k = 0; k < A::RANK; ++k) {
-        vpass<A::LEN, VS>(col, A::V[k], t);
+    for (int k = 0; k < PRANK; ++k) {
+      float t[VS];
 #pragma unroll
-        for (int i = 0; i < VS; ++i) tv[k][(st * VS + i) * WP + cc] = t[i];
-      }
-    } else {
+      for (int i = 0; i < VS; ++i) t[i] = 0.f;
+      vacc<VS>(c0, F::A[k], t);
+      vacc<VS>(c1, F::B[k], t);
 #pragma unroll
-      for (int k = 0; k < B::RANK; ++k) {
-        vpass<B::LEN, VS>(col, B::V[k], t);
-#pragma unroll
-        for (int i = 0; i < VS; ++i) tv[A::RANK + k][(st * VS + i) * WP + cc] = t[i];
-      }
+      for (int i = 0; i < VS; ++i) tv[k][(st * VS + i) * WP + cc] = t[i];
     }
   }
   __syncthreads();
@@ -295,11 +275,14 @@ __global__ void __launch_bounds__(T) k_pyr_two2one(const float* __restrict__ xa,
     const int r = it % TH, st = it / TH;
     float a[HS];
 #pragma unroll
-    for (int j = 0; j < HS; ++j) a[j] = 0.f;
+    for (int j = 0; j < HS; ++j) a[j] = tile[1][(r + HALO) * W + st * HS + j + HALO];  // identity term of F2
 #pragma unroll
-    for (int k = 0; k < A::RANK; ++k) hpass<A::LEN, HS>(&tv[k][r * WP + st * HS], A::W[k], a);
+    for (int k = 0; k < PRANK; ++k) {
+      float row[HS + 2 * HALO];
 #pragma unroll
-    for (int k = 0; k < B::RANK; ++k) hpass<B::LEN, HS>(&tv[A::RANK + k][r * WP + st * HS], B::W[k], a);
+      for (int i = 0; i < HS + 2 * HALO; ++i) row[i] = tv[k][r * WP + st * HS + i];
+      hacc<HS>(row, F::U[k], a);
+    }
     const int R = R0 + r;
     if (R < nout) {
       const size_t rowoff = (size_t)(pr + m * R) * E;
@@ -354,24 +337,24 @@ struct PyrCfg<2> {
   static constexpr int TH = 32, TW = 64, VS = 8, HS = 8, T = 256;
 };
 
-template <int FA, int FB, bool ADJ, int V>
+template <int P, bool ADJ, int V>
 static hipError_t one2two_v(const float* x, float* ya, float* yb, const ImgStrides& st, int nimg, int L, int m,
                             hipStream_t s) {
   using C = PyrCfg<V>;
   const int nout = (ADJ ? L + 2 * HALO * m : L) / m;
   dim3 grid((nout + C::TW - 1) / C::TW, (nout + C::TH - 1) / C::TH, m * m * nimg);
-  hipLaunchKernelGGL((k_pyr_one2two<FA, FB, ADJ, C::TH, C::TW, C::VS, C::HS, C::T>), grid, dim3(C::T), 0, s, x, ya,
+  hipLaunchKernelGGL((k_pyr_one2two<P, ADJ, C::TH, C::TW, C::VS, C::HS, C::T>), grid, dim3(C::T), 0, s, x, ya,
                      yb, st, L, m);
   return hipGetLastError();
 }
 
-template <int FA, int FB, bool ADJ, int V>
+template <int P, bool ADJ, int V>
 static hipError_t two2one_v(const float* xa, const float* xb, float* out, const ImgStrides& st, int nimg, int L,
                             int m, const LowThresh& lt, float sa, float sb, hipStream_t s) {
   using C = PyrCfg<V>;
   const int nout = (ADJ ? L + 2 * HALO * m : L) / m;
   dim3 grid((nout + C::TW - 1) / C::TW, (nout + C::TH - 1) / C::TH, m * m * nimg);
-  hipLaunchKernelGGL((k_pyr_two2one<FA, FB, ADJ, C::TH, C::TW, C::VS, C::HS, C::T>), grid, dim3(C::T), 0, s, xa, xb,
+  hipLaunchKernelGGL((k_pyr_two2one<P, ADJ, C::TH, C::TW, C::VS, C::HS, C::T>), grid, dim3(C::T), 0, s, xa, xb,
                      out, st, L, m, lt, sa, sb);
   return hipGetLastError();
 }
@@ -382,9 +365,9 @@ hipError_t launch_atrous_dec(const float* x, float* ylow, float* yhigh, const Im
                              hipStream_t s, int variant) {
   if (!level_ok(L, m, nimg)) return hipErrorInvalidValue;
   switch (variant) {
-    case 0: return one2two_v<0, 1, false, 0>(x, ylow, yhigh, st, nimg, L, m, s);
-    case 1: return one2two_v<0, 1, false, 1>(x, ylow, yhigh, st, nimg, L, m, s);
-    case 2: return one2two_v<0, 1, false, 2>(x, ylow, yhigh, st, nimg, L, m, s);
+    case 0: return one2two_v<0, false, 0>(x, ylow, yhigh, st, nimg, L, m, s);
+    case 1: return one2two_v<0, false, 1>(x, ylow, yhigh, st, nimg, L, m, s);
+    case 2: return one2two_v<0, false, 2>(x, ylow, yhigh, st, nimg, L, m, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -393,9 +376,9 @@ hipError_t launch_atrous_rec(const float* xlow, const float* yhigh, float* out, 
                              int m, const LowThresh& lt, float sx, float sy, hipStream_t s, int variant) {
   if (!level_ok(L, m, nimg)) return hipErrorInvalidValue;
   switch (variant) {
-    case 0: return two2one_v<2, 3, false, 0>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
-    case 1: return two2one_v<2, 3, false, 1>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
-    case 2: return two2one_v<2, 3, false, 2>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
+    case 0: return two2one_v<1, false, 0>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
+    case 1: return two2one_v<1, false, 1>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
+    case 2: return two2one_v<1, false, 2>(xlow, yhigh, out, st, nimg, L, m, lt, sx, sy, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -407,7 +390,7 @@ hipError_t launch_atrous_dec_adjoint(const float* glow, const float* ghigh, floa
   if (!level_ok(L, m, nimg)) return hipErrorInvalidValue;
   // Z = C_h0^T glow + C_h1^T ghigh on the extended grid, then fold-add
   ImgStrides sz{st.a, st.b, zstride};
-  hipError_t e = two2one_v<0, 1, true, 0>(glow, ghigh, zext, sz, nimg, L, m, LowThresh{0, 0.f, nullptr, 0}, 1.0f,
+  hipError_t e = two2one_v<0, true, 0>(glow, ghigh, zext, sz, nimg, L, m, LowThresh{0, 0.f, nullptr, 0}, 1.0f,
                                            1.0f, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_fold_add, dim3((L + 255) / 256, L, nimg), dim3(256), 0, s, zext, gx, L, HALO * m, zstride,
@@ -423,7 +406,7 @@ hipError_t launch_atrous_rec_adjoint(const float* gout, float* glow, float* ghig
   float* za = zext;
   float* zb = zext + E * E;
   ImgStrides sz{st.a, zstride, zstride};
-  hipError_t e = one2two_v<2, 3, true, 0>(gout, za, zb, sz, nimg, L, m, s);
+  hipError_t e = one2two_v<1, true, 0>(gout, za, zb, sz, nimg, L, m, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_fold_add, dim3((L + 255) / 256, L, nimg), dim3(256), 0, s, za, glow, L, HALO * m, zstride,
                      st.b);

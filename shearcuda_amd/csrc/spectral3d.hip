@@ -145,26 +145,36 @@ __global__ void __launch_bounds__(256) k3_rows_cmac(Rows3Cmac io, const float2* 
 #pragma unroll
       for (int k = 0; k < R2; ++k) buf[la * C::LD + n1 + R1 * k] = v[k];
     }
+    // band spectrum loads in flight across the barrier and the second pass
+    float2 f[R1];
+    const float2* F = io.F + (size_t)g * io.f_gs + lineB;
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) f[k1] = vB ? F[k2 + R2 * k1] : make_float2(0.f, 0.f);
     __syncthreads();
     if (vB) {
       float2 u[R1];
 #pragma unroll
       for (int a = 0; a < R1; ++a) u[a] = buf[lb * C::LD + a + R1 * k2];
       dft<R1, -1>(u);
-      const float2* F = io.F + (size_t)g * io.f_gs + lineB;
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) acc[k1] = cadd(acc[k1], cmulc(u[k1], F[k2 + R2 * k1]));
+      for (int k1 = 0; k1 < R1; ++k1) acc[k1] = cadd(acc[k1], cmulc(u[k1], f[k1]));
     }
     __syncthreads();
   }
   if (vB) {
     float2* dst = io.out + (size_t)blockIdx.y * io.out_bs + lineB;
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) {
-      const int k = k2 + R2 * k1;
-      const float2 a = cscale(acc[k1], io.scale);
-      dst[k] = io.accumulate ? cadd(dst[k], a) : a;
+    for (int k1 = 0; k1 < R1; ++k1) acc[k1] = cscale(acc[k1], io.scale);
+    if (io.accumulate) {  // every old value loaded before the first store
+      float2 old[R1];
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) old[k1] = dst[k2 + R2 * k1];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) acc[k1] = cadd(old[k1], acc[k1]);
     }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) dst[k2 + R2 * k1] = acc[k1];
   }
 }
 
@@ -373,24 +383,34 @@ __global__ void __launch_bounds__(CT * Split<N>::R2) k3_pencil_inv(Pencil3Inv io
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) acc[k1] = make_float2(0.f, 0.f);
   for (int g = 0; g < io.G; ++g) {
+    const float t = io.mode == 1 ? io.th[(size_t)b * io.th_bs + io.th0 + g] : 0.0f;
     if (n1 < R1) {
       const float2* src = io.in + (size_t)b * io.in_bs + (size_t)g * io.in_gs + 2 * (size_t)e;
+      // data and twiddle loads all in flight before the arithmetic
+      PassTw<R2> pt;
       float2 v[R2];
+      float4 ab[R2];
 #pragma unroll
       for (int n2 = 0; n2 < R2; ++n2) {
         const int m = n1 + R1 * n2;
         const int km = m <= N / 2 ? m : N - m;
-        const float4 ab = *reinterpret_cast<const float4*>(src + (size_t)km * E);
+        ab[n2] = *reinterpret_cast<const float4*>(src + (size_t)km * E);
+      }
+      passA_tw<R2, +1, 1>(pt, n1, tw);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        const int m = n1 + R1 * n2;
         // Z[m] = A[m] + i B[m]; Z[N-m] = conj(A[m]) + i conj(B[m]); the
         // imaginary parts at m = 0 and N/2 are dropped as in a real inverse.
         if (m == 0 || m == N / 2)
-          v[n2] = make_float2(ab.x, ab.z);
+          v[n2] = make_float2(ab[n2].x, ab[n2].z);
         else if (m < N / 2)
-          v[n2] = make_float2(ab.x - ab.w, ab.y + ab.z);
+          v[n2] = make_float2(ab[n2].x - ab[n2].w, ab[n2].y + ab[n2].z);
         else
-          v[n2] = make_float2(ab.x + ab.w, ab.z - ab.y);
+          v[n2] = make_float2(ab[n2].x + ab[n2].w, ab[n2].z - ab[n2].y);
       }
-      passA<R1, R2, +1, 1>(v, n1, tw);
+      passA_apply<R2, +1>(v, pt);
 #pragma unroll
       for (int k = 0; k < R2; ++k) buf[(n1 + R1 * k) * CT + c] = v[k];
     }
@@ -407,7 +427,6 @@ __global__ void __launch_bounds__(CT * Split<N>::R2) k3_pencil_inv(Pencil3Inv io
         if (in_window(j, io.n_out)) *reinterpret_cast<float2*>(dst + (size_t)j * E) = u[k1];
       }
     } else if (io.mode == 1) {
-      const float t = io.th[(size_t)b * io.th_bs + io.th0 + g];
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1)
         acc[k1] = cadd(acc[k1], make_float2(hard_keep(u[k1].x, t), hard_keep(u[k1].y, t)));
@@ -419,13 +438,22 @@ __global__ void __launch_bounds__(CT * Split<N>::R2) k3_pencil_inv(Pencil3Inv io
   }
   if (io.mode != 0) {
     float* dst = io.out + (size_t)b * io.out_bs + 2 * (size_t)e;
+    if (!io.first) {  // add to out: every old value loaded before the first store
+      float2 old[R1];
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const int j = k2 + R2 * k1 - io.o_out;
+        old[k1] = in_window(j, io.n_out) ? *reinterpret_cast<const float2*>(dst + (size_t)j * E)
+                                         : make_float2(0.f, 0.f);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) acc[k1] = cadd(old[k1], acc[k1]);
+    }
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       const int j = k2 + R2 * k1 - io.o_out;
-      if (in_window(j, io.n_out)) {
-        float2* p = reinterpret_cast<float2*>(dst + (size_t)j * E);
-        *p = io.first ? acc[k1] : cadd(*p, acc[k1]);
-      }
+      if (in_window(j, io.n_out)) *reinterpret_cast<float2*>(dst + (size_t)j * E) = acc[k1];
     }
   }
 }
