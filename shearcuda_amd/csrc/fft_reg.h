@@ -194,6 +194,17 @@ __device__ __forceinline__ void passA_tw(PassTw<RB>& t, int a, const float2* __r
   for (int q = 1; q < NHI; ++q) t.hi[q] = tw[a * q * LO * TWS];
 }
 
+// Opaque copy of loaded twiddles: products of its values cannot be hoisted
+// out of a loop (where they would stay live in registers across iterations).
+template <int RB>
+__device__ __forceinline__ PassTw<RB> pin(PassTw<RB> t) {
+#pragma unroll
+  for (int i = 1; i < PassTw<RB>::LO; ++i) asm volatile("" : "+v"(t.lo[i].x), "+v"(t.lo[i].y));
+#pragma unroll
+  for (int i = 1; i < PassTw<RB>::NHI; ++i) asm volatile("" : "+v"(t.hi[i].x), "+v"(t.hi[i].y));
+  return t;
+}
+
 template <int RB, int DIR>
 __device__ __forceinline__ void passA_apply(float2 (&v)[RB], PassTw<RB> t) {
   constexpr int LO = PassTw<RB>::LO, NHI = PassTw<RB>::NHI;

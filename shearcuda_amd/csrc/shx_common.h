@@ -35,4 +35,23 @@ __device__ __forceinline__ float hard_keep(float v, float th) { return fabsf(v) 
 
 __host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 
+// Raw buffer view of `bytes` bytes from a wave-uniform base address.  An
+// access takes a per-lane byte offset (one vector register) and a uniform
+// byte offset (a scalar register), so a run of accesses at different uniform
+// offsets shares one address register; accesses at or past `bytes` read zero
+// and drop writes.  These are vector memory instructions (the descriptor only
+// lives in scalar registers).
+struct BufView {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ BufView(const void* base, unsigned bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000)) {}
+  __device__ __forceinline__ float2 ld2(unsigned lane_off, unsigned uni_off) const {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, uni_off, 0));
+  }
+  __device__ __forceinline__ void st2(float2 v, unsigned lane_off, unsigned uni_off) const {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
+                                          lane_off, uni_off, 0);
+  }
+};
+
 }  // namespace shx

@@ -122,14 +122,9 @@ __device__ __forceinline__ void row_fwd_from_A(float2 (&v)[R2], bool validA, int
                                                const RowFwdTw<L, R1, R2>& t) {
   constexpr int N = L / 2;
   if (validA) {
-    // keep the products of the preloaded twiddles here, after the data:
+    // products of the preloaded twiddles formed here, after the data:
     // computed earlier they would make the first wait cover the twiddle loads
-    PassTw<R2> pt = t.pt;
-#pragma unroll
-    for (int i = 1; i < PassTw<R2>::LO; ++i) asm volatile("" : "+v"(pt.lo[i].x), "+v"(pt.lo[i].y));
-#pragma unroll
-    for (int i = 1; i < PassTw<R2>::NHI; ++i) asm volatile("" : "+v"(pt.hi[i].x), "+v"(pt.hi[i].y));
-    passA_apply<R2, -1>(v, pt);
+    passA_apply<R2, -1>(v, pin(t.pt));
 #pragma unroll
     for (int k = 0; k < R2; ++k) buf[lds_ix<PADSH>(r * N + n1 + R1 * k)] = v[k];
   }
@@ -350,6 +345,41 @@ struct ColCfg {
   static constexpr int LDS = lds_size<L * CT, PADSH>();
 };
 
+// Column tile loads and stores through a buffer view of one tile (tile0:
+// column c0 of row 0 of a plane with row stride L/2): element (row, c),
+// c < CT, at byte (row * L/2 + c) * 8.  The per-lane part of the offset stays
+// in one register for a whole run of accesses; the view ends with the tile.
+template <int L, int CT>
+__device__ __forceinline__ BufView col_view(const float2* tile0) {
+  return BufView(tile0, (unsigned)(((L - 1) * (L / 2) + CT) * sizeof(float2)));
+}
+template <int L>
+__device__ __forceinline__ unsigned col_off(int row, int c) {
+  return (unsigned)((row * (L / 2) + c) * sizeof(float2));
+}
+
+// Pass-A registers of thread (c, n1 < R1): v[n2] = src(n1 + R1 n2, c).
+template <int L, int CT_>
+__device__ __forceinline__ void col_load_A(float2 (&v)[Split<L>::R2], const float2* src) {
+  using C = ColCfg<L, CT_>;
+  constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT;
+  const BufView bv = col_view<L, CT>(src);
+  const unsigned lane = col_off<L>(threadIdx.x / CT, threadIdx.x % CT);
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) v[n2] = bv.ld2(lane, col_off<L>(R1 * n2, 0));
+}
+
+// Pass-B registers of thread (c, k2): x[k1] = src(k2 + R2 k1, c).
+template <int L, int CT_>
+__device__ __forceinline__ void col_load_B(float2 (&x)[Split<L>::R1], const float2* src) {
+  using C = ColCfg<L, CT_>;
+  constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT;
+  const BufView bv = col_view<L, CT>(src);
+  const unsigned lane = col_off<L>(threadIdx.x / CT, threadIdx.x % CT);
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) x[k1] = bv.ld2(lane, col_off<L>(R2 * k1, 0));
+}
+
 // Forward column transform: from global (row stride L/2) to pass-B layout
 // registers u[k1] = X[k2 + R2 k1] of thread (c, k2).  Pass A (global loads,
 // first DFT, LDS stores) and pass B (LDS loads, second DFT) are separate so a
@@ -363,8 +393,7 @@ __device__ __forceinline__ void col_fwd_A(const float2* __restrict__ src, float2
   const int c = tid % CT, n1 = tid / CT;
   if (n1 < R1) {
     float2 v[R2];
-#pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) v[n2] = src[(size_t)(n1 + R1 * n2) * NH + c];
+    col_load_A<L, CT_>(v, src);
     passA<R1, R2, -1, 1>(v, n1, tw);
 #pragma unroll
     for (int k = 0; k < R2; ++k) buf[lds_ix<C::PADSH>((n1 + R1 * k) * CT + c)] = v[k];
@@ -392,15 +421,23 @@ __device__ __forceinline__ void col_fwd_to_B(float2 (&u)[Split<L>::R1], const fl
 
 // Inverse column transform from pass-B layout registers (thread (c, k2)
 // holding Q[k2 + R2 k1]) to global (row stride L/2).  Starts with a barrier
-// so buf may still be in use by the caller.
+// so buf may still be in use by the caller.  t: the first pass's twiddles
+// (col_inv_tw), loaded by the caller ahead of time.
+template <int L>
+__device__ __forceinline__ PassTw<Split<L>::R1> col_inv_tw(int k2, const float2* __restrict__ tw) {
+  PassTw<Split<L>::R1> t;
+  passA_tw<Split<L>::R1, +1, 1>(t, k2, tw);
+  return t;
+}
+
 template <int L, int CT_>
 __device__ __forceinline__ void col_inv_from_B(float2 (&q)[Split<L>::R1], float2* __restrict__ dst, float2* buf,
-                                               const float2* __restrict__ tw) {
+                                               const PassTw<Split<L>::R1>& t) {
   using C = ColCfg<L, CT_>;
   constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
   const int tid = threadIdx.x;
   const int c = tid % CT, k2 = tid / CT;
-  passA<R2, R1, +1, 1>(q, k2, tw);
+  passA_apply<R1, +1>(q, t);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < R1; ++k) buf[lds_ix<C::PADSH>((k2 + R2 * k) * CT + c)] = q[k];
@@ -411,8 +448,10 @@ __device__ __forceinline__ void col_inv_from_B(float2 (&q)[Split<L>::R1], float2
 #pragma unroll
     for (int a = 0; a < R2; ++a) w[a] = buf[lds_ix<C::PADSH>((a + R2 * kb) * CT + c)];
     dft<R2, +1>(w);
+    const BufView bv = col_view<L, CT>(dst);
+    const unsigned lane = col_off<L>(kb, c);
 #pragma unroll
-    for (int k1 = 0; k1 < R2; ++k1) dst[(size_t)(kb + R1 * k1) * NH + c] = w[k1];
+    for (int k1 = 0; k1 < R2; ++k1) bv.st2(w[k1], lane, col_off<L>(R1 * k1, 0));
   }
 }
 
@@ -420,17 +459,13 @@ __device__ __forceinline__ void col_inv_from_B(float2 (&q)[Split<L>::R1], float2
 // batch of independent loads, ahead of the work that hides their latency.
 template <int L, int CT_>
 __device__ __forceinline__ void col_load_h(float2 (&h)[Split<L>::R1], const float2* __restrict__ H) {
-  using C = ColCfg<L, CT_>;
-  constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
-  const unsigned c = threadIdx.x % CT, k2 = threadIdx.x / CT;
-#pragma unroll
-  for (int k1 = 0; k1 < R1; ++k1) h[k1] = H[(k2 + R2 * k1) * NH + c];
+  col_load_B<L, CT_>(h, H);
 }
 
 // Filter pass-B registers by the band spectrum h (col_load_h; CONJ: its
 // conjugate).  In the block holding column 0 the packed (DC, Nyquist) column
 // needs X(-u): it is exchanged through LDS (block-uniform branch), and the
-// beta term is added on lane column 0 by a select.
+// beta term (row Hb) is added on lane column 0 by a select.
 template <int L, int CT_, bool CONJ, bool ACC>
 __device__ __forceinline__ void col_filter(float2 (&u)[Split<L>::R1], float2 (&out)[Split<L>::R1], int c0,
                                            const float2 (&h)[Split<L>::R1], const float2* __restrict__ Hb,
@@ -514,9 +549,10 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_fwd(float2* __restr
   float2* base = Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0;
   float2 u[R1];
   col_fwd_to_B<L, CT_>(u, base, buf, tw);
-  const int c = threadIdx.x % C::CT, k2 = threadIdx.x / C::CT;
+  const BufView bv = col_view<L, C::CT>(base);
+  const unsigned lane = col_off<L>(threadIdx.x / C::CT, threadIdx.x % C::CT);
 #pragma unroll
-  for (int k1 = 0; k1 < R1; ++k1) base[(size_t)(k2 + R2 * k1) * NH + c] = u[k1];
+  for (int k1 = 0; k1 < R1; ++k1) bv.st2(u[k1], lane, col_off<L>(R2 * k1, 0));
 }
 
 // Column pass of the decomposition: one block per (column tile, band, image).
@@ -536,19 +572,21 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_dec(const float2* _
   const int j = map_u8(map.scale_of, p);
   float2 u[C::R1], q[C::R1];
   const float2* src = Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0;
+  const int k2 = threadIdx.x / C::CT;
   float2 h[C::R1];
+  PassTw<C::R1> ivt;
   if constexpr (PRE) {
-    const unsigned c = threadIdx.x % C::CT, k2 = threadIdx.x / C::CT;
-#pragma unroll
-    for (int k1 = 0; k1 < C::R1; ++k1) u[k1] = src[(k2 + C::R2 * k1) * NH + c];
+    col_load_B<L, CT_>(u, src);
     col_load_h<L, CT_>(h, Hmain + (size_t)p * L * NH + c0);
+    ivt = col_inv_tw<L>(k2, tw);
     __builtin_amdgcn_sched_barrier(0);  // all loads in flight before the arithmetic
   } else {
     col_load_h<L, CT_>(h, Hmain + (size_t)p * L * NH + c0);
+    ivt = col_inv_tw<L>(k2, tw);
     col_fwd_to_B<L, CT_>(u, src, buf, tw);
   }
   col_filter<L, CT_, false, false>(u, q, c0, h, Hbeta + (size_t)p * L, buf);
-  col_inv_from_B<L, CT_>(q, Tout + (size_t)b * t_stride + (size_t)p * L * NH + c0, buf, tw);
+  col_inv_from_B<L, CT_>(q, Tout + (size_t)b * t_stride + (size_t)p * L * NH + c0, buf, ivt);
 }
 
 // Column pass of the reconstruction: one block per (column tile, direction
@@ -576,14 +614,15 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* _
   col_block_order(tile, g, b);
   const int c0 = tile * CT;
   const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
+  const int p0 = map_u16(map.group_band0, g), p1 = map_u16(map.group_band0, g + 1);
+  const float2* ub = U + (size_t)b * u_stride + c0;
   float2 acc[R1];
 #pragma unroll
   for (int k = 0; k < R1; ++k) acc[k] = make_float2(0.f, 0.f);
-  const int p0 = map_u16(map.group_band0, g), p1 = map_u16(map.group_band0, g + 1);
   for (int p = p0; p < p1; ++p) {
     float2 u[R1], h[R1];
     __syncthreads();  // previous direction's reads of buf are complete
-    col_fwd_A<L, CT_>(U + (size_t)b * u_stride + (size_t)p * L * NH + c0, buf, tw);
+    col_fwd_A<L, CT_>(ub + (size_t)p * L * NH, buf, tw);
     col_load_h<L, CT_>(h, Hmain + (size_t)p * L * NH + c0);  // in flight across pass B
     __syncthreads();
     col_fwd_B<L, CT_>(u, buf);
@@ -603,7 +642,7 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* _
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) acc[k1] = accl[lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c)];
   }
-  col_inv_from_B<L, CT_>(acc, Q + (size_t)b * q_stride + (size_t)g * L * NH + c0, buf, tw);
+  col_inv_from_B<L, CT_>(acc, Q + (size_t)b * q_stride + (size_t)g * L * NH + c0, buf, col_inv_tw<L>(k2, tw));
 }
 
 // ---------------------------------------------------------------------------
