@@ -137,12 +137,13 @@ int shx_batch_inverse_shear_adjoint(shx_batch* batch, const float* x_bar, float*
  * decomposition and reconstruction column passes on the plan's own workspace
  * (the dictionary's one-image plan when batch is NULL) and keeps the fastest.
  * chosen (may be NULL, else 4 ints) receives {pyramid variant, group,
- * decomposition column tile, reconstruction column tile}.  Allocates scratch
+ * decomposition column tile, reconstruction column tile}.  Each candidate is
+ * timed as a captured graph replay on a private stream.  Allocates scratch
  * and synchronises: call it outside graph capture.  No counterpart in the
  * reference. */
 int shx_autotune(shx_dict* dict, shx_batch* batch, int reps, int* chosen, void* stream);
 /* Sets a schedule explicitly (pyramid variant 0..2, group 2/4/8, column tiles
- * 8 or 16 (16 only for L <= 512), 0 = default). */
+ * 4 (L >= 128), 8, or 16 (L <= 512); 0 = default). */
 int shx_set_schedule(shx_dict* dict, shx_batch* batch, int pyr_variant, int group, int ct_dec, int ct_rec);
 
 /* ---- HIP graphs --------------------------------------------------------------

@@ -886,10 +886,11 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t caller, 
   w.pyr_variant = bestv;
   // reconstruction: directions per block x column tile; decomposition: tile
   const int gcand[3] = {2, 4, 8};
-  const int ccand[2] = {8, 16};
-  const int nct = (d->L / 2 >= 16 && d->L <= 512) ? 2 : 1;
-  double tg[3][2];
-  for (auto& r : tg) r[0] = r[1] = 1e30;
+  int ccand[3], nct = 0;
+  for (int ct : {8, 16, 4})
+    if (shx::col_tile_ok(d->L, ct)) ccand[nct++] = ct;
+  double tg[3][3];
+  for (auto& r : tg) r[0] = r[1] = r[2] = 1e30;
   for (int round = 0; round < 3; ++round)
     for (int i = 0; i < 3; ++i)
       for (int c = 0; c < nct; ++c) {
@@ -906,7 +907,7 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t caller, 
       if (tg[i][c] < tg[bi][bc]) bi = i, bc = c;
   set_group(d, w, gcand[bi]);
   w.ct_rec = ccand[bc];
-  double td[2] = {1e30, 1e30};
+  double td[3] = {1e30, 1e30, 1e30};
   for (int round = 0; round < 3; ++round)
     for (int c = 0; c < nct; ++c) {
       w.ct_dec = ccand[c];
@@ -915,7 +916,10 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t caller, 
       const double t = time_stages_graph(st, s, reps);
       if (t > 0 && t < td[c]) td[c] = t;
     }
-  w.ct_dec = (nct == 2 && td[1] < td[0]) ? 16 : 8;
+  int bd = 0;
+  for (int c = 1; c < nct; ++c)
+    if (td[c] < td[bd]) bd = c;
+  w.ct_dec = ccand[bd];
   hipStreamSynchronize(s);
   hipFree(img);
   hipFree(coeff);
@@ -1188,8 +1192,8 @@ int shx_set_schedule(shx_dict* d, shx_batch* b, int pyr_variant, int group, int 
   if (pyr_variant < 0 || pyr_variant >= shx::PYR_VARIANTS) return fail(SHX_ERR_ARG, "no pyramid variant %d", pyr_variant);
   if (group != 2 && group != 4 && group != 8) return fail(SHX_ERR_ARG, "direction group must be 2, 4 or 8");
   for (int ct : {ct_dec, ct_rec})
-    if (ct != 0 && ct != 8 && !(ct == 16 && d->L / 2 >= 16 && d->L <= 512))
-      return fail(SHX_ERR_ARG, "column tile %d not available for L = %d (0, 8 or 16)", ct, d->L);
+    if (ct != 0 && !shx::col_tile_ok(d->L, ct))
+      return fail(SHX_ERR_ARG, "column tile %d not available for L = %d (0, 4, 8 or 16 by size)", ct, d->L);
   Work& w = b ? b->w : d->w;
   w.pyr_variant = pyr_variant;
   set_group(d, w, group);

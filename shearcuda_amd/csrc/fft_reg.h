@@ -176,6 +176,64 @@ template <> struct Split<1024> { static constexpr int R1 = 32, R2 = 32; };
 // (tw[m*TWS] = exp(-2 pi i m / N)).
 // For RB > 8 only w^r (r < 8) and w^(8q) are loaded and w^(8q + r) is one
 // product of the two, which keeps the twiddle registers few.
+// ---- lane-pair split of a length-2R DFT ----------------------------------------
+// Partner lane (lane ^ 1) value through a DPP quad permutation [1, 0, 3, 2].
+__device__ __forceinline__ float swap_pair(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float2 swap_pair(float2 v) { return make_float2(swap_pair(v.x), swap_pair(v.y)); }
+
+// Length-2R DFT over a lane pair (h = lane & 1, both lanes active): the
+// thread holds z[j] = x[2j + h] (j < R) and gets X[k + R h] (k < R):
+//   X[k] = E0[k] + W^k E1[k],  X[k + R] = E0[k] - W^k E1[k],  W = w_2R.
+template <int R, int DIR>
+__device__ __forceinline__ void dft_pair(float2 (&z)[R], int h) {
+  dft<R, DIR>(z);
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const float2 o = swap_pair(z[k]);
+    const float2 e0 = h ? o : z[k], e1 = h ? z[k] : o;
+    const float2 t = ctw_mul<2 * R, DIR>(e1, k);
+    z[k] = h ? csub(e0, t) : cadd(e0, t);
+  }
+}
+
+// Pass twiddles of a paired pass A (index a, outputs k2 = k + R h): w^(a r)
+// (r < LO) and w^(a (R h + LO q)) (q < NQ), from the table tw (stride TWS).
+template <int R>
+struct PairTw {
+  static constexpr int LO = R < 8 ? R : 8;
+  static constexpr int NQ = R / LO;
+  float2 lo[LO];
+  float2 hi[NQ];
+};
+
+template <int R, int TWS>
+__device__ __forceinline__ void pair_tw(PairTw<R>& t, int a, int h, const float2* __restrict__ tw) {
+  constexpr int LO = PairTw<R>::LO, NQ = PairTw<R>::NQ;
+#pragma unroll
+  for (int r = 1; r < LO; ++r) t.lo[r] = tw[a * r * TWS];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) t.hi[q] = tw[a * (R * h + LO * q) * TWS];
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void pair_tw_apply(float2 (&v)[R], PairTw<R> t) {
+  constexpr int LO = PairTw<R>::LO, NQ = PairTw<R>::NQ;
+  if constexpr (DIR > 0) {
+#pragma unroll
+    for (int r = 1; r < LO; ++r) t.lo[r].y = -t.lo[r].y;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) t.hi[q].y = -t.hi[q].y;
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    v[q * LO] = cmul(v[q * LO], t.hi[q]);
+#pragma unroll
+    for (int r = 1; r < LO; ++r) v[q * LO + r] = cmul(v[q * LO + r], cmul(t.hi[q], t.lo[r]));
+  }
+}
+
 // Pass twiddles of one thread: w^r (r < LO) and w^(LO q) (q < NHI).
 template <int RB>
 struct PassTw {

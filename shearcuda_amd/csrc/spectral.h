@@ -66,8 +66,9 @@ hipError_t launch_rows_c2r(int L, const PlaneSet& in, const PlaneSet& out, int n
 // Denoise fusion: c = C2R(T) written to `band`, then U = R2C(keep(c, th)).
 hipError_t launch_rows_c2r_r2c(int L, const PlaneSet& T, const PlaneSet& band, const PlaneSet& U, int nplanes,
                                int nimg, const ThreshSpec& th, const float2* tw, hipStream_t s);
-// ct: column tile width of the block (8 or 16; 0 = the default for L).
-// pre: Y1 is already column-transformed (launch_cols_fwd).
+// ct: column tile width of the block (4, 8 or 16 where col_tile_ok; 0 = the
+// default for L).  pre: Y1 is already column-transformed (launch_cols_fwd).
+bool col_tile_ok(int L, int ct);
 hipError_t launch_cols_dec(int L, const float2* Y1, long long y1_stride, float2* Tout, long long t_stride,
                            const float2* Hm, const float2* Hb, const BandMap& map, int nimg, const float2* tw,
                            hipStream_t s, int ct = 0, bool pre = false);

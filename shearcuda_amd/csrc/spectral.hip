@@ -329,10 +329,14 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
 // stride L/2 complex); each column is a transform of N = L points split
 // R1 x R2.  T = CT * R2 threads; element (u, c) of the LDS tile is linear
 // element u*CT + c.
-// Column tile widths: 8, and 16 where the tile fits the LDS (L <= 512).
+// Column tile widths: 8; 16 where the tile fits the LDS (L <= 512); 4 where
+// a block still fills a wave (4 R2 >= 64).  Narrow tiles use a quarter of
+// each 128-byte row line; the XCD-aware block order keeps the neighbouring
+// tiles that share a line in one L2.
 template <int L>
 struct ColTile {
   static constexpr bool wide = L / 2 >= 16 && L <= 512;
+  static constexpr bool narrow = 4 * Split<L>::R2 >= 64;
   static constexpr int dflt = (L >= 1024) ? 8 : ((L / 2) < 16 ? (L / 2) : 16);
 };
 
@@ -341,7 +345,7 @@ struct ColCfg {
   static constexpr int R1 = Split<L>::R1, R2 = Split<L>::R2;
   static constexpr int CT = CT_;
   static constexpr int T = CT * R2;
-  static constexpr int PADSH = 7;
+  static constexpr int PADSH = CT >= 8 ? 7 : 6;  // one pad element per LDS row group
   static constexpr int LDS = lds_size<L * CT, PADSH>();
 };
 
@@ -384,12 +388,39 @@ __device__ __forceinline__ void col_load_B(float2 (&x)[Split<L>::R1], const floa
 // registers u[k1] = X[k2 + R2 k1] of thread (c, k2).  Pass A (global loads,
 // first DFT, LDS stores) and pass B (LDS loads, second DFT) are separate so a
 // caller can issue other loads between them; a barrier must separate them.
+//
+// When R2 = 2 R1 the length-R2 DFTs of pass A (forward) and of the inverse's
+// second pass run on lane pairs (dft_pair): every thread of the block takes
+// part, each holding R1 values instead of R2 for half of the threads.
+template <int L>
+struct ColPair {
+  static constexpr bool on = Split<L>::R2 == 2 * Split<L>::R1;
+};
+
 template <int L, int CT_>
 __device__ __forceinline__ void col_fwd_A(const float2* __restrict__ src, float2* buf,
                                           const float2* __restrict__ tw) {
   using C = ColCfg<L, CT_>;
   constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT, NH = L / 2;
   const int tid = threadIdx.x;
+  if constexpr (ColPair<L>::on) {
+    // thread (c, n1, h): z[j] = x[n1 + R1 (2j + h)], outputs k2 = k + R1 h
+    const int h = tid & 1, rest = tid >> 1;
+    const int c = rest % CT, n1 = rest / CT;
+    float2 v[R1];
+    const BufView bv = col_view<L, CT>(src);
+    const unsigned lane = col_off<L>(n1 + R1 * h, c);
+#pragma unroll
+    for (int j = 0; j < R1; ++j) v[j] = bv.ld2(lane, col_off<L>(2 * R1 * j, 0));
+    PairTw<R1> t;
+    pair_tw<R1, 1>(t, n1, h, tw);
+    __builtin_amdgcn_sched_barrier(0);  // data and twiddle loads in flight together
+    dft_pair<R1, -1>(v, h);
+    pair_tw_apply<R1, -1>(v, t);
+#pragma unroll
+    for (int k = 0; k < R1; ++k) buf[lds_ix<C::PADSH>((n1 + R1 * (k + R1 * h)) * CT + c)] = v[k];
+    return;
+  }
   const int c = tid % CT, n1 = tid / CT;
   if (n1 < R1) {
     float2 v[R2];
@@ -442,6 +473,20 @@ __device__ __forceinline__ void col_inv_from_B(float2 (&q)[Split<L>::R1], float2
 #pragma unroll
   for (int k = 0; k < R1; ++k) buf[lds_ix<C::PADSH>((k2 + R2 * k) * CT + c)] = q[k];
   __syncthreads();
+  if constexpr (ColPair<L>::on) {
+    // thread (c, kb, h): z[j] = Y[2j + h] of output column kb, outputs k + R1 h
+    const int h = tid & 1, rest = tid >> 1;
+    const int cp = rest % CT, kb = rest / CT;
+    float2 w[R1];
+#pragma unroll
+    for (int j = 0; j < R1; ++j) w[j] = buf[lds_ix<C::PADSH>((2 * j + h + R2 * kb) * CT + cp)];
+    dft_pair<R1, +1>(w, h);
+    const BufView bv = col_view<L, CT>(dst);
+    const unsigned lane = col_off<L>(kb + R1 * R1 * h, cp);
+#pragma unroll
+    for (int k = 0; k < R1; ++k) bv.st2(w[k], lane, col_off<L>(R1 * k, 0));
+    return;
+  }
   const int kb = tid / CT;
   if (kb < R1) {
     float2 w[R2];
@@ -702,6 +747,8 @@ struct Launch {
                              const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct,
                              bool pre) {
     if (ct == 0) ct = ColTile<L>::dflt;
+    if (ct == 4 && ColTile<L>::narrow)
+      return cols_dec_ct<(ColTile<L>::narrow ? 4 : 8)>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s, pre);
     if (ct == 8) return cols_dec_ct<8>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s, pre);
     if (ct == 16 && ColTile<L>::wide)
       return cols_dec_ct<(ColTile<L>::wide ? 16 : 8)>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s, pre);
@@ -709,6 +756,7 @@ struct Launch {
   }
   static hipError_t cols_fwd(float2* Y1, long long ys, int nplanes, int nimg, const float2* tw, hipStream_t s, int ct) {
     if (ct == 0) ct = ColTile<L>::dflt;
+    if (ct == 4 && ColTile<L>::narrow) return cols_fwd_ct<(ColTile<L>::narrow ? 4 : 8)>(Y1, ys, nplanes, nimg, tw, s);
     if (ct == 8) return cols_fwd_ct<8>(Y1, ys, nplanes, nimg, tw, s);
     if (ct == 16 && ColTile<L>::wide) return cols_fwd_ct<(ColTile<L>::wide ? 16 : 8)>(Y1, ys, nplanes, nimg, tw, s);
     return hipErrorInvalidValue;
@@ -716,6 +764,7 @@ struct Launch {
   static hipError_t cols_rec(const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
                              const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct) {
     if (ct == 0) ct = ColTile<L>::dflt;
+    if (ct == 4 && ColTile<L>::narrow) return cols_rec_ct<(ColTile<L>::narrow ? 4 : 8)>(U, us, Q, qs, Hm, Hb, map, nimg, tw, s);
     if (ct == 8) return cols_rec_ct<8>(U, us, Q, qs, Hm, Hb, map, nimg, tw, s);
     if (ct == 16 && ColTile<L>::wide) return cols_rec_ct<(ColTile<L>::wide ? 16 : 8)>(U, us, Q, qs, Hm, Hb, map, nimg, tw, s);
     return hipErrorInvalidValue;
@@ -758,6 +807,16 @@ hipError_t launch_cols_dec(int L, const float2* Y1, long long ys, float2* To, lo
 hipError_t launch_cols_fwd(int L, float2* Y1, long long ys, int nplanes, int nimg, const float2* tw, hipStream_t s,
                            int ct) {
   SHX_BY_L(L, LT::cols_fwd(Y1, ys, nplanes, nimg, tw, s, ct))
+}
+
+bool col_tile_ok(int L, int ct) {
+  switch (L) {
+#define SHX_CT_OK(LL) \
+  case LL: return ct == 8 || (ct == 16 && ColTile<LL>::wide) || (ct == 4 && ColTile<LL>::narrow);
+    SHX_CT_OK(32) SHX_CT_OK(64) SHX_CT_OK(128) SHX_CT_OK(256) SHX_CT_OK(512) SHX_CT_OK(1024)
+#undef SHX_CT_OK
+    default: return false;
+  }
 }
 
 hipError_t launch_cols_rec(int L, const float2* U, long long us, float2* Q, long long qs, const float2* Hm,

@@ -1,0 +1,104 @@
+"""Every schedule of a 2D plan computes the same transform.
+
+The autotuner picks, per plan, the pyramid tile shape (3 variants), the
+directions summed per reconstruction block (2, 4, 8) and the column tile width
+of the decomposition and reconstruction column passes (4, 8, 16 where the size
+allows).  The default schedule is pinned to the FP64 oracle in
+test_gpu_nsst.py; here every other schedule is compared with it: the
+coefficients must agree to FP32 rounding (they do not depend on the direction
+grouping) and so must the denoised image (the grouping only reorders the sum
+over directions).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _gpu(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def _tiles(api, shear):
+    from shearcuda_amd._lib import ShxError
+    ok = []
+    for ct in (4, 8, 16):
+        try:
+            api.set_schedule(shear, 0, 4, col_tile_dec=ct, col_tile_rec=ct)
+            ok.append(ct)
+        except ShxError:
+            pass
+    return ok
+
+
+def _bands(cells, L):
+    """Directional coefficient cells (lowpass excluded) as one (n, L, L) array."""
+    return np.concatenate([t.cpu().numpy().reshape(-1, L, L) for t in cells[1:]])
+
+
+@pytest.mark.parametrize("L", [128, 256, 512])
+def test_every_schedule_matches_default(gpu, L):
+    from shearcuda_amd import api
+    from shearcuda_amd.synth import noisy_ellipses, white_noise
+    print("case schedules L=%d" % L, flush=True)
+    shear = api.make_dictionary(L)
+    try:
+        sc = api.nsst_scalars_cuda(shear, _gpu(white_noise(L)))
+        _, x = noisy_ellipses(L, sigma=20.0, seed=11)
+        xg = _gpu(x)
+        tiles = _tiles(api, shear)
+        assert 8 in tiles and (L < 128 or 4 in tiles)
+        api.set_schedule(shear, 0, 4, col_tile_dec=0, col_tile_rec=0)
+        ref, cref = api.denoise_cuda(xg, shear, 20.0, sc)
+        ref = ref.cpu().numpy()
+        cref = _bands(cref, L)
+        worst_x = worst_c = 0.0
+        for pv in range(3):
+            for g in (2, 4, 8):
+                for ctd in tiles:
+                    for ctr in tiles:
+                        api.set_schedule(shear, pv, g, col_tile_dec=ctd, col_tile_rec=ctr)
+                        out, c = api.denoise_cuda(xg, shear, 20.0, sc)
+                        ex, ec = _rel(out.cpu().numpy(), ref), _rel(_bands(c, L), cref)
+                        worst_x, worst_c = max(worst_x, ex), max(worst_c, ec)
+                        assert ex <= TOL and ec <= TOL, (pv, g, ctd, ctr, ex, ec)
+        print("L=%d tiles %s: worst rel diff image %.2e coefficients %.2e" % (L, tiles, worst_x, worst_c), flush=True)
+    finally:
+        shear.close()
+
+
+def test_narrow_tiles_in_batched_plan(gpu):
+    """The batched plan (image index as a grid dimension) with the 4-column
+    tiles against the same images through the default schedule."""
+    from shearcuda_amd import api
+    from shearcuda_amd.synth import noisy_ellipses, white_noise
+    L, n = 256, 3
+    print("case batched narrow tiles L=%d n=%d" % (L, n), flush=True)
+    shear = api.make_dictionary(L)
+    b = api.Batch(shear, n)
+    try:
+        sc = api.nsst_scalars_cuda(shear, _gpu(white_noise(L)))
+        sig = np.array([10.0, 20.0, 30.0])
+        xs = _gpu(np.stack([noisy_ellipses(L, sigma=float(s), seed=40 + i)[1] for i, s in enumerate(sig)]))
+        th = api.batch_thresholds(shear, sig, sc)
+        outs = []
+        for ct in (8, 4):
+            api.set_schedule(shear, 0, 4, batch=b, col_tile_dec=ct, col_tile_rec=ct)
+            out = torch.empty_like(xs)
+            b.denoise(xs, b.alloc_coeffs(), th, out)
+            outs.append(out.cpu().numpy())
+        e = _rel(outs[1], outs[0])
+        print("batched 4-column vs 8-column rel diff %.2e" % e, flush=True)
+        assert e <= TOL
+    finally:
+        b.close()
+        shear.close()
