@@ -340,13 +340,27 @@ struct ColTile {
   static constexpr int dflt = (L >= 1024) ? 8 : ((L / 2) < 16 ? (L / 2) : 16);
 };
 
+// LDS layout of a column tile: row u of the column transforms, column c at
+//   u * CT + c + CT * (u / R1),
+// one pad of CT elements after every R1 rows.  Every access run of the column
+// passes is then a per-lane base plus compile-time offsets (no carry into the
+// pad term, so no per-element address registers), and the lanes of a pass-B
+// access (consecutive columns, rows R1 apart) fall on distinct bank pairs.
+template <int L, int CT>
+struct ColLds {
+  static constexpr int R1 = Split<L>::R1, R2 = Split<L>::R2;
+  static constexpr int BLK = R1 * CT + CT;  // R1 padded rows
+  static constexpr int SIZE = (L / R1) * BLK;
+  __device__ static __forceinline__ int ix(int u, int c) { return (u % R1) * CT + c + (u / R1) * BLK; }
+};
+
 template <int L, int CT_ = ColTile<L>::dflt>
 struct ColCfg {
   static constexpr int R1 = Split<L>::R1, R2 = Split<L>::R2;
   static constexpr int CT = CT_;
   static constexpr int T = CT * R2;
-  static constexpr int PADSH = CT >= 8 ? 7 : 6;  // one pad element per LDS row group
-  static constexpr int LDS = lds_size<L * CT, PADSH>();
+  using Lds = ColLds<L, CT_>;
+  static constexpr int LDS = Lds::SIZE;
 };
 
 // Column tile loads and stores through a buffer view of one tile (tile0:
@@ -417,8 +431,9 @@ __device__ __forceinline__ void col_fwd_A(const float2* __restrict__ src, float2
     __builtin_amdgcn_sched_barrier(0);  // data and twiddle loads in flight together
     dft_pair<R1, -1>(v, h);
     pair_tw_apply<R1, -1>(v, t);
+    float2* row = buf + n1 * CT + c + R1 * h * C::Lds::BLK;  // row n1 + R1 (k + R1 h)
 #pragma unroll
-    for (int k = 0; k < R1; ++k) buf[lds_ix<C::PADSH>((n1 + R1 * (k + R1 * h)) * CT + c)] = v[k];
+    for (int k = 0; k < R1; ++k) row[k * C::Lds::BLK] = v[k];
     return;
   }
   const int c = tid % CT, n1 = tid / CT;
@@ -426,8 +441,9 @@ __device__ __forceinline__ void col_fwd_A(const float2* __restrict__ src, float2
     float2 v[R2];
     col_load_A<L, CT_>(v, src);
     passA<R1, R2, -1, 1>(v, n1, tw);
+    float2* row = buf + n1 * CT + c;  // row n1 + R1 k
 #pragma unroll
-    for (int k = 0; k < R2; ++k) buf[lds_ix<C::PADSH>((n1 + R1 * k) * CT + c)] = v[k];
+    for (int k = 0; k < R2; ++k) row[k * C::Lds::BLK] = v[k];
   }
 }
 
@@ -437,8 +453,9 @@ __device__ __forceinline__ void col_fwd_B(float2 (&u)[Split<L>::R1], const float
   constexpr int R1 = C::R1, CT = C::CT;
   const int tid = threadIdx.x;
   const int c = tid % CT, k2 = tid / CT;
+  const float2* row = buf + c + k2 * C::Lds::BLK;  // row a + R1 k2
 #pragma unroll
-  for (int a = 0; a < R1; ++a) u[a] = buf[lds_ix<C::PADSH>((a + R1 * k2) * CT + c)];
+  for (int a = 0; a < R1; ++a) u[a] = row[a * CT];
   dft<R1, -1>(u);
 }
 
@@ -470,16 +487,20 @@ __device__ __forceinline__ void col_inv_from_B(float2 (&q)[Split<L>::R1], float2
   const int c = tid % CT, k2 = tid / CT;
   passA_apply<R1, +1>(q, t);
   __syncthreads();
+  {
+    float2* row = buf + C::Lds::ix(k2, c);  // row k2 + R2 k
 #pragma unroll
-  for (int k = 0; k < R1; ++k) buf[lds_ix<C::PADSH>((k2 + R2 * k) * CT + c)] = q[k];
+    for (int k = 0; k < R1; ++k) row[k * (R2 / R1) * C::Lds::BLK] = q[k];
+  }
   __syncthreads();
   if constexpr (ColPair<L>::on) {
     // thread (c, kb, h): z[j] = Y[2j + h] of output column kb, outputs k + R1 h
     const int h = tid & 1, rest = tid >> 1;
     const int cp = rest % CT, kb = rest / CT;
     float2 w[R1];
+    const float2* row = buf + h * CT + cp + kb * (R2 / R1) * C::Lds::BLK;  // row 2j + h + R2 kb
 #pragma unroll
-    for (int j = 0; j < R1; ++j) w[j] = buf[lds_ix<C::PADSH>((2 * j + h + R2 * kb) * CT + cp)];
+    for (int j = 0; j < R1; ++j) w[j] = row[((2 * j) % R1) * CT + ((2 * j) / R1) * C::Lds::BLK];
     dft_pair<R1, +1>(w, h);
     const BufView bv = col_view<L, CT>(dst);
     const unsigned lane = col_off<L>(kb + R1 * R1 * h, cp);
@@ -490,8 +511,9 @@ __device__ __forceinline__ void col_inv_from_B(float2 (&q)[Split<L>::R1], float2
   const int kb = tid / CT;
   if (kb < R1) {
     float2 w[R2];
+    const float2* row = buf + c + kb * (R2 / R1) * C::Lds::BLK;  // row a + R2 kb
 #pragma unroll
-    for (int a = 0; a < R2; ++a) w[a] = buf[lds_ix<C::PADSH>((a + R2 * kb) * CT + c)];
+    for (int a = 0; a < R2; ++a) w[a] = row[(a % R1) * CT + (a / R1) * C::Lds::BLK];
     dft<R2, +1>(w);
     const BufView bv = col_view<L, CT>(dst);
     const unsigned lane = col_off<L>(kb, c);
@@ -524,13 +546,16 @@ __device__ __forceinline__ void col_filter(float2 (&u)[Split<L>::R1], float2 (&o
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) hb[k1] = Hb[k2 + R2 * k1];
     __syncthreads();
+    {
+      float2* row = buf + C::Lds::ix(k2, c);  // row k2 + R2 k1
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) buf[lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c)] = u[k1];
+      for (int k1 = 0; k1 < R1; ++k1) row[k1 * (R2 / R1) * C::Lds::BLK] = u[k1];
+    }
     __syncthreads();
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       const int uu = k2 + R2 * k1;
-      const float2 pm = cconj(buf[lds_ix<C::PADSH>(((L - uu) & (L - 1)) * CT)]);
+      const float2 pm = cconj(buf[C::Lds::ix((L - uu) & (L - 1), 0)]);
       float2 q = CONJ ? cmulc(u[k1], h[k1]) : cmul(h[k1], u[k1]);
       const float2 e = CONJ ? cmulc(pm, hb[k1]) : cmul(hb[k1], pm);
       if (c == 0) q = cadd(q, e);
@@ -674,18 +699,25 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* _
     if constexpr (ColRecAcc<L, CT_>::LDS) {
       float2 q[R1];
       col_filter<L, CT_, true, false>(u, q, c0, h, Hbeta + (size_t)p * L, buf);
+      float2* row = accl + C::Lds::ix(k2, c);  // row k2 + R2 k1
+      if (p == p0) {
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) {
-        const int e = lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c);
-        accl[e] = (p == p0) ? q[k1] : cadd(accl[e], q[k1]);
+        for (int k1 = 0; k1 < R1; ++k1) row[k1 * (R2 / R1) * C::Lds::BLK] = q[k1];
+      } else {
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+          float2* e = row + k1 * (R2 / R1) * C::Lds::BLK;
+          *e = cadd(*e, q[k1]);
+        }
       }
     } else {
       col_filter<L, CT_, true, true>(u, acc, c0, h, Hbeta + (size_t)p * L, buf);
     }
   }
   if constexpr (ColRecAcc<L, CT_>::LDS) {
+    const float2* row = accl + C::Lds::ix(k2, c);
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) acc[k1] = accl[lds_ix<C::PADSH>((k2 + R2 * k1) * CT + c)];
+    for (int k1 = 0; k1 < R1; ++k1) acc[k1] = row[k1 * (R2 / R1) * C::Lds::BLK];
   }
   col_inv_from_B<L, CT_>(acc, Q + (size_t)b * q_stride + (size_t)g * L * NH + c0, buf, col_inv_tw<L>(k2, tw));
 }
