@@ -133,8 +133,9 @@ int shx_batch_shear_trans_adjoint(shx_batch* batch, const float* coeff_bar, floa
 int shx_batch_inverse_shear_adjoint(shx_batch* batch, const float* x_bar, float* coeff_bar, void* stream);
 
 /* Autotuner: times the pyramid tile shapes, the reconstruction's
- * directions-per-block grouping and the column tile width of the
- * decomposition and reconstruction column passes on the plan's own workspace
+ * directions-per-block grouping, the column tile width of the
+ * decomposition and reconstruction column passes and the decomposition's
+ * front end (shx_set_dec_front) on the plan's own workspace
  * (the dictionary's one-image plan when batch is NULL) and keeps the fastest.
  * chosen (may be NULL, else 4 ints) receives {pyramid variant, group,
  * decomposition column tile, reconstruction column tile}.  Each candidate is
@@ -145,6 +146,14 @@ int shx_autotune(shx_dict* dict, shx_batch* batch, int reps, int* chosen, void* 
 /* Sets a schedule explicitly (pyramid variant 0..2, group 2/4/8, column tiles
  * 4 (L >= 128), 8, or 16 (L <= 512); 0 = default). */
 int shx_set_schedule(shx_dict* dict, shx_batch* batch, int pyr_variant, int group, int ct_dec, int ct_rec);
+/* Front end of the decomposition (shear_trans, denoise) of a plan: 0 the a
+ * trous pyramid levels (atrousdec.m) followed by the FFT of every band-pass
+ * plane; 1 the same transform in the DCT-II domain, where the pyramid's
+ * symmetric extension makes every level a diagonal mask (three passes for
+ * all levels; L >= 64).  Both give the same coefficients to FP32 rounding.
+ * Default 0 until shx_autotune picks.  No counterpart in the reference. */
+int shx_set_dec_front(shx_dict* dict, shx_batch* batch, int front);
+int shx_get_dec_front(const shx_dict* dict, const shx_batch* batch, int* front);
 
 /* ---- HIP graphs --------------------------------------------------------------
  * Any sequence of shx calls that neither allocates nor synchronises

@@ -557,16 +557,32 @@ def batch_thresholds(shear, sigmas, scalars, Tscalars=DEFAULT_TSCALARS):
 
 def autotune(shear, batch=None, reps=5):
     """Pick the pyramid tile shape, the reconstruction's directions per column
-    block and the column tile widths for a plan by timing them (outside any
-    graph capture).  Returns the chosen schedule as a dict."""
+    block, the column tile widths and the decomposition front end for a plan
+    by timing them (outside any graph capture).  Returns the chosen schedule
+    as a dict."""
+    bh = batch._h if batch is not None else None
     chosen = (ctypes.c_int * 4)()
-    check(lib.shx_autotune(shear.handle, batch._h if batch is not None else None, int(reps), chosen, _stream()))
-    return {"pyramid_variant": chosen[0], "group": chosen[1], "col_tile_dec": chosen[2], "col_tile_rec": chosen[3]}
+    check(lib.shx_autotune(shear.handle, bh, int(reps), chosen, _stream()))
+    return {"pyramid_variant": chosen[0], "group": chosen[1], "col_tile_dec": chosen[2], "col_tile_rec": chosen[3],
+            "dct_front": get_dec_front(shear, batch)}
 
 
 def set_schedule(shear, pyramid_variant, group, batch=None, col_tile_dec=0, col_tile_rec=0):
     check(lib.shx_set_schedule(shear.handle, batch._h if batch is not None else None, int(pyramid_variant),
                                int(group), int(col_tile_dec), int(col_tile_rec)))
+
+
+def set_dec_front(shear, dct, batch=None):
+    """Decomposition front end of a plan: False the a trous pyramid levels
+    followed by the scale FFTs, True the same transform in the DCT-II domain
+    (L >= 64)."""
+    check(lib.shx_set_dec_front(shear.handle, batch._h if batch is not None else None, 1 if dct else 0))
+
+
+def get_dec_front(shear, batch=None):
+    f = ctypes.c_int()
+    check(lib.shx_get_dec_front(shear.handle, batch._h if batch is not None else None, ctypes.byref(f)))
+    return bool(f.value)
 
 
 # ---- undecimated wavelet transform -----------------------------------------------

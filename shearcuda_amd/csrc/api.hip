@@ -57,6 +57,7 @@ struct Work {
   int group = 4;        // directions per reconstruction column block
   int ct_dec = 0;       // column tile width of the decomposition / reconstruction
   int ct_rec = 0;       // column passes (0 = default for L)
+  bool dct_front = false;  // decomposition front end in the DCT domain (else pyramid levels + scale FFTs)
   BandMap map{};        // the dictionary's band map with this plan's groups
   // side streams (one per scale) and events of the concurrent denoise: each
   // scale's spectral chain overlaps the pyramid levels
@@ -73,6 +74,8 @@ struct shx_dict {
   float2* Hmain = nullptr;
   float2* Hbeta = nullptr;
   float2* tw = nullptr;
+  float* dmask = nullptr;  // DCT-domain pyramid masks, S + 1 planes (L >= 64)
+  float2* dtw = nullptr;   // DCT twiddles exp(-i pi k / 2L), k < L
   Work w;  // one image
   size_t bytes = 0;
   size_t plane() const { return (size_t)L * L; }
@@ -420,8 +423,9 @@ void stages_pyr_dec(shx_dict* d, const Work& w, int nimg, const float* x, long l
   }
 }
 
-void stages_spec_dec(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::vector<Stage>& st) {
-  const int L = d->L, S = d->S, nb = d->nbands;
+// Scale spectra (row FFT, column FFT of every pyramid band-pass plane).
+void stages_scale_fft(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::vector<Stage>& st) {
+  const int L = d->L, S = d->S;
   const double P = 4.0 * L * L * nimg;
   PlaneSet yset = shx::single_cell(w.planes, S, w.s_planes);
   PlaneSet specset = shx::single_cell(reinterpret_cast<float*>(w.spec), S, w.s_spec);
@@ -430,10 +434,56 @@ void stages_spec_dec(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::v
                 [=] { return shx::launch_rows_r2c(L, yset, specset, S, nimg, none, d->tw, s); }});
   st.push_back({"cols_fwd_scales", 2 * S * P,
                 [=] { return shx::launch_cols_fwd(L, w.spec, w.s_spec / 2, S, nimg, d->tw, s, w.ct_dec); }});
+}
+
+void stage_cols_dec(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::vector<Stage>& st) {
+  const int L = d->L, S = d->S, nb = d->nbands;
+  const double P = 4.0 * L * L * nimg;
   st.push_back({"cols_dec", (S + 2.0 * nb) * P, [=] {
                   return shx::launch_cols_dec(L, w.spec, w.s_spec / 2, w.band, w.s_band / 2, d->Hmain, d->Hbeta,
                                               w.map, nimg, d->tw, s, w.ct_dec, true);
                 }});
+}
+
+void stages_spec_dec(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::vector<Stage>& st) {
+  stages_scale_fft(d, w, nimg, s, st);
+  stage_cols_dec(d, w, nimg, s, st);
+}
+
+// DCT-domain front end: the lowpass (c.low) and the scale spectra (w.spec)
+// of the whole pyramid in three passes; the image's column DCT goes through
+// the first plane of w.planes.
+void stages_dct_front(shx_dict* d, const Work& w, int nimg, const float* x, long long xs, const Coeffs& c,
+                      hipStream_t s, std::vector<Stage>& st) {
+  const int L = d->L, S = d->S;
+  const double P = 4.0 * L * L * nimg;
+  const long long ss = w.s_spec / 2;
+  st.push_back({"dct_cols", 2 * P, [=] {
+                  return shx::launch_dct_dec(L, 0, x, xs, w.planes, w.s_planes, w.spec, ss, S, c.low, c.low_stride,
+                                             d->dmask, nimg, d->tw, d->dtw, s);
+                }});
+  // reads the column DCT and the S + 1 masks, writes S spectra and the lowpass
+  st.push_back({"dct_rows_dec", (2.0 * S + 3) * P, [=] {
+                  return shx::launch_dct_dec(L, 1, x, xs, w.planes, w.s_planes, w.spec, ss, S, c.low, c.low_stride,
+                                             d->dmask, nimg, d->tw, d->dtw, s);
+                }});
+  st.push_back({"dct_cols_dec", 2.0 * (S + 1) * P, [=] {
+                  return shx::launch_dct_dec(L, 2, x, xs, w.planes, w.s_planes, w.spec, ss, S, c.low, c.low_stride,
+                                             d->dmask, nimg, d->tw, d->dtw, s);
+                }});
+}
+
+// Decomposition up to the band spectra T (w.band): either front end, then the
+// column filter pass.
+void stages_front_dec(shx_dict* d, const Work& w, int nimg, const float* x, long long xs, const Coeffs& c,
+                      hipStream_t s, std::vector<Stage>& st) {
+  if (w.dct_front) {
+    stages_dct_front(d, w, nimg, x, xs, c, s, st);
+  } else {
+    stages_pyr_dec(d, w, nimg, x, xs, c, s, st);
+    stages_scale_fft(d, w, nimg, s, st);
+  }
+  stage_cols_dec(d, w, nimg, s, st);
 }
 
 shx::ThreshSpec band_thresh(const shx_dict* d, const Thr& t) {
@@ -492,8 +542,7 @@ void stages_pyr_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const
 // Forward transform: pyramid, spectral pass, band rows written to c.bands.
 void stages_dec(shx_dict* d, const Work& w, int nimg, const float* x, long long xs, const Coeffs& c, hipStream_t s,
                 std::vector<Stage>& st) {
-  stages_pyr_dec(d, w, nimg, x, xs, c, s, st);
-  stages_spec_dec(d, w, nimg, s, st);
+  stages_front_dec(d, w, nimg, x, xs, c, s, st);
   const int L = d->L, nb = d->nbands;
   const double P = 4.0 * L * L * nimg;
   PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
@@ -520,8 +569,7 @@ void stages_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const Thr
 // thresholded band row pass run as one kernel over each band row.
 void stages_denoise(shx_dict* d, const Work& w, int nimg, const float* x, long long xs, const Coeffs& c,
                     const Thr& t, float* out, long long os, hipStream_t s, std::vector<Stage>& st) {
-  stages_pyr_dec(d, w, nimg, x, xs, c, s, st);
-  stages_spec_dec(d, w, nimg, s, st);
+  stages_front_dec(d, w, nimg, x, xs, c, s, st);
   const int L = d->L, nb = d->nbands;
   const double P = 4.0 * L * L * nimg;
   PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
@@ -553,7 +601,17 @@ void stages_denoise_dag(shx_dict* d, Work& w, int nimg, const float* x, long lon
   const long long pl = (long long)L * L;
   const double P = 4.0 * pl * nimg;
   std::vector<int> made(S), done(S);
-  {
+  if (w.dct_front) {
+    // every scale spectrum comes out of the front end's last pass
+    std::vector<Stage> tmp;
+    stages_dct_front(d, w, nimg, x, xs, c, s0, tmp);
+    for (Stage& g : tmp) {
+      g.stream = s0;
+      st.push_back(g);
+    }
+    st.back().record = true;
+    for (int j = 0; j < S; ++j) made[j] = (int)st.size() - 1;
+  } else {
     std::vector<Stage> tmp;
     stages_pyr_dec(d, w, nimg, x, xs, c, s0, tmp);
     for (int lev = 0; lev < S; ++lev) {
@@ -594,20 +652,23 @@ void stages_denoise_dag(shx_dict* d, Work& w, int nimg, const float* x, long lon
     const float2* Hb = d->Hbeta + (size_t)f * L;
     shx::ThreshSpec none{};
     const std::string sfx = "_s" + std::to_string(j);
-    Stage a{"rows_r2c" + sfx, 2 * P, [=] { return shx::launch_rows_r2c(L, ypl, spc, 1, nimg, none, d->tw, sj); }};
-    a.stream = sj;
-    a.waits = {made[j]};
-    st.push_back(a);
-    float2* Y1w = w.spec + (size_t)j * L * NH;
-    Stage cf{"cols_fwd" + sfx, 2 * P,
-             [=] { return shx::launch_cols_fwd(L, Y1w, w.s_spec / 2, 1, nimg, d->tw, sj, w.ct_dec); }};
-    cf.stream = sj;
-    st.push_back(cf);
+    if (!w.dct_front) {
+      Stage a{"rows_r2c" + sfx, 2 * P, [=] { return shx::launch_rows_r2c(L, ypl, spc, 1, nimg, none, d->tw, sj); }};
+      a.stream = sj;
+      a.waits = {made[j]};
+      st.push_back(a);
+      float2* Y1w = w.spec + (size_t)j * L * NH;
+      Stage cf{"cols_fwd" + sfx, 2 * P,
+               [=] { return shx::launch_cols_fwd(L, Y1w, w.s_spec / 2, 1, nimg, d->tw, sj, w.ct_dec); }};
+      cf.stream = sj;
+      st.push_back(cf);
+    }
     Stage b{"cols_dec" + sfx, (1 + 2.0 * nb) * P, [=] {
               return shx::launch_cols_dec(L, Y1, w.s_spec / 2, T, w.s_band / 2, Hm, Hb, m, nimg, d->tw, sj, w.ct_dec,
                                           true);
             }};
     b.stream = sj;
+    if (w.dct_front) b.waits = {made[j]};
     st.push_back(b);
     Stage cc{"rows_c2r_r2c" + sfx, 3.0 * nb * P,
              [=] { return shx::launch_rows_c2r_r2c(L, bnd, cb, bnd, nb, nimg, th, d->tw, sj); }};
@@ -920,6 +981,24 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t caller, 
   for (int c = 1; c < nct; ++c)
     if (td[c] < td[bd]) bd = c;
   w.ct_dec = ccand[bd];
+  // decomposition front end: pyramid levels + scale FFTs, or the DCT domain
+  if (d->dmask) {
+    double tf[2] = {1e30, 1e30};
+    for (int round = 0; round < 3; ++round)
+      for (int f = 0; f < 2; ++f) {
+        w.dct_front = f == 1;
+        std::vector<Stage> st;
+        if (w.dct_front) {
+          stages_dct_front(d, w, nimg, img, pl, c, s, st);
+        } else {
+          stages_pyr_dec(d, w, nimg, img, pl, c, s, st);
+          stages_scale_fft(d, w, nimg, s, st);
+        }
+        const double t = time_stages_graph(st, s, reps);
+        if (t > 0 && t < tf[f]) tf[f] = t;
+      }
+    w.dct_front = tf[1] < tf[0];
+  }
   hipStreamSynchronize(s);
   hipFree(img);
   hipFree(coeff);
@@ -1006,10 +1085,21 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
                            hb.data() + (size_t)d->map.first[j] * L * 2);
   }
   shx::twiddles_fp32(L, tw.data());
+  // DCT-domain front end (L >= 64): masks and twiddles
+  std::vector<float> dmask, dtw;
+  if (L >= 64) {
+    dmask.resize((size_t)(nscales + 1) * L * L);
+    dtw.resize((size_t)2 * L);
+    shx::dct_pyramid_masks(L, nscales, dmask.data());
+    shx::dct_twiddles_fp32(L, dtw.data());
+  }
   const size_t bytes_hm = hm.size() * sizeof(float), bytes_hb = hb.size() * sizeof(float);
+  const size_t bytes_dm = dmask.size() * sizeof(float), bytes_dt = dtw.size() * sizeof(float);
   hipError_t e = hipSuccess;
   if ((e = hipMalloc(&d->Hmain, bytes_hm)) != hipSuccess || (e = hipMalloc(&d->Hbeta, bytes_hb)) != hipSuccess ||
-      (e = hipMalloc(&d->tw, tw.size() * sizeof(float))) != hipSuccess) {
+      (e = hipMalloc(&d->tw, tw.size() * sizeof(float))) != hipSuccess ||
+      (bytes_dm && ((e = hipMalloc(&d->dmask, bytes_dm)) != hipSuccess ||
+                    (e = hipMalloc(&d->dtw, bytes_dt)) != hipSuccess))) {
     shx_clear_shearing_filters(d);
     return fail(SHX_ERR_ALLOC, "device allocation failed: %s", hipGetErrorString(e));
   }
@@ -1017,10 +1107,12 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
     shx_clear_shearing_filters(d);
     return r;
   }
-  d->bytes = bytes_hm + bytes_hb + tw.size() * sizeof(float) + d->w.bytes;
+  d->bytes = bytes_hm + bytes_hb + tw.size() * sizeof(float) + bytes_dm + bytes_dt + d->w.bytes;
   if ((e = hipMemcpy(d->Hmain, hm.data(), bytes_hm, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(d->Hbeta, hb.data(), bytes_hb, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(d->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
+      (e = hipMemcpy(d->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+      (bytes_dm && ((e = hipMemcpy(d->dmask, dmask.data(), bytes_dm, hipMemcpyHostToDevice)) != hipSuccess ||
+                    (e = hipMemcpy(d->dtw, dtw.data(), bytes_dt, hipMemcpyHostToDevice)) != hipSuccess))) {
     shx_clear_shearing_filters(d);
     return fail(SHX_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
   }
@@ -1033,6 +1125,8 @@ int shx_clear_shearing_filters(shx_dict* d) {
   hipFree(d->Hmain);
   hipFree(d->Hbeta);
   hipFree(d->tw);
+  hipFree(d->dmask);
+  hipFree(d->dtw);
   free_work(d->w);
   delete d;
   return SHX_OK;
@@ -1185,6 +1279,22 @@ int shx_autotune(shx_dict* d, shx_batch* b, int reps, int* chosen, void* stream)
   if (reps < 1) reps = 3;
   Work& w = b ? b->w : d->w;
   return autotune_work(d, w, b ? b->nimg : 1, reps, (hipStream_t)stream, chosen);
+}
+
+int shx_set_dec_front(shx_dict* d, shx_batch* b, int front) {
+  if (!d) return fail(SHX_ERR_ARG, "null dictionary");
+  if (b && b->d != d) return fail(SHX_ERR_ARG, "batch belongs to another dictionary");
+  if (front != 0 && front != 1) return fail(SHX_ERR_ARG, "decomposition front end %d (0 pyramid levels, 1 DCT domain)", front);
+  if (front == 1 && !d->dmask) return fail(SHX_ERR_UNSUPPORTED, "DCT-domain front end needs L >= 64 (L = %d)", d->L);
+  (b ? b->w : d->w).dct_front = front == 1;
+  return SHX_OK;
+}
+
+int shx_get_dec_front(const shx_dict* d, const shx_batch* b, int* front) {
+  if (!d || !front) return fail(SHX_ERR_ARG, "null dictionary or output");
+  if (b && b->d != d) return fail(SHX_ERR_ARG, "batch belongs to another dictionary");
+  *front = (b ? b->w : d->w).dct_front ? 1 : 0;
+  return SHX_OK;
 }
 
 int shx_set_schedule(shx_dict* d, shx_batch* b, int pyr_variant, int group, int ct_dec, int ct_rec) {

@@ -131,4 +131,12 @@ void twiddles_fp32(int L, float* out) {
   }
 }
 
+void dct_twiddles_fp32(int L, float* out) {
+  for (int k = 0; k < L; ++k) {
+    const double a = -M_PI * (double)k / (2.0 * L);
+    out[2 * k + 0] = (float)std::cos(a);
+    out[2 * k + 1] = (float)std::sin(a);
+  }
+}
+
 }  // namespace shx

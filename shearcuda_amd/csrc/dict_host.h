@@ -14,5 +14,8 @@ void scale_spectra_fp64(int L, int D, int n1, const double* w, int shift, std::v
 void pack_spectra_fp32(int L, int D, const std::vector<std::complex<double>>& H, float* main_out, float* beta_out);
 // tw[k] = exp(-2 pi i k / L), k < L, interleaved (re, im).
 void twiddles_fp32(int L, float* out);
+// dtw[k] = exp(-i pi k / (2L)), k < L, interleaved (re, im): the DCT-II
+// pre/post twiddles of an L-point transform computed by an L-point FFT.
+void dct_twiddles_fp32(int L, float* out);
 
 }  // namespace shx

@@ -226,6 +226,57 @@ bool maxflat_pair_fp64(int pair, int* len, int* rank, const double** u, const do
   return true;
 }
 
+namespace {
+// R[kr][kc] = sum_{a,b} h[a][b] cos(pi kr (a-c) m / L) cos(pi kc (b-c) m / L),
+// c the centre tap: the eigenvalues of the symmetric filter h dilated by m
+// under the half-sample symmetric extension (period 2L), on the DCT-II basis.
+std::vector<double> dct_response(const Full& f, int m, int L) {
+  const int n = f.len, c = n / 2;
+  const double pi = 3.14159265358979323846;
+  std::vector<double> C((size_t)L * n);
+  for (int k = 0; k < L; ++k)
+    for (int t = 0; t < n; ++t) {
+      long long q = ((long long)k * (t - c) * m) % (2LL * L);  // angle pi q / L, reduced exactly
+      if (q < 0) q += 2LL * L;
+      C[(size_t)k * n + t] = std::cos(pi * (double)q / L);
+    }
+  std::vector<double> T((size_t)n * L);  // T = h C^T
+  for (int a = 0; a < n; ++a)
+    for (int k = 0; k < L; ++k) {
+      double s = 0.0;
+      for (int b = 0; b < n; ++b) s += f.h[(size_t)a * n + b] * C[(size_t)k * n + b];
+      T[(size_t)a * L + k] = s;
+    }
+  std::vector<double> R((size_t)L * L, 0.0);
+  for (int kr = 0; kr < L; ++kr) {
+    double* r = &R[(size_t)kr * L];
+    for (int a = 0; a < n; ++a) {
+      const double ca = C[(size_t)kr * n + a];
+      const double* t = &T[(size_t)a * L];
+      for (int kc = 0; kc < L; ++kc) r[kc] += ca * t[kc];
+    }
+  }
+  return R;
+}
+}  // namespace
+
+void dct_pyramid_masks(int L, int nscales, float* out) {
+  const Maxflat& M = maxflat();
+  const size_t pe = (size_t)L * L;
+  const double inv = 1.0 / ((double)L * L);
+  std::vector<double> P(pe, 1.0);
+  for (int lev = 0; lev < nscales; ++lev) {
+    const int m = 1 << lev;
+    const std::vector<double> H1 = dct_response(M.full[1], m, L);
+    float* o = out + (size_t)(nscales - 1 - lev) * pe;
+    for (size_t i = 0; i < pe; ++i) o[i] = (float)(H1[i] * P[i] * inv);
+    const std::vector<double> H0 = dct_response(M.full[0], m, L);
+    for (size_t i = 0; i < pe; ++i) P[i] *= H0[i];
+  }
+  float* o = out + (size_t)nscales * pe;
+  for (size_t i = 0; i < pe; ++i) o[i] = (float)(P[i] * inv);
+}
+
 double maxflat_factor_error() {
   const Maxflat& M = maxflat();
   if (!M.ok) return INFINITY;

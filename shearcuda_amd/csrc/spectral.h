@@ -75,6 +75,14 @@ hipError_t launch_cols_dec(int L, const float2* Y1, long long y1_stride, float2*
 // In-place forward column FFT of nplanes packed spectra (stride y1_stride per image).
 hipError_t launch_cols_fwd(int L, float2* Y1, long long y1_stride, int nplanes, int nimg, const float2* tw,
                            hipStream_t s, int ct = 0);
+// DCT-domain pyramid decomposition (L >= 64), steps 0..2: 0: xwork = column
+// DCT-II of x; 1: rows (DCT-II, mask plane p, inverse, row FFT) into the
+// packed spec planes p < nspec and the real lowpass rows; 2: columns (inverse
+// DCT-II, column FFT) in place.  mask: nspec + 1 planes of L*L, plane nspec
+// the lowpass.  dtw[k] = e^{-i pi k / (2L)}, k < L.
+hipError_t launch_dct_dec(int L, int step, const float* x, long long xs, float* xwork, long long ws, float2* spec,
+                          long long ss, int nspec, float* low, long long ls, const float* mask, int nimg,
+                          const float2* tw, const float2* dtw, hipStream_t s);
 hipError_t launch_cols_rec(int L, const float2* U, long long u_stride, float2* Q, long long q_stride,
                            const float2* Hm, const float2* Hb, const BandMap& map, int nimg, const float2* tw,
                            hipStream_t s, int ct = 0);

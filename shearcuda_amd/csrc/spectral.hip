@@ -154,6 +154,34 @@ __device__ __forceinline__ void row_fwd_from_A(float2 (&v)[R2], bool validA, int
   }
 }
 
+// As row_fwd_from_A, but the packed half spectrum stays in registers: thread
+// (rB, k2) gets P[k1] = P(k2 + R2 k1).  All threads of the block must call it.
+template <int L, int R1, int R2, int PADSH>
+__device__ __forceinline__ void row_fwd_to_P(float2 (&v)[R2], bool validA, int r, int n1, float2* buf, int rB,
+                                             int k2, const RowFwdTw<L, R1, R2>& t, float2 (&P)[R1]) {
+  constexpr int N = L / 2;
+  if (validA) {
+    passA_apply<R2, -1>(v, pin(t.pt));
+#pragma unroll
+    for (int k = 0; k < R2; ++k) buf[lds_ix<PADSH>(r * N + n1 + R1 * k)] = v[k];
+  }
+  __syncthreads();
+  float2 u[R1];
+#pragma unroll
+  for (int a = 0; a < R1; ++a) u[a] = buf[lds_ix<PADSH>(rB * N + a + R1 * k2)];
+  dft<R1, -1>(u);
+  const int lane = threadIdx.x & 63;
+  const int src = (lane & ~(R2 - 1)) | ((R2 - k2) & (R2 - 1));
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const float2 sh = shfl2(u[R1 - 1 - k1], src);
+    const int k = k2 + R2 * k1;
+    const float2 zm = (k2 == 0) ? u[(R1 - k1) & (R1 - 1)] : sh;
+    P[k1] = (k == 0) ? make_float2(u[0].x + u[0].y, u[0].x - u[0].y)
+                     : r2c_post(u[k1], zm, ctw_mul<L / R2, -1>(t.w0, k1));
+  }
+}
+
 // P[k] and P[N-k] of one packed row for k = n1 + R1 n2, all loads issued
 // before any is used.
 template <int N, int R1, int R2>
@@ -465,6 +493,98 @@ __device__ __forceinline__ void col_fwd_to_B(float2 (&u)[Split<L>::R1], const fl
   col_fwd_A<L, CT_>(src, buf, tw);
   __syncthreads();
   col_fwd_B<L, CT_>(u, buf);
+}
+
+// Pass A of a forward column FFT whose input element (row, c) of this
+// thread's column comes from load(row, c) (a permuted or LDS source).  SYNC:
+// a barrier between the loads and the LDS stores (the source is buf itself).
+template <int L, int CT_, bool SYNC, class LOAD>
+__device__ __forceinline__ void col_fwd_A_src(LOAD load, float2* buf, const float2* __restrict__ tw) {
+  using C = ColCfg<L, CT_>;
+  constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT;
+  const int tid = threadIdx.x;
+  if constexpr (ColPair<L>::on) {
+    const int h = tid & 1, rest = tid >> 1;
+    const int c = rest % CT, n1 = rest / CT;
+    float2 v[R1];
+#pragma unroll
+    for (int j = 0; j < R1; ++j) v[j] = load(n1 + R1 * (2 * j + h), c);
+    PairTw<R1> t;
+    pair_tw<R1, 1>(t, n1, h, tw);
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (SYNC) __syncthreads();
+    dft_pair<R1, -1>(v, h);
+    pair_tw_apply<R1, -1>(v, t);
+    float2* row = buf + n1 * CT + c + R1 * h * C::Lds::BLK;
+#pragma unroll
+    for (int k = 0; k < R1; ++k) row[k * C::Lds::BLK] = v[k];
+  } else {
+    const int c = tid % CT, n1 = tid / CT;
+    float2 v[R2];
+    if (n1 < R1) {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) v[n2] = load(n1 + R1 * n2, c);
+    }
+    if constexpr (SYNC) __syncthreads();
+    if (n1 < R1) {
+      passA<R1, R2, -1, 1>(v, n1, tw);
+      float2* row = buf + n1 * CT + c;
+#pragma unroll
+      for (int k = 0; k < R2; ++k) row[k * C::Lds::BLK] = v[k];
+    }
+  }
+}
+
+// First half of an inverse column transform: pass-A twiddles on the pass-B
+// layout registers q (thread (c, k2) holding Q[k2 + R2 k1]) and the LDS
+// transpose, between two barriers.
+template <int L, int CT_>
+__device__ __forceinline__ void col_inv_A(float2 (&q)[Split<L>::R1], float2* buf, const PassTw<Split<L>::R1>& t) {
+  using C = ColCfg<L, CT_>;
+  constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT;
+  const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
+  passA_apply<R1, +1>(q, t);
+  __syncthreads();
+  float2* row = buf + C::Lds::ix(k2, c);
+#pragma unroll
+  for (int k = 0; k < R1; ++k) row[k * (R2 / R1) * C::Lds::BLK] = q[k];
+  __syncthreads();
+}
+
+// Second half: the last DFT, then store(row, c, value) for every output
+// element (row of the inverse transform, column c).  SYNC: a barrier between
+// the LDS reads and the stores (store may write buf).
+template <int L, int CT_, bool SYNC, class STORE>
+__device__ __forceinline__ void col_inv_B_to(const float2* buf, STORE store) {
+  using C = ColCfg<L, CT_>;
+  constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT;
+  const int tid = threadIdx.x;
+  if constexpr (ColPair<L>::on) {
+    const int h = tid & 1, rest = tid >> 1;
+    const int cp = rest % CT, kb = rest / CT;
+    float2 w[R1];
+    const float2* row = buf + h * CT + cp + kb * (R2 / R1) * C::Lds::BLK;
+#pragma unroll
+    for (int j = 0; j < R1; ++j) w[j] = row[((2 * j) % R1) * CT + ((2 * j) / R1) * C::Lds::BLK];
+    dft_pair<R1, +1>(w, h);
+    if constexpr (SYNC) __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R1; ++k) store(kb + R1 * (k + R1 * h), cp, w[k]);
+  } else {
+    const int c = tid % CT, kb = tid / CT;
+    float2 w[R2];
+    if (kb < R1) {
+      const float2* row = buf + c + kb * (R2 / R1) * C::Lds::BLK;
+#pragma unroll
+      for (int a = 0; a < R2; ++a) w[a] = row[(a % R1) * CT + (a / R1) * C::Lds::BLK];
+      dft<R2, +1>(w);
+    }
+    if constexpr (SYNC) __syncthreads();
+    if (kb < R1) {
+#pragma unroll
+      for (int k1 = 0; k1 < R2; ++k1) store(kb + R1 * k1, c, w[k1]);
+    }
+  }
 }
 
 // Inverse column transform from pass-B layout registers (thread (c, k2)
@@ -839,6 +959,262 @@ hipError_t launch_cols_dec(int L, const float2* Y1, long long ys, float2* To, lo
 hipError_t launch_cols_fwd(int L, float2* Y1, long long ys, int nplanes, int nimg, const float2* tw, hipStream_t s,
                            int ct) {
   SHX_BY_L(L, LT::cols_fwd(Y1, ys, nplanes, nimg, tw, s, ct))
+}
+
+// ---------------------------------------------------------------------------
+// DCT-domain pyramid decomposition.
+//
+// With the half-sample symmetric extension of the a trous pyramid and its
+// symmetric filters, every level is diagonal in the 2D DCT-II basis.  With
+// X = D2(x), the raw DCT-II X[k] = sum_n x[n] cos(pi k (2n+1) / 2L) on both
+// axes, the band-pass plane of scale j is y_j = D2^-1(M_j . X) and the
+// lowpass D2^-1(M_low . X), where M_j = H1(m_j) prod_{i<j} H0(m_i) multiplies
+// the filters' cosine responses at the dilations m = 1, 2, 4, ... (built on
+// the host in FP64, divided by L^2 for the two unnormalised inverse passes).
+// The pyramid's S dependent levels and the scale spectra's two FFT passes
+// become three passes:
+//   k_dct2_cols     columns of x: D2 (two real columns as one complex column)
+//   k_dct_rows_dec  per (row block, plane): rows of X: D2, times the plane's
+//                   mask, D2^-1, then the forward real row FFT (scale planes:
+//                   the packed spectrum rows k_rows_r2c writes; lowpass: the
+//                   real row)
+//   k_dct_cols_dec  per (column tile, plane): D2^-1 along the columns, then
+//                   the forward column FFT (scale planes, in place: the
+//                   spectra k_cols_fwd produces); lowpass: D2^-1 only
+// DCT-II by an L-point FFT (Makhoul): v[n] = x[perm(n)], perm(n) = 2n for
+// n < L/2 and 2L-1-2n above, V = FFT(v) and
+//   X[k] = (e^{-i t_k} V[k] + e^{i t_k} V[L-k]) / 2,   t_k = pi k / (2L)
+// (Re(e^{-i t_k} V[k]) for real x).  Inverse: V[k] = e^{i t_k} (X[k] - i X[L-k])
+// with X[L] = 0, v = IFFT(V), x[perm(n)] = v[n].  dtw[k] = e^{-i t_k}.
+__device__ __forceinline__ int dct_perm(int n, int L) { return n < L / 2 ? 2 * n : 2 * L - 1 - 2 * n; }
+
+// Columns of real planes (pairs of columns as one complex column): X = D2.
+// grid: (L/2 / CT, nimg).
+template <int L, int CT_>
+__global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct2_cols(const float* __restrict__ x, long long xs,
+                                                             float* __restrict__ out, long long os,
+                                                             const float2* __restrict__ tw,
+                                                             const float2* __restrict__ dtw) {
+  using C = ColCfg<L, CT_>;
+  constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
+  __shared__ float2 buf[C::LDS];
+  const int c0 = blockIdx.x * CT, b = blockIdx.y;
+  const float2* src = reinterpret_cast<const float2*>(x + (size_t)b * xs) + c0;
+  float2* dst = reinterpret_cast<float2*>(out + (size_t)b * os) + c0;
+  col_fwd_A_src<L, CT_, false>([&](int row, int c) { return src[(size_t)dct_perm(row, L) * NH + c]; }, buf, tw);
+  __syncthreads();
+  float2 u[R1];
+  col_fwd_B<L, CT_>(u, buf);
+  const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
+  const float2 w0 = dtw[k2];
+  __syncthreads();  // pass-B reads of buf are complete
+  {
+    float2* row = buf + C::Lds::ix(k2, c);  // V at row k = k2 + R2 k1
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) row[k1 * (R2 / R1) * C::Lds::BLK] = u[k1];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const int k = k2 + R2 * k1;
+    const float2 vm = buf[C::Lds::ix((L - k) & (L - 1), c)];
+    const float2 e = ctw_mul<4 * L / R2, -1>(w0, k1);
+    dst[(size_t)k * NH + c] = cscale(cadd(cmul(e, u[k1]), cmulc(vm, e)), 0.5f);
+  }
+}
+
+// Rows of X: D2, times mask plane p, D2^-1 and (scale planes) the forward
+// real row FFT.  Plane p < nspec: packed spectrum rows of spec plane p;
+// p == nspec: the real rows of the lowpass plane.  grid: (L / ROWS, nspec + 1,
+// nimg).  The rows of one block go through the row transforms of RowCfg<L, 4>.
+template <int L>
+using DctRowCfg = RowCfg<L, 4>;
+
+template <int L>
+__global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_dec(const float* __restrict__ X, long long xs,
+                                                                   float2* __restrict__ spec, long long ss, int nspec,
+                                                                   float* __restrict__ low, long long ls,
+                                                                   const float* __restrict__ mask,
+                                                                   const float2* __restrict__ tw,
+                                                                   const float2* __restrict__ dtw) {
+  using C = DctRowCfg<L>;
+  constexpr int N = C::N, R1 = C::R1, R2 = C::R2, ROWS = C::ROWS;
+  __shared__ float2 buf[C::LDS];
+  __shared__ float vrow[ROWS][L];
+  const int tid = threadIdx.x;
+  const int p = blockIdx.y, b = blockIdx.z;
+  const int row0 = blockIdx.x * ROWS;
+  const float* xr0 = X + (size_t)b * xs + (size_t)row0 * L;
+  // D2 of each row: the real FFT of v = x[perm(.)], whose (even, odd) pairs
+  // are z[n] = (x[perm(2n)], x[perm(2n+1)])
+  const int r = tid / R1, n1 = tid % R1;
+  const bool validA = r < ROWS;
+  const int rB = tid / R2, k2 = tid % R2;
+  float2 v[R2];
+  if (validA) {
+    const float* xr = xr0 + (size_t)r * L;
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = n1 + R1 * n2;
+      v[n2] = make_float2(xr[dct_perm(2 * n, L)], xr[dct_perm(2 * n + 1, L)]);
+    }
+  }
+  const auto ftw = row_fwd_tw<L, R1, R2>(n1, k2, tw);
+  const float2 w0 = dtw[k2];
+  const float* mrow = mask + (size_t)p * L * L + (size_t)(row0 + rB) * L;
+  float mk[R1], mm[R1];
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const int k = k2 + R2 * k1;
+    mk[k1] = mrow[k];
+    mm[k1] = mrow[k == 0 ? N : L - k];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  float2 P[R1];
+  row_fwd_to_P<L, R1, R2, C::PADSH>(v, validA, r, n1, buf, rB, k2, ftw, P);
+  // mask and D2^-1 pre-twiddle: P'[k] = e^{i t_k} (m[k] W.x + i m[L-k] W.y),
+  // W = e^{-i t_k} P[k]; k = 0 packs (m[0] X[0], m[L/2] V[L/2])
+  float2 Q[R1];
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const int k = k2 + R2 * k1;
+    const float2 e = ctw_mul<4 * L / R2, -1>(w0, k1);
+    const float2 W = cmul(e, P[k1]);
+    Q[k1] = (k == 0) ? make_float2(mk[k1] * P[k1].x, mm[k1] * P[k1].y)
+                     : cmulc(make_float2(mk[k1] * W.x, mm[k1] * W.y), e);
+  }
+  // inverse real row FFT from registers (split swapped): thread (rB, k2) is
+  // the pass-A index k2 holding Q(k2 + R2 k1); partners through shuffles
+  const int lane = threadIdx.x & 63;
+  const int srcl = (lane & ~(R2 - 1)) | ((R2 - k2) & (R2 - 1));
+  auto gather = [&](int, int, float2 (&xq)[R1], float2 (&xm)[R1]) {
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const float2 sh = shfl2(Q[R1 - 1 - k1], srcl);
+      xq[k1] = Q[k1];
+      xm[k1] = (k2 == 0) ? Q[(R1 - k1) & (R1 - 1)] : sh;
+    }
+  };
+  __syncthreads();  // pass-B reads of buf are complete
+  const int rC = tid / R1, kc = tid % R1;
+  const bool validC = tid < ROWS * R1;
+  float2 z[R2];
+  row_inv_to_B<L, R2, R1, C::PADSH>(z, true, rB, k2, buf, validC, rC, kc, tw, gather);
+  // z[k1] = L (v'[2m], v'[2m+1]), m = kc + R1 k1: v' in natural order to LDS
+  if (validC) {
+#pragma unroll
+    for (int k1 = 0; k1 < R2; ++k1) {
+      const int m = kc + R1 * k1;
+      vrow[rC][2 * m] = z[k1].x;
+      vrow[rC][2 * m + 1] = z[k1].y;
+    }
+  }
+  __syncthreads();
+  // y[perm(n)] = v'[n]: the real row y as pairs (y[2n], y[2n+1]) = (v'[n], v'[L-1-n])
+  float2 yz[R2];
+  if (validA) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = n1 + R1 * n2;
+      yz[n2] = make_float2(vrow[r][n], vrow[r][L - 1 - n]);
+    }
+  }
+  if (p == nspec) {
+    if (validA) {
+      float2* lr = reinterpret_cast<float2*>(low + (size_t)b * ls + (size_t)(row0 + r) * L);
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) lr[n1 + R1 * n2] = yz[n2];
+    }
+    return;
+  }
+  float2* dst = spec + (size_t)b * ss + (size_t)p * L * N + (size_t)row0 * N;
+  row_fwd_from_A<L, R1, R2, C::PADSH>(yz, validA, r, n1, buf, dst, true, rB, k2, ftw);
+}
+
+// Columns of the planes k_dct_rows_dec wrote: D2^-1, then (scale planes) the
+// forward column FFT, in place.  Plane p < nspec: spec plane p (complex
+// columns); p == nspec: the lowpass plane (pairs of real columns).
+// grid: (L/2 / CT, nspec + 1, nimg).
+template <int L, int CT_>
+__global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct_cols_dec(float2* __restrict__ spec, long long ss,
+                                                                int nspec, float* __restrict__ low, long long ls,
+                                                                const float2* __restrict__ tw,
+                                                                const float2* __restrict__ dtw) {
+  using C = ColCfg<L, CT_>;
+  constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
+  __shared__ float2 buf[C::LDS];
+  const int c0 = blockIdx.x * CT, p = blockIdx.y, b = blockIdx.z;
+  float2* base = (p == nspec ? reinterpret_cast<float2*>(low + (size_t)b * ls)
+                             : spec + (size_t)b * ss + (size_t)p * L * NH) + c0;
+  const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
+  // pre-twiddle V[k] = e^{i t_k} (X[k] - i X[L-k]) in pass-B layout
+  float2 xk[R1], xm[R1];
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const int k = k2 + R2 * k1;
+    xk[k1] = base[(size_t)k * NH + c];
+    xm[k1] = k == 0 ? make_float2(0.f, 0.f) : base[(size_t)(L - k) * NH + c];
+  }
+  const float2 w0 = dtw[k2];
+  const PassTw<R1> ivt = col_inv_tw<L>(k2, tw);
+  __builtin_amdgcn_sched_barrier(0);
+  float2 q[R1];
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const float2 e = ctw_mul<4 * L / R2, -1>(w0, k1);
+    q[k1] = cmulc(make_float2(xk[k1].x + xm[k1].y, xk[k1].y - xm[k1].x), e);
+  }
+  col_inv_A<L, CT_>(q, buf, ivt);
+  if (p == nspec) {
+    col_inv_B_to<L, CT_, false>(buf, [&](int row, int cc, float2 val) { base[(size_t)dct_perm(row, L) * NH + cc] = val; });
+    return;
+  }
+  // y[perm(n)] = v[n] into LDS (natural rows), then the forward FFT from there
+  col_inv_B_to<L, CT_, true>(buf, [&](int row, int cc, float2 val) { buf[C::Lds::ix(dct_perm(row, L), cc)] = val; });
+  __syncthreads();
+  col_fwd_A_src<L, CT_, true>([&](int row, int cc) { return buf[C::Lds::ix(row, cc)]; }, buf, tw);
+  __syncthreads();
+  float2 u[R1];
+  col_fwd_B<L, CT_>(u, buf);
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) base[(size_t)(k2 + R2 * k1) * NH + c] = u[k1];
+}
+
+// Column tiles: the single-plane first pass takes the narrowest tile (most
+// blocks); the per-plane last pass the narrow tile.
+template <int L>
+struct LaunchDct {
+  static hipError_t dec(const float* x, long long xs, float* xwork, long long ws, float2* spec, long long ss,
+                        int nspec, float* low, long long ls, const float* mask, int nimg, const float2* tw,
+                        const float2* dtw, hipStream_t s, int step) {
+    constexpr int ct = ColTile<L>::narrow ? 4 : 8;
+    constexpr int ct1 = 2;
+    if (step == 0) {
+      hipLaunchKernelGGL((k_dct2_cols<L, ct1>), dim3(L / 2 / ct1, nimg), dim3(ColCfg<L, ct1>::T), 0, s, x, xs,
+                         xwork, ws, tw, dtw);
+    } else if (step == 1) {
+      hipLaunchKernelGGL((k_dct_rows_dec<L>), dim3(L / DctRowCfg<L>::ROWS, nspec + 1, nimg), dim3(DctRowCfg<L>::T),
+                         0, s, xwork, ws, spec, ss, nspec, low, ls, mask, tw, dtw);
+    } else {
+      hipLaunchKernelGGL((k_dct_cols_dec<L, ct>), dim3(L / 2 / ct, nspec + 1, nimg), dim3(ColCfg<L, ct>::T), 0, s,
+                         spec, ss, nspec, low, ls, tw, dtw);
+    }
+    return hipGetLastError();
+  }
+};
+
+hipError_t launch_dct_dec(int L, int step, const float* x, long long xs, float* xwork, long long ws, float2* spec,
+                          long long ss, int nspec, float* low, long long ls, const float* mask, int nimg,
+                          const float2* tw, const float2* dtw, hipStream_t s) {
+  if (step < 0 || step > 2) return hipErrorInvalidValue;
+  switch (L) {
+    case 64: return LaunchDct<64>::dec(x, xs, xwork, ws, spec, ss, nspec, low, ls, mask, nimg, tw, dtw, s, step);
+    case 128: return LaunchDct<128>::dec(x, xs, xwork, ws, spec, ss, nspec, low, ls, mask, nimg, tw, dtw, s, step);
+    case 256: return LaunchDct<256>::dec(x, xs, xwork, ws, spec, ss, nspec, low, ls, mask, nimg, tw, dtw, s, step);
+    case 512: return LaunchDct<512>::dec(x, xs, xwork, ws, spec, ss, nspec, low, ls, mask, nimg, tw, dtw, s, step);
+    case 1024: return LaunchDct<1024>::dec(x, xs, xwork, ws, spec, ss, nspec, low, ls, mask, nimg, tw, dtw, s, step);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 bool col_tile_ok(int L, int ct) {

@@ -1,0 +1,153 @@
+"""The DCT-domain decomposition front end against the pyramid front end and
+the FP64 oracle.
+
+With the half-sample symmetric extension of the a trous pyramid
+(atrousdec.m with symext.m, folded with period 2L) and the symmetric maxflat
+filters, every pyramid level is a diagonal mask on the 2D DCT-II
+coefficients.  The front end computes the lowpass and the scale spectra of all
+levels in three passes from those masks (shx_set_dec_front).  Both front ends
+must give the same coefficients to FP32 rounding, and the DCT front end's
+coefficients must meet the oracle bar of the pyramid front end (max and RMS
+relative error <= 1e-5 per band).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import nsst as o
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _gpu(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    e = a - b
+    return np.abs(e).max() / np.abs(b).max(), np.sqrt(np.mean(e * e) / np.mean(b * b))
+
+
+def _cells(C):
+    return [c.cpu().numpy() for c in C]
+
+
+@pytest.mark.parametrize("L", [64, 128, 256, 512, 1024])
+def test_fronts_agree(gpu, L):
+    from shearcuda_amd import api
+    from shearcuda_amd.synth import noisy_ellipses
+    print("case dct front vs pyramid L=%d" % L, flush=True)
+    shear = api.make_dictionary(L)
+    try:
+        _, x = noisy_ellipses(L, sigma=20.0, seed=21)
+        xg = _gpu(x)
+        assert not api.get_dec_front(shear)
+        ref = _cells(api.shear_trans_cuda(xg, "maxflat", shear))
+        api.set_dec_front(shear, True)
+        assert api.get_dec_front(shear)
+        got = _cells(api.shear_trans_cuda(xg, "maxflat", shear))
+        worst = 0.0
+        for j, (g, r) in enumerate(zip(got, ref)):
+            g3 = g.reshape(-1, L, L)
+            r3 = r.reshape(-1, L, L)
+            for k in range(r3.shape[0]):
+                mx, rms = _rel(g3[k], r3[k])
+                worst = max(worst, mx)
+                assert mx <= TOL and rms <= TOL, (L, j, k, mx, rms)
+        print("L=%d worst max-rel diff between the fronts %.2e" % (L, worst), flush=True)
+    finally:
+        shear.close()
+
+
+@pytest.mark.parametrize("L,shift", [(256, "none"), (512, "none"), (256, "center")])
+def test_dct_front_matches_oracle(gpu, L, shift):
+    from shearcuda_amd import api
+    from shearcuda_amd.synth import noisy_ellipses
+    print("case dct front vs oracle L=%d shift=%s" % (L, shift), flush=True)
+    _, x = noisy_ellipses(L, sigma=20.0, seed=3)
+    shear = api.make_dictionary(L, shift=shift)
+    try:
+        api.set_dec_front(shear, True)
+        C = _cells(api.shear_trans_cuda(_gpu(x), "maxflat", shear))
+        ref = o.nsst_dec2(x, o.make_dictionary(L, shift=shift))
+        mx, rms = _rel(C[0], ref[0])
+        print("lowpass max-rel %.3e rms-rel %.3e" % (mx, rms), flush=True)
+        assert mx <= TOL and rms <= TOL
+        for j in range(1, len(ref)):
+            for k in range(ref[j].shape[0]):
+                mx, rms = _rel(C[j][k], ref[j][k])
+                assert mx <= TOL and rms <= TOL, (j, k, mx, rms)
+    finally:
+        shear.close()
+
+
+def test_denoise_and_batch_with_dct_front(gpu):
+    """Denoise (fused row pass, reconstruction) and the batched concurrent
+    plan with the DCT front end against the pyramid front end."""
+    from shearcuda_amd import api
+    from shearcuda_amd.synth import noisy_ellipses, white_noise
+    L, n = 256, 3
+    print("case dct front denoise + batch L=%d n=%d" % (L, n), flush=True)
+    shear = api.make_dictionary(L)
+    b = api.Batch(shear, n)
+    try:
+        sc = api.nsst_scalars_cuda(shear, _gpu(white_noise(L)))
+        _, x = noisy_ellipses(L, sigma=20.0, seed=12)
+        xg = _gpu(x)
+        ref, _ = api.denoise_cuda(xg, shear, 20.0, sc)
+        api.set_dec_front(shear, True)
+        got, _ = api.denoise_cuda(xg, shear, 20.0, sc)
+        mx, rms = _rel(got.cpu().numpy(), ref.cpu().numpy())
+        print("plain denoise max-rel %.2e rms-rel %.2e" % (mx, rms), flush=True)
+        assert mx <= TOL and rms <= TOL
+        sig = np.array([10.0, 20.0, 30.0])
+        xs = _gpu(np.stack([noisy_ellipses(L, sigma=float(s), seed=50 + i)[1] for i, s in enumerate(sig)]))
+        th = api.batch_thresholds(shear, sig, sc)
+        outs, coeffs = [], []
+        for dct in (False, True):
+            api.set_dec_front(shear, dct, batch=b)
+            assert api.get_dec_front(shear, batch=b) == dct
+            out = torch.empty_like(xs)
+            c = b.alloc_coeffs()
+            b.denoise(xs, c, th, out)
+            outs.append(out.cpu().numpy())
+            coeffs.append(c.cpu().numpy())
+        mx, rms = _rel(outs[1], outs[0])
+        print("batched denoise max-rel %.2e rms-rel %.2e" % (mx, rms), flush=True)
+        assert mx <= TOL and rms <= TOL
+        # unthresholded coefficient planes (the lowpass and every band)
+        c0 = coeffs[0].reshape(n, -1, L, L)
+        c1 = coeffs[1].reshape(n, -1, L, L)
+        for i in range(n):
+            for p in range(c0.shape[1]):
+                if np.abs(c0[i, p]).max() > 0:
+                    mx, _ = _rel(c1[i, p], c0[i, p])
+                    assert mx <= TOL, (i, p, mx)
+        # the batched forward transform through the batch API
+        c2 = b.alloc_coeffs()
+        b.shear_trans(xs, c2)
+        api.set_dec_front(shear, False, batch=b)
+        c3 = b.alloc_coeffs()
+        b.shear_trans(xs, c3)
+        mx, rms = _rel(c2.cpu().numpy(), c3.cpu().numpy())
+        print("batched shear_trans max-rel %.2e rms-rel %.2e" % (mx, rms), flush=True)
+        assert mx <= TOL and rms <= TOL
+    finally:
+        b.close()
+        shear.close()
+
+
+def test_dct_front_needs_l64(gpu):
+    from shearcuda_amd import api
+    from shearcuda_amd._lib import ShxError
+    shear = api.make_dictionary(32)
+    try:
+        with pytest.raises(ShxError):
+            api.set_dec_front(shear, True)
+        assert not api.get_dec_front(shear)
+    finally:
+        shear.close()
