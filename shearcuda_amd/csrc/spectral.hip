@@ -537,14 +537,15 @@ __device__ __forceinline__ void col_fwd_A_src(LOAD load, float2* buf, const floa
 
 // First half of an inverse column transform: pass-A twiddles on the pass-B
 // layout registers q (thread (c, k2) holding Q[k2 + R2 k1]) and the LDS
-// transpose, between two barriers.
-template <int L, int CT_>
+// transpose, followed by a barrier (and preceded by one when buf may still be
+// in use: IN_USE).
+template <int L, int CT_, bool IN_USE>
 __device__ __forceinline__ void col_inv_A(float2 (&q)[Split<L>::R1], float2* buf, const PassTw<Split<L>::R1>& t) {
   using C = ColCfg<L, CT_>;
   constexpr int R1 = C::R1, R2 = C::R2, CT = C::CT;
   const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
   passA_apply<R1, +1>(q, t);
-  __syncthreads();
+  if constexpr (IN_USE) __syncthreads();
   float2* row = buf + C::Lds::ix(k2, c);
 #pragma unroll
   for (int k = 0; k < R1; ++k) row[k * (R2 / R1) * C::Lds::BLK] = q[k];
@@ -1050,13 +1051,21 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_dec(const float* _
   const int r = tid / R1, n1 = tid % R1;
   const bool validA = r < ROWS;
   const int rB = tid / R2, k2 = tid % R2;
+  // (x[4n], x[4n+2]) for n < N/2 and (x[4m+3], x[4m+1]), m = N-1-n, above:
+  // one 16-byte load per pair
   float2 v[R2];
   if (validA) {
-    const float* xr = xr0 + (size_t)r * L;
+    const float4* xr = reinterpret_cast<const float4*>(xr0 + (size_t)r * L);
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const int n = n1 + R1 * n2;
-      v[n2] = make_float2(xr[dct_perm(2 * n, L)], xr[dct_perm(2 * n + 1, L)]);
+      if (n2 < R2 / 2) {
+        const float4 f = xr[n];
+        v[n2] = make_float2(f.x, f.z);
+      } else {
+        const float4 f = xr[N - 1 - n];
+        v[n2] = make_float2(f.w, f.y);
+      }
     }
   }
   const auto ftw = row_fwd_tw<L, R1, R2>(n1, k2, tw);
@@ -1143,6 +1152,7 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct_cols_dec(float2* __r
   using C = ColCfg<L, CT_>;
   constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
   __shared__ float2 buf[C::LDS];
+  __shared__ float2 ybuf[C::LDS];  // the plane's columns in natural order
   const int c0 = blockIdx.x * CT, p = blockIdx.y, b = blockIdx.z;
   float2* base = (p == nspec ? reinterpret_cast<float2*>(low + (size_t)b * ls)
                              : spec + (size_t)b * ss + (size_t)p * L * NH) + c0;
@@ -1164,15 +1174,16 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct_cols_dec(float2* __r
     const float2 e = ctw_mul<4 * L / R2, -1>(w0, k1);
     q[k1] = cmulc(make_float2(xk[k1].x + xm[k1].y, xk[k1].y - xm[k1].x), e);
   }
-  col_inv_A<L, CT_>(q, buf, ivt);
+  col_inv_A<L, CT_, false>(q, buf, ivt);
   if (p == nspec) {
     col_inv_B_to<L, CT_, false>(buf, [&](int row, int cc, float2 val) { base[(size_t)dct_perm(row, L) * NH + cc] = val; });
     return;
   }
-  // y[perm(n)] = v[n] into LDS (natural rows), then the forward FFT from there
-  col_inv_B_to<L, CT_, true>(buf, [&](int row, int cc, float2 val) { buf[C::Lds::ix(dct_perm(row, L), cc)] = val; });
+  // y[perm(n)] = v[n] into the second tile (natural rows), then the forward
+  // FFT from there (its pass A writes buf, whose reads the barrier has ended)
+  col_inv_B_to<L, CT_, false>(buf, [&](int row, int cc, float2 val) { ybuf[C::Lds::ix(dct_perm(row, L), cc)] = val; });
   __syncthreads();
-  col_fwd_A_src<L, CT_, true>([&](int row, int cc) { return buf[C::Lds::ix(row, cc)]; }, buf, tw);
+  col_fwd_A_src<L, CT_, false>([&](int row, int cc) { return ybuf[C::Lds::ix(row, cc)]; }, buf, tw);
   __syncthreads();
   float2 u[R1];
   col_fwd_B<L, CT_>(u, buf);

@@ -86,8 +86,10 @@ def test_dct_front_matches_oracle(gpu, L, shift):
 
 
 def test_denoise_and_batch_with_dct_front(gpu):
-    """Denoise (fused row pass, reconstruction) and the batched concurrent
-    plan with the DCT front end against the pyramid front end."""
+    """Denoise with the DCT front end: the output is the reconstruction of its
+    own coefficients under its own keep decisions (|c| > th), the coefficients
+    agree with the pyramid front end's, and the batched concurrent plan
+    reproduces the plain plan bit for bit (same kernels per image)."""
     from shearcuda_amd import api
     from shearcuda_amd.synth import noisy_ellipses, white_noise
     L, n = 256, 3
@@ -98,35 +100,41 @@ def test_denoise_and_batch_with_dct_front(gpu):
         sc = api.nsst_scalars_cuda(shear, _gpu(white_noise(L)))
         _, x = noisy_ellipses(L, sigma=20.0, seed=12)
         xg = _gpu(x)
-        ref, _ = api.denoise_cuda(xg, shear, 20.0, sc)
+        _, cref = api.denoise_cuda(xg, shear, 20.0, sc)
+        cref = _cells(cref)
         api.set_dec_front(shear, True)
-        got, _ = api.denoise_cuda(xg, shear, 20.0, sc)
-        mx, rms = _rel(got.cpu().numpy(), ref.cpu().numpy())
-        print("plain denoise max-rel %.2e rms-rel %.2e" % (mx, rms), flush=True)
+        got, C = api.denoise_cuda(xg, shear, 20.0, sc)
+        C = _cells(C)
+        for j, (g, r) in enumerate(zip(C, cref)):
+            mx, rms = _rel(g, r)
+            assert mx <= TOL and rms <= TOL, (j, mx, rms)
+        th = api.band_thresholds(shear, 20.0, sc).astype(np.float32)
+        masked, k = [], 0
+        for c in C:
+            c3 = c.reshape(-1, L, L)
+            m = np.stack([np.where(np.abs(c3[i]) > th[k + i], c3[i], 0.0) for i in range(c3.shape[0])])
+            k += c3.shape[0]
+            masked.append(_gpu(m.reshape(c.shape)))
+        api.set_dec_front(shear, False)
+        rec = api.inverse_shear_cuda(masked, "maxflat", shear).cpu().numpy()
+        mx, rms = _rel(got.cpu().numpy(), rec)
+        print("plain denoise vs masked reconstruction max-rel %.2e rms-rel %.2e" % (mx, rms), flush=True)
         assert mx <= TOL and rms <= TOL
+        api.set_dec_front(shear, True)
         sig = np.array([10.0, 20.0, 30.0])
         xs = _gpu(np.stack([noisy_ellipses(L, sigma=float(s), seed=50 + i)[1] for i, s in enumerate(sig)]))
-        th = api.batch_thresholds(shear, sig, sc)
-        outs, coeffs = [], []
-        for dct in (False, True):
-            api.set_dec_front(shear, dct, batch=b)
-            assert api.get_dec_front(shear, batch=b) == dct
-            out = torch.empty_like(xs)
-            c = b.alloc_coeffs()
-            b.denoise(xs, c, th, out)
-            outs.append(out.cpu().numpy())
-            coeffs.append(c.cpu().numpy())
-        mx, rms = _rel(outs[1], outs[0])
-        print("batched denoise max-rel %.2e rms-rel %.2e" % (mx, rms), flush=True)
-        assert mx <= TOL and rms <= TOL
-        # unthresholded coefficient planes (the lowpass and every band)
-        c0 = coeffs[0].reshape(n, -1, L, L)
-        c1 = coeffs[1].reshape(n, -1, L, L)
+        thb = api.batch_thresholds(shear, sig, sc)
+        api.set_dec_front(shear, True, batch=b)
+        assert api.get_dec_front(shear, batch=b)
+        out = torch.empty_like(xs)
+        cb = b.alloc_coeffs()
+        b.denoise(xs, cb, thb, out)
+        torch.cuda.synchronize()
         for i in range(n):
-            for p in range(c0.shape[1]):
-                if np.abs(c0[i, p]).max() > 0:
-                    mx, _ = _rel(c1[i, p], c0[i, p])
-                    assert mx <= TOL, (i, p, mx)
+            pd, Ct = api.denoise_cuda(xs[i].contiguous(), shear, float(sig[i]), sc)
+            assert np.array_equal(out[i].cpu().numpy(), pd.cpu().numpy()), i
+            flat = torch.cat([Ct[0][None]] + list(Ct[1:])).cpu().numpy()
+            assert np.array_equal(cb[i].cpu().numpy(), flat), i
         # the batched forward transform through the batch API
         c2 = b.alloc_coeffs()
         b.shear_trans(xs, c2)
