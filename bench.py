@@ -76,13 +76,13 @@ def plain_setup(L, sigma, seed=0):
     return shear, xg, sc
 
 
-def profile_plain(shear, xg, sc, sigma, reps):
+def profile_plain(shear, xg, sc, sigma, reps, coeffs=False):
     """Per-kernel mean durations of one plain denoise step (HIP events)."""
     import numpy as np
     import torch
 
     from shearcuda_amd._lib import check, lib, ptr_array
-    Ct = shear.alloc_coeffs()
+    cells = ptr_array([c.data_ptr() for c in shear.alloc_coeffs()]) if coeffs else None
     out = torch.empty_like(xg)
     MAXS, NL = 64, 32
     stage_ms = (ctypes.c_double * MAXS)()
@@ -91,8 +91,7 @@ def profile_plain(shear, xg, sc, sigma, reps):
     nst = ctypes.c_int()
     scv = np.concatenate([[float(sc[0])]] + [np.asarray(s, dtype=np.float64) for s in sc[1:]])
     T = np.array([0.0, 3.0, 4.0])
-    check(lib.shx_denoise_profile(shear.handle, ctypes.c_void_p(xg.data_ptr()), ptr_array([c.data_ptr() for c in Ct]),
-                                  sigma, T.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+    check(lib.shx_denoise_profile(shear.handle, ctypes.c_void_p(xg.data_ptr()), cells, sigma, T.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                   scv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.c_void_p(out.data_ptr()),
                                   reps, MAXS, stage_ms, stage_b, names, NL, ctypes.byref(nst),
                                   ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
@@ -105,15 +104,21 @@ def profile_plain(shear, xg, sc, sigma, reps):
 
 
 def bench_plain(L, sigma, steps, warmup):
+    """ms per step of the plain denoise graph returning the image only (the
+    timed step), ms per step of the same step also writing the unthresholded
+    coefficients, the per-kernel profile of the timed step and the schedule."""
     from shearcuda_amd import api
     shear, xg, sc = plain_setup(L, sigma)
     sched = api.autotune(shear)
-    g, Ct, out = api.denoise_graph(xg, shear, sigma, sc)
+    g, _, out = api.denoise_graph(xg, shear, sigma, sc, coeffs=False)
     el = _timed(g.launch, steps, warmup)
+    gc, Ct, outc = api.denoise_graph(xg, shear, sigma, sc)
+    elc = _timed(gc.launch, steps, warmup)
     stages = profile_plain(shear, xg, sc, sigma, max(20, steps))
     g.close()
+    gc.close()
     shear.close()
-    return el * 1e3 / steps, stages, sched
+    return el * 1e3 / steps, elc * 1e3 / steps, stages, sched
 
 
 # mixed-size batch: 48 x 256^2, 32 x 512^2, 16 x 1024^2, sigma ~ U[5, 40]
@@ -138,28 +143,37 @@ def bench_batched(steps, warmup, mix=BATCH_MIX):
         b = api.Batch(shear, n)
         sched = api.autotune(shear, b)
         plans.append(dict(sched=sched, L=L, n=n, shear=shear, sc=sc, sig=sig, b=b, x=torch.from_numpy(xs).cuda(),
-                          c=b.alloc_coeffs(), th=api.batch_thresholds(shear, sig, sc),
-                          out=torch.empty((n, L, L), device="cuda")))
+                          th=api.batch_thresholds(shear, sig, sc), out=torch.empty((n, L, L), device="cuda")))
         npx += n * L * L
-    g = api.capture(lambda: [p["b"].denoise(p["x"], p["c"], p["th"], p["out"]) for p in plans])
+    # timed: the denoised images only; beside it the same launch also writing
+    # every image's unthresholded coefficients
+    g = api.capture(lambda: [p["b"].denoise(p["x"], None, p["th"], p["out"]) for p in plans])
     el = _timed(g.launch, steps, warmup)
     batched = npx * steps / el / 1e6
+    g.close()
+    for p in plans:
+        p["c"] = p["b"].alloc_coeffs()
+    gcf = api.capture(lambda: [p["b"].denoise(p["x"], p["c"], p["th"], p["out"]) for p in plans])
+    elc = _timed(gcf.launch, steps, warmup)
+    gcf.close()
+    for p in plans:
+        del p["c"]
     # the same images one at a time through the plain form (one graph each)
     singles = []
     for p in plans:
         for i in range(p["n"]):
-            gi, _, _ = api.denoise_graph(p["x"][i], p["shear"], float(p["sig"][i]), p["sc"])
+            gi, _, _ = api.denoise_graph(p["x"][i], p["shear"], float(p["sig"][i]), p["sc"], coeffs=False)
             singles.append(gi)
     el1 = _timed(lambda: [gi.launch() for gi in singles], max(2, steps // 2), 1)
     plain = npx * max(2, steps // 2) / el1 / 1e6
     for gi in singles:
         gi.close()
-    g.close()
     for p in plans:
         p["b"].close()
         p["shear"].close()
     return {"images": sum(n for _, n in mix), "pixels": npx, "mix": [[L, n] for L, n in mix],
             "ms_per_launch": round(el * 1e3 / steps, 4), "Mpixel_s": round(batched, 2),
+            "Mpixel_s_with_coeffs": round(npx * steps / elc / 1e6, 2),
             "plain_image_by_image_Mpixel_s": round(plain, 2), "speedup_vs_plain": round(batched / plain, 3),
             "path_roofline_frac": round(400.0 * npx / (el / steps) / 1e9 / HBM_PEAK_GBS, 4),
             "schedules": {str(p["L"]): p["sched"] for p in plans}}
@@ -355,7 +369,7 @@ def main():
         print(json.dumps(r), flush=True)
         return
     L, sigma = args.size, 20.0
-    ms, stages, sched = bench_plain(L, sigma, args.steps, args.warmup)
+    ms, ms_coeffs, stages, sched = bench_plain(L, sigma, args.steps, args.warmup)
     value = L * L / (ms * 1e-3) / 1e6
     dom = max(stages, key=lambda s: s["ms"])
     roof = {"kernel": dom["name"], "bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS,
@@ -370,16 +384,20 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plain 2D NSST hard-threshold denoise (dec+thresh+rec), one %dx%d FP32 image per step, "
                                "maxflat 4 levels, directions [8,8,16,16], Meyer sizes [32,32,16,16], T=[0,3,4], "
-                               "sigma=20, one hipGraph launch per step" % (L, L),
+                               "sigma=20, one hipGraph launch per step returning the denoised image" % (L, L),
                    "L": L, "images_per_step": 1, "form": "plain", "schedule": sched},
+        "with_coeffs": {"ms_per_step": round(ms_coeffs, 5),
+                        "Mpixel_s": round(L * L / (ms_coeffs * 1e-3) / 1e6, 2),
+                        "note": "the same step also writing the 48 unthresholded band coefficient planes"},
         "roofline": roof,
         "path_roofline": {"bytes_per_step": path_bytes, "achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
         "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()} for s in stages],
     }
     if L == 512 and not args.no_plain_1024:
-        ms1k, _, _ = bench_plain(1024, sigma, max(20, args.steps // 4), 5)
+        ms1k, ms1kc, _, _ = bench_plain(1024, sigma, max(20, args.steps // 4), 5)
         res["plain_1024"] = {"ms_per_image": round(ms1k, 5), "Mpixel_s": round(1024 * 1024 / (ms1k * 1e-3) / 1e6, 2),
+                             "Mpixel_s_with_coeffs": round(1024 * 1024 / (ms1kc * 1e-3) / 1e6, 2),
                              "path_roofline_frac": round(400.0 * 1024 * 1024 / (ms1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if not args.no_batched:
         res["batched"] = bench_batched(max(5, args.steps // 20), 2)

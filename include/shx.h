@@ -83,7 +83,10 @@ int shx_inverse_shear_thresh(shx_dict* dict, const float* const* coeff, const fl
  * lowpass, [1] scales 1..S-1, [2] finest; shearlet_toolbox_1/nsst_HT.m:32-47)
  * fused into the inverse.  scalars: 1 + sum(ndir) per-band noise scalars
  * (lowpass first).  coeff: caller-owned band storage (receives the
- * unthresholded coefficients). */
+ * unthresholded coefficients), or NULL when the caller needs only the image:
+ * then the band coefficients are never written to memory (the reference's
+ * demo_denoise/demoImageDenoise/mainwindow.cpp:223-233 drops them after the
+ * reconstruction too). */
 int shx_denoise(shx_dict* dict, const float* x, float* const* coeff, double sigma, const double* Tscalars,
                 const double* scalars, float* out, void* stream);
 
@@ -115,6 +118,7 @@ size_t shx_batch_bytes(const shx_batch* batch);
 int shx_batch_shear_trans(shx_batch* batch, const float* x, float* coeff, void* stream);
 /* th_dev may be NULL (plain inverse). */
 int shx_batch_inverse_shear(shx_batch* batch, const float* coeff, const float* th_dev, float* out, void* stream);
+/* coeff may be NULL (no coefficients returned or written), as in shx_denoise. */
 int shx_batch_denoise(shx_batch* batch, const float* x, float* coeff, const float* th_dev, float* out,
                       void* stream);
 

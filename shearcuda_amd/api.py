@@ -218,19 +218,23 @@ def band_thresholds(shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS):
     return np.array(th, dtype=np.float64)
 
 
-def denoise_cuda(x, shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS, Ct=None, out=None):
+def denoise_cuda(x, shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS, Ct=None, out=None, coeffs=True):
     """Plain NSST hard-threshold denoise: shear_trans + nsst_HT + inverse with
-    the threshold fused into the inverse.  Returns (x_hat, Ct)."""
+    the threshold fused into the inverse.  Returns (x_hat, Ct); with
+    coeffs=False the coefficients are neither returned nor written
+    (x_hat, None)."""
     _require_gpu(x)
-    if Ct is None:
+    if not coeffs:
+        Ct = None
+    elif Ct is None:
         Ct = shear.alloc_coeffs()
     if out is None:
         out = torch.empty_like(x)
     sc = np.concatenate([[float(scalars[0])]] + [np.asarray(s, dtype=np.float64) for s in scalars[1:]])
     sc = np.ascontiguousarray(sc, dtype=np.float64)
     T = np.ascontiguousarray(np.asarray(Tscalars, dtype=np.float64))
-    check(lib.shx_denoise(shear.handle, ctypes.c_void_p(x.data_ptr()), ptr_array([c.data_ptr() for c in Ct]),
-                          float(sigma), T.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+    cells = ptr_array([c.data_ptr() for c in Ct]) if Ct is not None else None
+    check(lib.shx_denoise(shear.handle, ctypes.c_void_p(x.data_ptr()), cells, float(sigma), T.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                           sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.c_void_p(out.data_ptr()),
                           _stream()))
     return out, Ct
@@ -468,14 +472,16 @@ def capture(fn, keep=()):
     return Graph(h, keep)
 
 
-def denoise_graph(x, shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS, Ct=None, out=None):
-    """Graph of one plain denoise step (replay with .launch())."""
-    if Ct is None:
+def denoise_graph(x, shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS, Ct=None, out=None, coeffs=True):
+    """Graph of one plain denoise step (replay with .launch()); coeffs=False:
+    the image only (no coefficients written)."""
+    if Ct is None and coeffs:
         Ct = shear.alloc_coeffs()
     if out is None:
         out = torch.empty_like(x)
-    g = capture(lambda: denoise_cuda(x, shear, sigma, scalars, Tscalars, Ct=Ct, out=out), keep=(x, Ct, out))
-    return g, Ct, out
+    g = capture(lambda: denoise_cuda(x, shear, sigma, scalars, Tscalars, Ct=Ct, out=out, coeffs=coeffs),
+                keep=(x, Ct, out))
+    return g, (Ct if coeffs else None), out
 
 
 # ---- batched form -----------------------------------------------------------------
@@ -539,11 +545,16 @@ class Batch:
         return coeff_bar
 
     def denoise(self, x, coeff, th, out):
-        self._chk(x, coeff, out)
+        """coeff None: the coefficients are neither returned nor written."""
+        if coeff is None:
+            self._chk(x, out)
+        else:
+            self._chk(x, coeff, out)
         _require_gpu(th, "th")
         if tuple(th.shape) != (self.nimg, self.shear.nbands):
             raise ValueError("thresholds must be (%d, %d)" % (self.nimg, self.shear.nbands))
-        check(lib.shx_batch_denoise(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(coeff.data_ptr()),
+        cptr = ctypes.c_void_p(coeff.data_ptr()) if coeff is not None else None
+        check(lib.shx_batch_denoise(self._h, ctypes.c_void_p(x.data_ptr()), cptr,
                                     ctypes.c_void_p(th.data_ptr()), ctypes.c_void_p(out.data_ptr()), _stream()))
         return out
 

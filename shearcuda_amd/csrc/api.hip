@@ -45,7 +45,8 @@ bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
 // Per-image intermediate buffers of the NSST path.
 struct Work {
-  float* planes = nullptr;  // per image: S pyramid bands (ypyr) + 2 temporaries
+  float* planes = nullptr;  // per image: S pyramid bands (ypyr) + 2 temporaries + the lowpass of a denoise
+                            // whose caller takes no coefficients
   float2* spec = nullptr;   // per image: S packed spectra (Y1 / rec input of the row pass)
   float2* band = nullptr;   // per image: nbands packed spectra (T / U)
   float2* q = nullptr;      // per image: ngroups packed partial reconstructions
@@ -308,7 +309,7 @@ void set_group(const shx_dict* d, Work& w, int g) {
 int alloc_work(const shx_dict* d, int nimg, Work& w) {
   const long long pl = (long long)d->L * d->L;
   set_group(d, w, nimg == 1 ? 4 : 4);
-  w.s_planes = pl * (d->S + 2);
+  w.s_planes = pl * (d->S + 3);
   w.s_spec = pl * d->S;
   w.s_band = pl * d->nbands;
   w.s_q = pl * max_groups(d);
@@ -372,6 +373,17 @@ Coeffs cell_coeffs(const shx_dict* d, float* const* coeff) {
 }
 
 // contiguous stacks: image i at base + i*(1+nbands)*L*L, lowpass first
+// A denoise whose caller takes no coefficients: the lowpass (which the
+// reconstruction reads) lives in the workspace and the band coefficients are
+// not written (empty band set).
+Coeffs work_coeffs(const shx_dict* d, const Work& w) {
+  Coeffs c{};
+  c.low = w.planes + (size_t)(d->S + 2) * d->plane();
+  c.low_stride = w.s_planes;
+  c.bands.ncells = 0;
+  return c;
+}
+
 Coeffs stack_coeffs(const shx_dict* d, float* base) {
   const long long pl = (long long)d->L * d->L, st = pl * (1 + d->nbands);
   Coeffs c{};
@@ -575,8 +587,9 @@ void stages_denoise(shx_dict* d, const Work& w, int nimg, const float* x, long l
   PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
   PlaneSet cb = c.bands;
   shx::ThreshSpec th = band_thresh(d, t);
-  // reads T, writes the coefficients and U (in place of T, row by row)
-  st.push_back({"rows_c2r_r2c_bands", 3.0 * nb * P,
+  // reads T, writes the coefficients (when the caller takes them) and U (in
+  // place of T, row by row)
+  st.push_back({"rows_c2r_r2c_bands", (cb.ncells ? 3.0 : 2.0) * nb * P,
                 [=] { return shx::launch_rows_c2r_r2c(L, bandset, cb, bandset, nb, nimg, th, d->tw, s); }});
   stages_spec_rec_tail(d, w, nimg, s, st);
   stages_pyr_rec(d, w, nimg, c, t, out, os, s, st);
@@ -643,7 +656,7 @@ void stages_denoise_dag(shx_dict* d, Work& w, int nimg, const float* x, long lon
     PlaneSet ypl = shx::single_cell(w.planes + (size_t)j * pl, 1, w.s_planes);
     PlaneSet spc = shx::single_cell(reinterpret_cast<float*>(w.spec) + (size_t)j * pl, 1, w.s_spec);
     PlaneSet bnd = shx::single_cell(reinterpret_cast<float*>(w.band) + (size_t)f * pl, nb, w.s_band);
-    PlaneSet cb = sub_planes(c.bands, f, nb, pl);
+    PlaneSet cb = c.bands.ncells ? sub_planes(c.bands, f, nb, pl) : c.bands;
     PlaneSet qset = shx::single_cell(reinterpret_cast<float*>(w.q) + (size_t)g0 * pl, ng, w.s_q);
     const float2* Y1 = w.spec + (size_t)j * L * NH;
     float2* T = w.band + (size_t)f * L * NH;
@@ -670,7 +683,7 @@ void stages_denoise_dag(shx_dict* d, Work& w, int nimg, const float* x, long lon
     b.stream = sj;
     if (w.dct_front) b.waits = {made[j]};
     st.push_back(b);
-    Stage cc{"rows_c2r_r2c" + sfx, 3.0 * nb * P,
+    Stage cc{"rows_c2r_r2c" + sfx, (cb.ncells ? 3.0 : 2.0) * nb * P,
              [=] { return shx::launch_rows_c2r_r2c(L, bnd, cb, bnd, nb, nimg, th, d->tw, sj); }};
     cc.stream = sj;
     st.push_back(cc);
@@ -1177,7 +1190,8 @@ int shx_denoise(shx_dict* d, const float* x, float* const* coeff, double sigma, 
                 float* out, void* stream) {
   if (int r = check_dict(d)) return r;
   if (!x || !out || !T || !sc) return fail(SHX_ERR_ARG, "null input, output or threshold scalars");
-  if (int r = check_cells(d, (const void* const*)coeff)) return r;
+  if (coeff)
+    if (int r = check_cells(d, (const void* const*)coeff)) return r;
   std::vector<float> th = nsst_ht_thresholds(d, sigma, T, sc);
   Thr t;
   t.mode = 1;
@@ -1185,7 +1199,8 @@ int shx_denoise(shx_dict* d, const float* x, float* const* coeff, double sigma, 
   // one image: the kernels are short, so the single-stream chain beats the
   // concurrent form (whose cross-stream edges cost more than the overlap wins)
   std::vector<Stage> st;
-  stages_denoise(d, d->w, 1, x, 0, cell_coeffs(d, coeff), t, out, 0, (hipStream_t)stream, st);
+  stages_denoise(d, d->w, 1, x, 0, coeff ? cell_coeffs(d, coeff) : work_coeffs(d, d->w), t, out, 0,
+                 (hipStream_t)stream, st);
   return run_stages(st);
 }
 
@@ -1195,13 +1210,15 @@ int shx_denoise_profile(shx_dict* d, const float* x, float* const* coeff, double
   if (int r = check_dict(d)) return r;
   if (!x || !out || !T || !sc || !stage_ms || !nstages || reps < 1)
     return fail(SHX_ERR_ARG, "invalid profiling arguments");
-  if (int r = check_cells(d, (const void* const*)coeff)) return r;
+  if (coeff)
+    if (int r = check_cells(d, (const void* const*)coeff)) return r;
   std::vector<float> th = nsst_ht_thresholds(d, sigma, T, sc);
   Thr t;
   t.mode = 1;
   t.host = th.data();
   std::vector<Stage> st;
-  stages_denoise(d, d->w, 1, x, 0, cell_coeffs(d, coeff), t, out, 0, (hipStream_t)stream, st);
+  stages_denoise(d, d->w, 1, x, 0, coeff ? cell_coeffs(d, coeff) : work_coeffs(d, d->w), t, out, 0,
+                 (hipStream_t)stream, st);
   return profile_stages(st, (hipStream_t)stream, reps, max_stages, stage_ms, stage_bytes, names, name_len, nstages);
 }
 
@@ -1261,7 +1278,7 @@ int shx_batch_inverse_shear(shx_batch* b, const float* coeff, const float* th_de
 }
 
 int shx_batch_denoise(shx_batch* b, const float* x, float* coeff, const float* th_dev, float* out, void* stream) {
-  if (int r = batch_args(b, x, coeff, out)) return r;
+  if (int r = batch_args(b, x, out, out)) return r;
   if (!th_dev) return fail(SHX_ERR_ARG, "null threshold table");
   shx_dict* d = b->d;
   Thr t;
@@ -1269,7 +1286,8 @@ int shx_batch_denoise(shx_batch* b, const float* x, float* coeff, const float* t
   t.dev = th_dev;
   std::vector<Stage> st;
   const long long pl = (long long)d->L * d->L;
-  stages_denoise_dag(d, b->w, b->nimg, x, pl, stack_coeffs(d, coeff), t, out, pl, (hipStream_t)stream, st);
+  stages_denoise_dag(d, b->w, b->nimg, x, pl, coeff ? stack_coeffs(d, coeff) : work_coeffs(d, b->w), t, out, pl,
+                     (hipStream_t)stream, st);
   return run_stage_dag(st, b->w);
 }
 

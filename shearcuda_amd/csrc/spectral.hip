@@ -328,7 +328,9 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
   const int row0 = blockIdx.x * C::ROWS;
   const size_t pe = (size_t)L * L;
   const float2* src = reinterpret_cast<const float2*>(plane_ptr(Tin, p, pe, b)) + (size_t)row0 * N;
-  float2* cdst = reinterpret_cast<float2*>(plane_ptr(band, p, pe, b)) + (size_t)row0 * N;
+  // an empty band set: the caller takes no coefficients
+  const bool store_c = band.ncells > 0;
+  float2* cdst = store_c ? reinterpret_cast<float2*>(plane_ptr(band, p, pe, b)) + (size_t)row0 * N : nullptr;
   float2* udst = reinterpret_cast<float2*>(plane_ptr(Uout, p, pe, b)) + (size_t)row0 * N;
   const bool use_th = th.mode != 0;
   const float t = use_th ? thresh_of(th, p, b) : 0.0f;
@@ -344,7 +346,7 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) {
     const float2 c = u[k1];
-    cdst[(size_t)rB * N + k2 + R2 * k1] = c;
+    if (store_c) cdst[(size_t)rB * N + k2 + R2 * k1] = c;
     u[k1] = use_th ? make_float2(hard_keep(c.x, t), hard_keep(c.y, t)) : c;
   }
   __syncthreads();  // pass-B reads of buf are complete

@@ -245,7 +245,12 @@ def test_batch_matches_plain(api, L, nimg):
         assert np.array_equal(out[i].cpu().numpy(), pd.cpu().numpy()), i
         flat = torch.cat([Ct[0][None]] + list(Ct[1:])).cpu().numpy()
         assert np.array_equal(coeff[i].cpu().numpy(), flat), i
-        assert np.array_equal(coeff2[i].cpu().numpy(), flat), i
+        # the forward transform against the plain forward transform (the same
+        # kernels; the denoise computes the coefficients in its fused row pass)
+        Cf = api.shear_trans_cuda(xg[i].contiguous(), "maxflat", shear)
+        flat_f = torch.cat([Cf[0][None]] + list(Cf[1:])).cpu().numpy()
+        assert np.array_equal(coeff2[i].cpu().numpy(), flat_f), i
+        assert_close(flat_f, flat, tol=1e-6, what="forward vs denoise coefficients %d" % i)
         assert_close(rt[i].cpu().numpy(), xs[i], what="batch round trip %d" % i)
     b.close()
     shear.close()
@@ -275,3 +280,42 @@ def test_mixed_size_batches_in_one_graph(api):
     for p in plans:
         p[1].close()
         p[0].close()
+
+
+@pytest.mark.parametrize("dct", [False, True])
+def test_denoise_without_coefficients(api, dct):
+    """A denoise whose caller takes no coefficients (coeff NULL: the band
+    coefficients are never written) returns the same image bit for bit, plain
+    and batched, with either decomposition front end."""
+    L, n = 256, 2
+    print("case denoise without coefficients dct=%s" % dct, flush=True)
+    shear = api.make_dictionary(L)
+    b = api.Batch(shear, n)
+    try:
+        api.set_dec_front(shear, dct)
+        api.set_dec_front(shear, dct, batch=b)
+        sc = api.nsst_scalars_cuda(shear, to_gpu(np.random.default_rng(6).standard_normal((L, L))))
+        _, x = image(L, seed=61)
+        xg = to_gpu(x)
+        ref, _ = api.denoise_cuda(xg, shear, 20.0, sc)
+        got, C = api.denoise_cuda(xg, shear, 20.0, sc, coeffs=False)
+        assert C is None
+        assert np.array_equal(got.cpu().numpy(), ref.cpu().numpy())
+        g, _, gout = api.denoise_graph(xg, shear, 20.0, sc, coeffs=False)
+        for _ in range(3):
+            gout.zero_()
+            g.launch()
+            torch.cuda.synchronize()
+            assert np.array_equal(gout.cpu().numpy(), ref.cpu().numpy())
+        g.close()
+        sig = [15.0, 25.0]
+        xs = to_gpu(np.stack([image(L, seed=62 + i, sigma=s)[1] for i, s in enumerate(sig)]))
+        th = api.batch_thresholds(shear, sig, sc)
+        o1, o2 = torch.empty_like(xs), torch.empty_like(xs)
+        b.denoise(xs, b.alloc_coeffs(), th, o1)
+        b.denoise(xs, None, th, o2)
+        torch.cuda.synchronize()
+        assert np.array_equal(o1.cpu().numpy(), o2.cpu().numpy())
+    finally:
+        b.close()
+        shear.close()
