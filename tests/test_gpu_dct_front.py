@@ -36,12 +36,17 @@ def _cells(C):
     return [c.cpu().numpy() for c in C]
 
 
-@pytest.mark.parametrize("L", [64, 128, 256, 512, 1024])
-def test_fronts_agree(gpu, L):
+@pytest.mark.parametrize("L,dcomp,dsize", [
+    (64, None, None), (128, None, None), (256, None, None), (512, None, None), (1024, None, None),
+    (256, (2, 3), (32, 16)),                       # two levels
+    (256, (3, 3, 4, 4, 4), (32, 32, 16, 16, 16)),  # five levels (dilation 16)
+    (128, (1,), (16,)),                            # one level
+])
+def test_fronts_agree(gpu, L, dcomp, dsize):
     from shearcuda_amd import api
     from shearcuda_amd.synth import noisy_ellipses
-    print("case dct front vs pyramid L=%d" % L, flush=True)
-    shear = api.make_dictionary(L)
+    print("case dct front vs pyramid L=%d dcomp=%s" % (L, dcomp), flush=True)
+    shear = api.make_dictionary(L) if dcomp is None else api.make_dictionary(L, dcomp=dcomp, dsize=dsize)
     try:
         _, x = noisy_ellipses(L, sigma=20.0, seed=21)
         xg = _gpu(x)
