@@ -140,6 +140,25 @@ __device__ __forceinline__ void tile_store(float* __restrict__ tile, const float
 // adjoint (EXT), E = L + 2*HALO*m per side, in sub-lattice rows of `m`.
 __device__ __forceinline__ int out_side(int L, int m, bool ext) { return ext ? L + 2 * HALO * m : L; }
 
+// XCD-aware block order of the level kernels.  Workgroups go to the 8 XCDs in
+// turn; the flat id is remapped so that each XCD takes a contiguous range of
+// (sub-lattice phase fastest, then tile column, tile row, image): the m*m
+// phases of one tile read the same 128-byte lines (a line interleaves m
+// phases of a row), and neighbouring tiles share their halo lines, so they
+// hit one L2 instead of fetching those lines once per XCD.
+__device__ __forceinline__ void pyr_block(int mm, int& bx, int& by, int& ph, int& img) {
+  const int nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
+  const int total = nx * ny * nz;
+  int lid = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  if (total % 8 == 0) lid = (lid % 8) * (total / 8) + lid / 8;
+  ph = __builtin_amdgcn_readfirstlane(lid % mm);
+  int r = lid / mm;
+  bx = __builtin_amdgcn_readfirstlane(r % nx);
+  r /= nx;
+  by = __builtin_amdgcn_readfirstlane(r % ny);
+  img = __builtin_amdgcn_readfirstlane(r / ny);
+}
+
 // ---------------------------------------------------------------------------
 // One input, two outputs with pair P: ya = F1 (x) x, yb = F2 (x) x.
 //   forward decomposition level: P = 0 (h0, h1)
@@ -154,14 +173,15 @@ __global__ void __launch_bounds__(T) k_pyr_one2two(const float* __restrict__ x, 
   constexpr int WP = W | 1;  // odd row stride for the intermediates
   __shared__ float tile[TileCfg<TH, TW, T>::PADDED];
   __shared__ float tv[PRANK][TH * WP];
-  const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
+  int bx, by, ph, img;
+  pyr_block(m * m, bx, by, ph, img);
   const int pr = ph / m, pc = ph - pr * m;
   x += (size_t)img * st3.a;
   ya += (size_t)img * st3.b;
   yb += (size_t)img * st3.c;
   const int E = out_side(L, m, ADJ);
   const int nout = E / m;
-  const int R0 = blockIdx.y * TH, C0 = blockIdx.x * TW;
+  const int R0 = by * TH, C0 = bx * TW;
   const int shift = ADJ ? HALO : 0;  // extended row R' sits on input row R' - HALO
   float v[TileCfg<TH, TW, T>::NIT];
   tile_fetch<TH, TW, T, ADJ>(v, x, L, m, pr, pc, R0 - shift, C0 - shift);
@@ -232,7 +252,8 @@ __global__ void __launch_bounds__(T) k_pyr_two2one(const float* __restrict__ xa,
   constexpr int WP = W | 1;
   __shared__ float tile[2][TileCfg<TH, TW, T>::PADDED];
   __shared__ float tv[PRANK][TH * WP];
-  const int ph = blockIdx.z % (m * m), img = blockIdx.z / (m * m);
+  int bx, by, ph, img;
+  pyr_block(m * m, bx, by, ph, img);
   const int pr = ph / m, pc = ph - pr * m;
   xa += (size_t)img * st3.a;
   xb += (size_t)img * st3.b;
@@ -240,7 +261,7 @@ __global__ void __launch_bounds__(T) k_pyr_two2one(const float* __restrict__ xa,
   const float th = lt.mode == 0 ? -1.0f : (lt.mode == 1 ? lt.v : lt.dev[(size_t)img * lt.stride]);
   const int E = out_side(L, m, ADJ);
   const int nout = E / m;
-  const int R0 = blockIdx.y * TH, C0 = blockIdx.x * TW;
+  const int R0 = by * TH, C0 = bx * TW;
   const int shift = ADJ ? HALO : 0;
   float va[TileCfg<TH, TW, T>::NIT], vb[TileCfg<TH, TW, T>::NIT];
   tile_fetch<TH, TW, T, ADJ>(va, xa, L, m, pr, pc, R0 - shift, C0 - shift);

@@ -992,7 +992,8 @@ hipError_t launch_cols_fwd(int L, float2* Y1, long long ys, int nplanes, int nim
 __device__ __forceinline__ int dct_perm(int n, int L) { return n < L / 2 ? 2 * n : 2 * L - 1 - 2 * n; }
 
 // Columns of real planes (pairs of columns as one complex column): X = D2.
-// grid: (L/2 / CT, nimg).
+// grid: (L/2 / CT, 1, nimg), walked in the XCD-aware order (neighbouring
+// narrow tiles share 128-byte lines).
 template <int L, int CT_>
 __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct2_cols(const float* __restrict__ x, long long xs,
                                                              float* __restrict__ out, long long os,
@@ -1001,7 +1002,9 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct2_cols(const float* _
   using C = ColCfg<L, CT_>;
   constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
   __shared__ float2 buf[C::LDS];
-  const int c0 = blockIdx.x * CT, b = blockIdx.y;
+  int tile, plane, b;
+  col_block_order(tile, plane, b);
+  const int c0 = tile * CT;
   const float2* src = reinterpret_cast<const float2*>(x + (size_t)b * xs) + c0;
   float2* dst = reinterpret_cast<float2*>(out + (size_t)b * os) + c0;
   col_fwd_A_src<L, CT_, false>([&](int row, int c) { return src[(size_t)dct_perm(row, L) * NH + c]; }, buf, tw);
@@ -1145,7 +1148,7 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_dec(const float* _
 // Columns of the planes k_dct_rows_dec wrote: D2^-1, then (scale planes) the
 // forward column FFT, in place.  Plane p < nspec: spec plane p (complex
 // columns); p == nspec: the lowpass plane (pairs of real columns).
-// grid: (L/2 / CT, nspec + 1, nimg).
+// grid: (L/2 / CT, nspec + 1, nimg), walked in the XCD-aware order.
 template <int L, int CT_>
 __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct_cols_dec(float2* __restrict__ spec, long long ss,
                                                                 int nspec, float* __restrict__ low, long long ls,
@@ -1155,7 +1158,9 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct_cols_dec(float2* __r
   constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
   __shared__ float2 buf[C::LDS];
   __shared__ float2 ybuf[C::LDS];  // the plane's columns in natural order
-  const int c0 = blockIdx.x * CT, p = blockIdx.y, b = blockIdx.z;
+  int tile, p, b;
+  col_block_order(tile, p, b);
+  const int c0 = tile * CT;
   float2* base = (p == nspec ? reinterpret_cast<float2*>(low + (size_t)b * ls)
                              : spec + (size_t)b * ss + (size_t)p * L * NH) + c0;
   const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
@@ -1203,7 +1208,7 @@ struct LaunchDct {
     constexpr int ct = ColTile<L>::narrow ? 4 : 8;
     constexpr int ct1 = 2;
     if (step == 0) {
-      hipLaunchKernelGGL((k_dct2_cols<L, ct1>), dim3(L / 2 / ct1, nimg), dim3(ColCfg<L, ct1>::T), 0, s, x, xs,
+      hipLaunchKernelGGL((k_dct2_cols<L, ct1>), dim3(L / 2 / ct1, 1, nimg), dim3(ColCfg<L, ct1>::T), 0, s, x, xs,
                          xwork, ws, tw, dtw);
     } else if (step == 1) {
       hipLaunchKernelGGL((k_dct_rows_dec<L>), dim3(L / DctRowCfg<L>::ROWS, nspec + 1, nimg), dim3(DctRowCfg<L>::T),
