@@ -138,8 +138,8 @@ int shx_batch_inverse_shear_adjoint(shx_batch* batch, const float* x_bar, float*
 
 /* Autotuner: times the pyramid tile shapes, the reconstruction's
  * directions-per-block grouping, the column tile width of the
- * decomposition and reconstruction column passes and the decomposition's
- * front end (shx_set_dec_front) on the plan's own workspace
+ * decomposition and reconstruction column passes and the plan options
+ * (shx_set_plan_options) on the plan's own workspace
  * (the dictionary's one-image plan when batch is NULL) and keeps the fastest.
  * chosen (may be NULL, else 4 ints) receives {pyramid variant, group,
  * decomposition column tile, reconstruction column tile}.  Each candidate is
@@ -150,14 +150,18 @@ int shx_autotune(shx_dict* dict, shx_batch* batch, int reps, int* chosen, void* 
 /* Sets a schedule explicitly (pyramid variant 0..2, group 2/4/8, column tiles
  * 4 (L >= 128), 8, or 16 (L <= 512); 0 = default). */
 int shx_set_schedule(shx_dict* dict, shx_batch* batch, int pyr_variant, int group, int ct_dec, int ct_rec);
-/* Front end of the decomposition (shear_trans, denoise) of a plan: 0 the a
- * trous pyramid levels (atrousdec.m) followed by the FFT of every band-pass
- * plane; 1 the same transform in the DCT-II domain, where the pyramid's
- * symmetric extension makes every level a diagonal mask (three passes for
- * all levels; L >= 64).  Both give the same coefficients to FP32 rounding.
- * Default 0 until shx_autotune picks.  No counterpart in the reference. */
-int shx_set_dec_front(shx_dict* dict, shx_batch* batch, int front);
-int shx_get_dec_front(const shx_dict* dict, const shx_batch* batch, int* front);
+/* Plan options (both picked by shx_autotune; defaults 0, 0; L >= 64 for 1).
+ * With the pyramid's half-sample symmetric extension every a trous level is
+ * a diagonal mask on the 2D DCT-II coefficients, so the dependent levels of
+ * a pyramid collapse into three passes.  dct_front: the decomposition
+ * (shear_trans, denoise) computes the lowpass and the scale spectra that way
+ * instead of running the atrousdec.m levels and then the FFT of every
+ * band-pass plane.  dct_back: the reconstruction (inverse_shear*, denoise)
+ * goes from the direction-group partials to the image that way instead of
+ * the scale row pass and the atrousrec.m levels.  Every option gives the same
+ * results to FP32 rounding.  No counterpart in the reference. */
+int shx_set_plan_options(shx_dict* dict, shx_batch* batch, int dct_front, int dct_back);
+int shx_get_plan_options(const shx_dict* dict, const shx_batch* batch, int* dct_front, int* dct_back);
 
 /* ---- HIP graphs --------------------------------------------------------------
  * Any sequence of shx calls that neither allocates nor synchronises

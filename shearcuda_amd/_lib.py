@@ -51,8 +51,8 @@ _SIGNATURES = [
     ("shx_batch_inverse_shear_adjoint", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("shx_autotune", c_int, [c_void_p, c_void_p, c_int, P(c_int), c_void_p]),
     ("shx_set_schedule", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int]),
-    ("shx_set_dec_front", c_int, [c_void_p, c_void_p, c_int]),
-    ("shx_get_dec_front", c_int, [c_void_p, c_void_p, P(c_int)]),
+    ("shx_set_plan_options", c_int, [c_void_p, c_void_p, c_int, c_int]),
+    ("shx_get_plan_options", c_int, [c_void_p, c_void_p, P(c_int), P(c_int)]),
     ("shx_capture_begin", c_int, [c_void_p]),
     ("shx_capture_end", c_int, [c_void_p, P(c_void_p)]),
     ("shx_graph_launch", c_int, [c_void_p, c_void_p]),

@@ -574,8 +574,9 @@ def autotune(shear, batch=None, reps=5):
     bh = batch._h if batch is not None else None
     chosen = (ctypes.c_int * 4)()
     check(lib.shx_autotune(shear.handle, bh, int(reps), chosen, _stream()))
+    opts = get_plan_options(shear, batch)
     return {"pyramid_variant": chosen[0], "group": chosen[1], "col_tile_dec": chosen[2], "col_tile_rec": chosen[3],
-            "dct_front": get_dec_front(shear, batch)}
+            "dct_front": opts["dct_front"], "dct_back": opts["dct_back"]}
 
 
 def set_schedule(shear, pyramid_variant, group, batch=None, col_tile_dec=0, col_tile_rec=0):
@@ -583,17 +584,31 @@ def set_schedule(shear, pyramid_variant, group, batch=None, col_tile_dec=0, col_
                                int(group), int(col_tile_dec), int(col_tile_rec)))
 
 
+def get_plan_options(shear, batch=None):
+    """{"dct_front": bool, "dct_back": bool} of a plan (shx_get_plan_options)."""
+    f, k = ctypes.c_int(), ctypes.c_int()
+    check(lib.shx_get_plan_options(shear.handle, batch._h if batch is not None else None, ctypes.byref(f),
+                                   ctypes.byref(k)))
+    return {"dct_front": bool(f.value), "dct_back": bool(k.value)}
+
+
+def set_plan_options(shear, batch=None, dct_front=None, dct_back=None):
+    """Plan options (None keeps the current value): dct_front, the
+    decomposition's pyramid and scale spectra in the DCT-II domain; dct_back,
+    the reconstruction's scale sum and pyramid in the DCT-II domain (L >= 64)."""
+    cur = get_plan_options(shear, batch)
+    f = cur["dct_front"] if dct_front is None else bool(dct_front)
+    k = cur["dct_back"] if dct_back is None else bool(dct_back)
+    check(lib.shx_set_plan_options(shear.handle, batch._h if batch is not None else None, int(f), int(k)))
+
+
 def set_dec_front(shear, dct, batch=None):
-    """Decomposition front end of a plan: False the a trous pyramid levels
-    followed by the scale FFTs, True the same transform in the DCT-II domain
-    (L >= 64)."""
-    check(lib.shx_set_dec_front(shear.handle, batch._h if batch is not None else None, 1 if dct else 0))
+    """Decomposition front end of a plan (set_plan_options(dct_front=...))."""
+    set_plan_options(shear, batch, dct_front=dct)
 
 
 def get_dec_front(shear, batch=None):
-    f = ctypes.c_int()
-    check(lib.shx_get_dec_front(shear.handle, batch._h if batch is not None else None, ctypes.byref(f)))
-    return bool(f.value)
+    return get_plan_options(shear, batch)["dct_front"]
 
 
 # ---- undecimated wavelet transform -----------------------------------------------

@@ -59,6 +59,7 @@ struct Work {
   int ct_dec = 0;       // column tile width of the decomposition / reconstruction
   int ct_rec = 0;       // column passes (0 = default for L)
   bool dct_front = false;  // decomposition front end in the DCT domain (else pyramid levels + scale FFTs)
+  bool dct_back = false;   // reconstruction back end in the DCT domain (else scale row pass + pyramid levels)
   BandMap map{};        // the dictionary's band map with this plan's groups
   // side streams (one per scale) and events of the concurrent denoise: each
   // scale's spectral chain overlaps the pyramid levels
@@ -76,6 +77,7 @@ struct shx_dict {
   float2* Hbeta = nullptr;
   float2* tw = nullptr;
   float* dmask = nullptr;  // DCT-domain pyramid masks, S + 1 planes (L >= 64)
+  float* rmask = nullptr;  // the same for the reconstruction filters
   float2* dtw = nullptr;   // DCT twiddles exp(-i pi k / 2L), k < L
   Work w;  // one image
   size_t bytes = 0;
@@ -511,13 +513,18 @@ shx::ThreshSpec band_thresh(const shx_dict* d, const Thr& t) {
   return th;
 }
 
-void stages_spec_rec_tail(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::vector<Stage>& st) {
+// Reconstruction column pass (direction-group partials in w.q) and, unless
+// the DCT-domain back end takes the partials itself, the scale row pass
+// (scale planes y_j in w.planes).
+void stages_spec_rec_tail(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::vector<Stage>& st,
+                          bool scale_rows = true) {
   const int L = d->L, S = d->S, nb = d->nbands, ng = w.map.ngroups;
   const double P = 4.0 * L * L * nimg;
   st.push_back({"cols_rec", (2.0 * nb + ng) * P, [=] {
                   return shx::launch_cols_rec(L, w.band, w.s_band / 2, w.q, w.s_q / 2, d->Hmain, d->Hbeta, w.map,
                                               nimg, d->tw, s, w.ct_rec);
                 }});
+  if (!scale_rows) return;
   PlaneSet qset = shx::single_cell(reinterpret_cast<float*>(w.q), ng, w.s_q);
   PlaneSet yset = shx::single_cell(w.planes, S, w.s_planes);
   std::vector<int> sf(w.map.group_first, w.map.group_first + S + 1);
@@ -525,14 +532,20 @@ void stages_spec_rec_tail(shx_dict* d, const Work& w, int nimg, hipStream_t s, s
                 [=] { return shx::launch_rows_c2r(L, qset, yset, S, nimg, sf.data(), d->tw, s); }});
 }
 
+// The lowpass hard threshold of a reconstruction (lowpass first in the tables).
+shx::LowThresh low_thresh(const shx_dict* d, const Thr& t) {
+  shx::LowThresh lt{0, 0.f, nullptr, 0};
+  if (t.mode == 1) lt = shx::LowThresh{1, t.host[0], nullptr, 0};
+  if (t.mode == 2) lt = shx::LowThresh{2, 0.f, t.dev, 1 + d->nbands};
+  return lt;
+}
+
 void stages_pyr_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const Thr& t, float* x, long long xs,
                     hipStream_t s, std::vector<Stage>& st) {
   const int L = d->L, S = d->S;
   const long long pl = (long long)L * L;
   const double P = 4.0 * pl * nimg;
-  shx::LowThresh lt{0, 0.f, nullptr, 0};
-  if (t.mode == 1) lt = shx::LowThresh{1, t.host[0], nullptr, 0};
-  if (t.mode == 2) lt = shx::LowThresh{2, 0.f, t.dev, 1 + d->nbands};
+  const shx::LowThresh lt = low_thresh(d, t);
   const float* xl = c.low;
   long long xls = c.low_stride;
   for (int lev = S - 1; lev >= 0; --lev) {
@@ -548,6 +561,45 @@ void stages_pyr_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const
                   }});
     xl = o;
     xls = os;
+  }
+}
+
+// DCT-domain back end: from the direction-group partials (w.q) and the
+// lowpass to the image in three passes (the scale planes' row DCTs go through
+// w.planes' first S + 1 planes).
+void stages_dct_back(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const Thr& t, float* x, long long xs,
+                     hipStream_t s, std::vector<Stage>& st) {
+  const int L = d->L, S = d->S, ng = w.map.ngroups;
+  const double P = 4.0 * L * L * nimg;
+  const shx::LowThresh lt = low_thresh(d, t);
+  std::vector<int> gf(w.map.group_first, w.map.group_first + S + 1);
+  const long long qs = w.s_q / 2;
+  // reads the partials and the lowpass, writes S + 1 row-transformed planes
+  st.push_back({"dct_rows_rec", (ng + 1.0 + (S + 1)) * P, [=] {
+                  return shx::launch_dct_rec(L, 0, w.q, qs, gf.data(), S, c.low, c.low_stride, lt, w.planes,
+                                             w.s_planes, d->rmask, x, xs, nimg, d->tw, d->dtw, s);
+                }});
+  // reads and writes every plane, reads its mask
+  st.push_back({"dct_cols_rec", 3.0 * (S + 1) * P, [=] {
+                  return shx::launch_dct_rec(L, 1, w.q, qs, gf.data(), S, c.low, c.low_stride, lt, w.planes,
+                                             w.s_planes, d->rmask, x, xs, nimg, d->tw, d->dtw, s);
+                }});
+  st.push_back({"dct_rows_out", (S + 2.0) * P, [=] {
+                  return shx::launch_dct_rec(L, 2, w.q, qs, gf.data(), S, c.low, c.low_stride, lt, w.planes,
+                                             w.s_planes, d->rmask, x, xs, nimg, d->tw, d->dtw, s);
+                }});
+}
+
+// From the band spectra U (w.band) to the image: the reconstruction column
+// pass, then either back end.
+void stages_back_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const Thr& t, float* x, long long xs,
+                     hipStream_t s, std::vector<Stage>& st) {
+  if (w.dct_back) {
+    stages_spec_rec_tail(d, w, nimg, s, st, false);
+    stages_dct_back(d, w, nimg, c, t, x, xs, s, st);
+  } else {
+    stages_spec_rec_tail(d, w, nimg, s, st);
+    stages_pyr_rec(d, w, nimg, c, t, x, xs, s, st);
   }
 }
 
@@ -573,8 +625,7 @@ void stages_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const Thr
   shx::ThreshSpec th = band_thresh(d, t);
   st.push_back({"rows_r2c_bands", 2.0 * nb * P,
                 [=] { return shx::launch_rows_r2c(L, in, bandset, nb, nimg, th, d->tw, s); }});
-  stages_spec_rec_tail(d, w, nimg, s, st);
-  stages_pyr_rec(d, w, nimg, c, t, x, xs, s, st);
+  stages_back_rec(d, w, nimg, c, t, x, xs, s, st);
 }
 
 // Denoise: the decomposition's band row pass and the reconstruction's
@@ -591,8 +642,7 @@ void stages_denoise(shx_dict* d, const Work& w, int nimg, const float* x, long l
   // place of T, row by row)
   st.push_back({"rows_c2r_r2c_bands", (cb.ncells ? 3.0 : 2.0) * nb * P,
                 [=] { return shx::launch_rows_c2r_r2c(L, bandset, cb, bandset, nb, nimg, th, d->tw, s); }});
-  stages_spec_rec_tail(d, w, nimg, s, st);
-  stages_pyr_rec(d, w, nimg, c, t, out, os, s, st);
+  stages_back_rec(d, w, nimg, c, t, out, os, s, st);
 }
 
 // Planes [first, first + count) of a set whose range lies in one cell.
@@ -691,6 +741,12 @@ void stages_denoise_dag(shx_dict* d, Work& w, int nimg, const float* x, long lon
               return shx::launch_cols_rec(L, T, w.s_band / 2, Q, w.s_q / 2, Hm, Hb, m, nimg, d->tw, sj, w.ct_rec);
             }};
     e.stream = sj;
+    if (w.dct_back) {  // the back end takes the partials of every scale at once
+      e.record = true;
+      done[j] = (int)st.size();
+      st.push_back(e);
+      continue;
+    }
     st.push_back(e);
     const int sf[2] = {0, ng};
     Stage h{"rows_c2r_sum" + sfx, (ng + 1.0) * P,
@@ -700,8 +756,15 @@ void stages_denoise_dag(shx_dict* d, Work& w, int nimg, const float* x, long lon
     done[j] = (int)st.size();
     st.push_back(h);
   }
-  {
-    std::vector<Stage> tmp;
+  std::vector<Stage> tmp;
+  if (w.dct_back) {
+    stages_dct_back(d, w, nimg, c, t, out, os, s0, tmp);
+    tmp[0].waits = done;
+    for (Stage& g : tmp) {
+      g.stream = s0;
+      st.push_back(g);
+    }
+  } else {
     stages_pyr_rec(d, w, nimg, c, t, out, os, s0, tmp);
     for (int k = 0; k < S; ++k) {
       tmp[k].stream = s0;
@@ -1011,6 +1074,26 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t caller, 
         if (t > 0 && t < tf[f]) tf[f] = t;
       }
     w.dct_front = tf[1] < tf[0];
+    // reconstruction back end: scale row pass + pyramid levels, or the DCT domain
+    double tb[2] = {1e30, 1e30};
+    for (int round = 0; round < 3; ++round)
+      for (int f = 0; f < 2; ++f) {
+        std::vector<Stage> st;
+        if (f == 1) {
+          stages_dct_back(d, w, nimg, c, Thr{}, img + pl * nimg, pl, s, st);
+        } else {
+          const int L = d->L, S = d->S, ng = w.map.ngroups;
+          PlaneSet qset = shx::single_cell(reinterpret_cast<float*>(w.q), ng, w.s_q);
+          PlaneSet yset = shx::single_cell(w.planes, S, w.s_planes);
+          std::vector<int> sf(w.map.group_first, w.map.group_first + S + 1);
+          st.push_back({"rows_c2r_scales", 0.0,
+                        [=] { return shx::launch_rows_c2r(L, qset, yset, S, nimg, sf.data(), d->tw, s); }});
+          stages_pyr_rec(d, w, nimg, c, Thr{}, img + pl * nimg, pl, s, st);
+        }
+        const double t = time_stages_graph(st, s, reps);
+        if (t > 0 && t < tb[f]) tb[f] = t;
+      }
+    w.dct_back = tb[1] < tb[0];
   }
   hipStreamSynchronize(s);
   hipFree(img);
@@ -1099,11 +1182,13 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
   }
   shx::twiddles_fp32(L, tw.data());
   // DCT-domain front end (L >= 64): masks and twiddles
-  std::vector<float> dmask, dtw;
+  std::vector<float> dmask, rmask, dtw;
   if (L >= 64) {
     dmask.resize((size_t)(nscales + 1) * L * L);
+    rmask.resize(dmask.size());
     dtw.resize((size_t)2 * L);
-    shx::dct_pyramid_masks(L, nscales, dmask.data());
+    shx::dct_pyramid_masks(L, nscales, 0, dmask.data());
+    shx::dct_pyramid_masks(L, nscales, 1, rmask.data());
     shx::dct_twiddles_fp32(L, dtw.data());
   }
   const size_t bytes_hm = hm.size() * sizeof(float), bytes_hb = hb.size() * sizeof(float);
@@ -1112,6 +1197,7 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
   if ((e = hipMalloc(&d->Hmain, bytes_hm)) != hipSuccess || (e = hipMalloc(&d->Hbeta, bytes_hb)) != hipSuccess ||
       (e = hipMalloc(&d->tw, tw.size() * sizeof(float))) != hipSuccess ||
       (bytes_dm && ((e = hipMalloc(&d->dmask, bytes_dm)) != hipSuccess ||
+                    (e = hipMalloc(&d->rmask, bytes_dm)) != hipSuccess ||
                     (e = hipMalloc(&d->dtw, bytes_dt)) != hipSuccess))) {
     shx_clear_shearing_filters(d);
     return fail(SHX_ERR_ALLOC, "device allocation failed: %s", hipGetErrorString(e));
@@ -1120,11 +1206,12 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
     shx_clear_shearing_filters(d);
     return r;
   }
-  d->bytes = bytes_hm + bytes_hb + tw.size() * sizeof(float) + bytes_dm + bytes_dt + d->w.bytes;
+  d->bytes = bytes_hm + bytes_hb + tw.size() * sizeof(float) + 2 * bytes_dm + bytes_dt + d->w.bytes;
   if ((e = hipMemcpy(d->Hmain, hm.data(), bytes_hm, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(d->Hbeta, hb.data(), bytes_hb, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(d->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
       (bytes_dm && ((e = hipMemcpy(d->dmask, dmask.data(), bytes_dm, hipMemcpyHostToDevice)) != hipSuccess ||
+                    (e = hipMemcpy(d->rmask, rmask.data(), bytes_dm, hipMemcpyHostToDevice)) != hipSuccess ||
                     (e = hipMemcpy(d->dtw, dtw.data(), bytes_dt, hipMemcpyHostToDevice)) != hipSuccess))) {
     shx_clear_shearing_filters(d);
     return fail(SHX_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
@@ -1139,6 +1226,7 @@ int shx_clear_shearing_filters(shx_dict* d) {
   hipFree(d->Hbeta);
   hipFree(d->tw);
   hipFree(d->dmask);
+  hipFree(d->rmask);
   hipFree(d->dtw);
   free_work(d->w);
   delete d;
@@ -1299,19 +1387,25 @@ int shx_autotune(shx_dict* d, shx_batch* b, int reps, int* chosen, void* stream)
   return autotune_work(d, w, b ? b->nimg : 1, reps, (hipStream_t)stream, chosen);
 }
 
-int shx_set_dec_front(shx_dict* d, shx_batch* b, int front) {
+int shx_set_plan_options(shx_dict* d, shx_batch* b, int dct_front, int dct_back) {
   if (!d) return fail(SHX_ERR_ARG, "null dictionary");
   if (b && b->d != d) return fail(SHX_ERR_ARG, "batch belongs to another dictionary");
-  if (front != 0 && front != 1) return fail(SHX_ERR_ARG, "decomposition front end %d (0 pyramid levels, 1 DCT domain)", front);
-  if (front == 1 && !d->dmask) return fail(SHX_ERR_UNSUPPORTED, "DCT-domain front end needs L >= 64 (L = %d)", d->L);
-  (b ? b->w : d->w).dct_front = front == 1;
+  if ((dct_front != 0 && dct_front != 1) || (dct_back != 0 && dct_back != 1))
+    return fail(SHX_ERR_ARG, "plan options %d, %d (0 pyramid levels, 1 DCT domain)", dct_front, dct_back);
+  if ((dct_front || dct_back) && !d->dmask)
+    return fail(SHX_ERR_UNSUPPORTED, "the DCT-domain pyramid needs L >= 64 (L = %d)", d->L);
+  Work& w = b ? b->w : d->w;
+  w.dct_front = dct_front == 1;
+  w.dct_back = dct_back == 1;
   return SHX_OK;
 }
 
-int shx_get_dec_front(const shx_dict* d, const shx_batch* b, int* front) {
-  if (!d || !front) return fail(SHX_ERR_ARG, "null dictionary or output");
+int shx_get_plan_options(const shx_dict* d, const shx_batch* b, int* dct_front, int* dct_back) {
+  if (!d) return fail(SHX_ERR_ARG, "null dictionary");
   if (b && b->d != d) return fail(SHX_ERR_ARG, "batch belongs to another dictionary");
-  *front = (b ? b->w : d->w).dct_front ? 1 : 0;
+  const Work& w = b ? b->w : d->w;
+  if (dct_front) *dct_front = w.dct_front ? 1 : 0;
+  if (dct_back) *dct_back = w.dct_back ? 1 : 0;
   return SHX_OK;
 }
 

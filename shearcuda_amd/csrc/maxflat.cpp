@@ -260,17 +260,19 @@ std::vector<double> dct_response(const Full& f, int m, int L) {
 }
 }  // namespace
 
-void dct_pyramid_masks(int L, int nscales, float* out) {
+void dct_pyramid_masks(int L, int nscales, int pair, float* out) {
   const Maxflat& M = maxflat();
+  const Full& f0 = M.full[pair == 0 ? 0 : 2];  // h0 / g0
+  const Full& f1 = M.full[pair == 0 ? 1 : 3];  // h1 / g1
   const size_t pe = (size_t)L * L;
   const double inv = 1.0 / ((double)L * L);
   std::vector<double> P(pe, 1.0);
   for (int lev = 0; lev < nscales; ++lev) {
     const int m = 1 << lev;
-    const std::vector<double> H1 = dct_response(M.full[1], m, L);
+    const std::vector<double> H1 = dct_response(f1, m, L);
     float* o = out + (size_t)(nscales - 1 - lev) * pe;
     for (size_t i = 0; i < pe; ++i) o[i] = (float)(H1[i] * P[i] * inv);
-    const std::vector<double> H0 = dct_response(M.full[0], m, L);
+    const std::vector<double> H0 = dct_response(f0, m, L);
     for (size_t i = 0; i < pe; ++i) P[i] *= H0[i];
   }
   float* o = out + (size_t)nscales * pe;

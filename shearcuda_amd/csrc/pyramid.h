@@ -15,12 +15,13 @@ double maxflat_factor_error();
 bool maxflat_pair_fp64(int pair, int* len, int* rank, const double** u, const double** a, const double** b);
 // True when both pairs have the rank the kernels are specialised for.
 bool maxflat_ok();
-// Masks of the DCT-domain pyramid decomposition: nscales + 1 planes of L x L
-// in FP32, divided by L^2.  Plane p < nscales: the band-pass of level
-// lev = nscales - 1 - p, H1(2^lev) prod_{i<lev} H0(2^i); plane nscales: the
-// lowpass prod_{i<nscales} H0(2^i).  H(m)[kr][kc] is the filter's response on
-// the DCT-II grid at dilation m (row frequency kr, column frequency kc).
-void dct_pyramid_masks(int L, int nscales, float* out);
+// Masks of the DCT-domain pyramid: nscales + 1 planes of L x L in FP32,
+// divided by L^2.  pair 0 (decomposition, h0 / h1) or 1 (reconstruction,
+// g0 / g1).  Plane p < nscales: the band-pass of level lev = nscales - 1 - p,
+// F1(2^lev) prod_{i<lev} F0(2^i); plane nscales: the lowpass
+// prod_{i<nscales} F0(2^i).  F(m)[kr][kc] is the filter's response on the
+// DCT-II grid at dilation m (row frequency kr, column frequency kc).
+void dct_pyramid_masks(int L, int nscales, int pair, float* out);
 
 // Per-image element strides of the three planes a level touches
 // (dec: x, ylow, yhigh; rec: xlow, yhigh, out).  0 for a single image.

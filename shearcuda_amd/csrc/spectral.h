@@ -83,6 +83,17 @@ hipError_t launch_cols_fwd(int L, float2* Y1, long long y1_stride, int nplanes, 
 hipError_t launch_dct_dec(int L, int step, const float* x, long long xs, float* xwork, long long ws, float2* spec,
                           long long ss, int nspec, float* low, long long ls, const float* mask, int nimg,
                           const float2* tw, const float2* dtw, hipStream_t s);
+// DCT-domain pyramid reconstruction (L >= 64), steps 0..2: 0: D planes
+// p < nspec = row DCT-II of C2R(sum of Qg planes gfirst[p]..gfirst[p+1]-1),
+// plane nspec = row DCT-II of the lowpass (hard threshold lt); 1: column
+// DCT-II, times mask plane p, inverse column DCT-II, in place; 2: x = inverse
+// row DCT-II of the sum of the nspec + 1 planes.  mask: nspec + 1 planes of
+// L*L.  lt is declared with the pyramid kernels.
+struct LowThresh;
+hipError_t launch_dct_rec(int L, int step, const float2* Qg, long long qs, const int* gfirst, int nspec,
+                          const float* low, long long ls, const LowThresh& lt, float* D, long long ds,
+                          const float* mask, float* x, long long xs, int nimg, const float2* tw, const float2* dtw,
+                          hipStream_t s);
 hipError_t launch_cols_rec(int L, const float2* U, long long u_stride, float2* Q, long long q_stride,
                            const float2* Hm, const float2* Hb, const BandMap& map, int nimg, const float2* tw,
                            hipStream_t s, int ct = 0);

@@ -29,6 +29,7 @@
 // beta = (H(u,0)-H(u,N))/2 (stored separately):
 //   Q(u) = alpha(u) P(u) + beta(u) conj(P(-u)).
 #include "fft_reg.h"
+#include "pyramid.h"
 #include "spectral.h"
 
 namespace shx {
@@ -1231,6 +1232,299 @@ hipError_t launch_dct_dec(int L, int step, const float* x, long long xs, float* 
     case 256: return LaunchDct<256>::dec(x, xs, xwork, ws, spec, ss, nspec, low, ls, mask, nimg, tw, dtw, s, step);
     case 512: return LaunchDct<512>::dec(x, xs, xwork, ws, spec, ss, nspec, low, ls, mask, nimg, tw, dtw, s, step);
     case 1024: return LaunchDct<1024>::dec(x, xs, xwork, ws, spec, ss, nspec, low, ls, mask, nimg, tw, dtw, s, step);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DCT-domain pyramid reconstruction.  The a trous reconstruction of the scale
+// planes y_j and the lowpass is x = D2^-1( sum_j G_j . D2(y_j) + G_low . D2(low) )
+// with G_j = G1(m_j) prod_{i<j} G0(m_i) (synthesis filters' cosine responses,
+// divided by L^2 on the host), so the S dependent levels and the scale sum's
+// row pass become three passes:
+//   k_dct_rows_rec  per (row block, plane): y_j = C2R(sum of the scale's
+//                   direction-group partials) (lowpass: the thresholded real
+//                   row), then the row DCT-II; real rows out
+//   k_dct_cols_rec  per (column tile, plane): column DCT-II, times the plane's
+//                   mask, inverse column DCT-II, in place
+//   k_dct_rows_out  per row block: sum of the planes, inverse row DCT-II
+// (The lowpass hard threshold of the reconstruction is fused into the first.)
+
+// Real DCT-II row values from the packed half spectrum P of v = y[perm(.)]
+// (thread (rB, k2) holding P(k2 + R2 k1)): X[k] = Re(e_k V[k]),
+// X[L-k] = -Im(e_k V[k]), X[0] = V[0], X[L/2] = V[L/2] / sqrt(2).
+template <int L, int R1, int R2>
+__device__ __forceinline__ void dct_row_store(const float2 (&P)[R1], int k2, float2 w0, float* __restrict__ drow) {
+  constexpr int N = L / 2;
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const int k = k2 + R2 * k1;
+    if (k == 0) {
+      drow[0] = P[k1].x;
+      drow[N] = P[k1].y * 0.70710678118654752f;
+    } else {
+      const float2 W = cmul(ctw_mul<4 * L / R2, -1>(w0, k1), P[k1]);
+      drow[k] = W.x;
+      drow[L - k] = -W.y;
+    }
+  }
+}
+
+// grid: (L / ROWS, nspec + 1, nimg).  Plane p < nspec: scale p from the
+// direction-group partials gf.first[p] .. gf.first[p+1]-1 of Qg (packed rows,
+// plane stride L*L/2 complex); p == nspec: the lowpass.  D: nspec + 1 real
+// planes (stride ds per image).
+template <int L>
+__global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_rec(const float2* __restrict__ Qg, long long qs,
+                                                                   SumTable gf, int nspec,
+                                                                   const float* __restrict__ low, long long ls,
+                                                                   LowThresh lt, float* __restrict__ D, long long ds,
+                                                                   const float2* __restrict__ tw,
+                                                                   const float2* __restrict__ dtw) {
+  using C = DctRowCfg<L>;
+  constexpr int N = C::N, R1 = C::R1, R2 = C::R2, ROWS = C::ROWS;
+  __shared__ float2 buf[C::LDS];
+  __shared__ float2 yrow[ROWS][N];  // the real row y as pairs (y[2i], y[2i+1])
+  const int tid = threadIdx.x;
+  const int p = blockIdx.y, b = blockIdx.z;
+  const int row0 = blockIdx.x * ROWS;
+  const int r = tid / R1, n1 = tid % R1;
+  const bool validA = r < ROWS;
+  const int rB = tid / R2, k2 = tid % R2;
+  const auto ftw = row_fwd_tw<L, R1, R2>(n1, k2, tw);
+  const float2 w0 = dtw[k2];
+  // z[n] = (y[perm(2n)], y[perm(2n+1)]): (y[4n], y[4n+2]) for n < N/2 and
+  // (y[4m+3], y[4m+1]), m = N-1-n, above
+  float2 v[R2];
+  if (p < nspec) {
+    const int q0 = gf.first[p], q1 = gf.first[p + 1];
+    const float2* qb = Qg + (size_t)b * qs;
+    auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) x[n2] = xm[n2] = make_float2(0.f, 0.f);
+      for (int q = q0; q < q1; ++q) {
+        float2 a[R2], am[R2];
+        gather_row<N, R1, R2>(qb + (size_t)q * L * N + (size_t)(row0 + rr) * N, nn, a, am);
+        __builtin_amdgcn_sched_barrier(0);  // the group's loads in flight together
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+          x[n2] = cadd(x[n2], a[n2]);
+          xm[n2] = cadd(xm[n2], am[n2]);
+        }
+      }
+    };
+    float2 u[R1];
+    row_inv_to_B<L, R1, R2, C::PADSH>(u, validA, r, n1, buf, true, rB, k2, tw, gather);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) yrow[rB][k2 + R2 * k1] = u[k1];
+    __syncthreads();  // also: every pass-B read of buf is complete
+    if (validA) {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        const int n = n1 + R1 * n2;
+        const int m = n2 < R2 / 2 ? n : N - 1 - n;
+        const float2 a = yrow[r][2 * m], c = yrow[r][2 * m + 1];
+        v[n2] = n2 < R2 / 2 ? make_float2(a.x, c.x) : make_float2(c.y, a.y);
+      }
+    }
+  } else {
+    const float th = lt.mode == 0 ? -1.0f : (lt.mode == 1 ? lt.v : lt.dev[(size_t)b * lt.stride]);
+    if (validA) {
+      const float4* xr = reinterpret_cast<const float4*>(low + (size_t)b * ls + (size_t)(row0 + r) * L);
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        const int n = n1 + R1 * n2;
+        if (n2 < R2 / 2) {
+          const float4 f = xr[n];
+          v[n2] = make_float2(hard_keep(f.x, th), hard_keep(f.z, th));
+        } else {
+          const float4 f = xr[N - 1 - n];
+          v[n2] = make_float2(hard_keep(f.w, th), hard_keep(f.y, th));
+        }
+      }
+    }
+  }
+  float2 P[R1];
+  row_fwd_to_P<L, R1, R2, C::PADSH>(v, validA, r, n1, buf, rB, k2, ftw, P);
+  dct_row_store<L, R1, R2>(P, k2, w0, D + (size_t)b * ds + (size_t)p * L * L + (size_t)(row0 + rB) * L);
+}
+
+// Column DCT-II of plane p of D (pairs of real columns), times the plane's
+// mask, then the inverse column DCT-II, in place.  grid: (L/2 / CT, nplanes,
+// nimg), XCD-aware order.
+template <int L, int CT_>
+__global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct_cols_rec(float* __restrict__ D, long long ds,
+                                                                const float* __restrict__ mask,
+                                                                const float2* __restrict__ tw,
+                                                                const float2* __restrict__ dtw) {
+  using C = ColCfg<L, CT_>;
+  constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
+  __shared__ float2 buf[C::LDS];
+  int tile, p, b;
+  col_block_order(tile, p, b);
+  const int c0 = tile * CT;
+  float2* base = reinterpret_cast<float2*>(D + (size_t)b * ds + (size_t)p * L * L) + c0;
+  const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
+  col_fwd_A_src<L, CT_, false>([&](int row, int cc) { return base[(size_t)dct_perm(row, L) * NH + cc]; }, buf, tw);
+  const float2 w0 = dtw[k2];
+  const PassTw<R1> ivt = col_inv_tw<L>(k2, tw);
+  // the mask of this thread's two real columns at rows k = k2 + R2 k1
+  const float2* mcol = reinterpret_cast<const float2*>(mask + (size_t)p * L * L) + c0 + c;
+  float2 mk[R1];
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) mk[k1] = mcol[(size_t)(k2 + R2 * k1) * NH];
+  __syncthreads();
+  float2 u[R1];
+  col_fwd_B<L, CT_>(u, buf);
+  __syncthreads();  // pass-B reads of buf are complete
+  float2* row = buf + C::Lds::ix(k2, c);  // row k2 + R2 k1
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) row[k1 * (R2 / R1) * C::Lds::BLK] = u[k1];
+  __syncthreads();
+  // X = D2 (two real columns as one complex column), times the mask
+  float2 xm[R1];
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const int k = k2 + R2 * k1;
+    const float2 vm = buf[C::Lds::ix((L - k) & (L - 1), c)];
+    const float2 e = ctw_mul<4 * L / R2, -1>(w0, k1);
+    const float2 X = cscale(cadd(cmul(e, u[k1]), cmulc(vm, e)), 0.5f);
+    xm[k1] = make_float2(mk[k1].x * X.x, mk[k1].y * X.y);
+  }
+  __syncthreads();  // reads of the exchanged spectrum are complete
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) row[k1 * (R2 / R1) * C::Lds::BLK] = xm[k1];
+  __syncthreads();
+  // D2^-1 pre-twiddle V[k] = e^{i t_k} (X[k] - i X[L-k]), X[L] = 0
+  float2 q[R1];
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const int k = k2 + R2 * k1;
+    const float2 xl = k == 0 ? make_float2(0.f, 0.f) : buf[C::Lds::ix(L - k, c)];
+    const float2 e = ctw_mul<4 * L / R2, -1>(w0, k1);
+    q[k1] = cmulc(make_float2(xm[k1].x + xl.y, xm[k1].y - xl.x), e);
+  }
+  col_inv_A<L, CT_, true>(q, buf, ivt);
+  col_inv_B_to<L, CT_, false>(buf, [&](int rr, int cc, float2 val) { base[(size_t)dct_perm(rr, L) * NH + cc] = val; });
+}
+
+// Sum of the nplanes planes of D, inverse row DCT-II, into x.
+// grid: (L / ROWS, 1, nimg).
+template <int L>
+__global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_out(const float* __restrict__ D, long long ds,
+                                                                   int nplanes, float* __restrict__ x, long long xs,
+                                                                   const float2* __restrict__ tw,
+                                                                   const float2* __restrict__ dtw) {
+  using C = DctRowCfg<L>;
+  constexpr int N = C::N, R1 = C::R1, R2 = C::R2, ROWS = C::ROWS;
+  static_assert(L % (4 * R2) == 0, "row chunking");
+  __shared__ float2 buf[C::LDS];
+  __shared__ float xrow[ROWS][L];
+  const int tid = threadIdx.x;
+  const int b = blockIdx.z;
+  const int row0 = blockIdx.x * ROWS;
+  const int r = tid / R1, n1 = tid % R1;
+  const bool validA = r < ROWS;
+  const int rB = tid / R2, k2 = tid % R2;
+  // the summed row X (thread (rB, k2): 16-byte chunks k2 + R2 i)
+  {
+    constexpr int NC = L / 4 / R2;
+    const float4* src = reinterpret_cast<const float4*>(D + (size_t)b * ds + (size_t)(row0 + rB) * L);
+    float4 acc[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int pl = 0; pl < nplanes; ++pl) {
+      float4 a[NC];
+#pragma unroll
+      for (int i = 0; i < NC; ++i) a[i] = src[(size_t)pl * L * L / 4 + k2 + R2 * i];
+#pragma unroll
+      for (int i = 0; i < NC; ++i) {
+        acc[i].x += a[i].x;
+        acc[i].y += a[i].y;
+        acc[i].z += a[i].z;
+        acc[i].w += a[i].w;
+      }
+    }
+    float4* xr = reinterpret_cast<float4*>(&xrow[rB][0]);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) xr[k2 + R2 * i] = acc[i];
+  }
+  const float2 e1 = dtw[n1];
+  __syncthreads();
+  // packed spectrum of v: P'[k] = e^{i t_k} (X[k] - i X[L-k]) (k >= 1),
+  // P'[0] = (X[0], sqrt(2) X[L/2]); partners P'[N-k], e_{N-k} = e^{-i pi/4} conj(e_k)
+  auto gather = [&](int rr, int nn, float2 (&xq)[R2], float2 (&xmq)[R2]) {
+    const float* X = xrow[rr];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int k = nn + R1 * n2;
+      const float2 e = ctw_mul<4 * L / R1, -1>(e1, n2);
+      if (k == 0) {
+        xq[n2] = xmq[n2] = make_float2(X[0], 1.41421356237309505f * X[N]);
+      } else {
+        xq[n2] = cmulc(make_float2(X[k], -X[L - k]), e);
+        // e_{N-k}^* = e^{i pi/4} e_k
+        const float2 em = cmul(make_float2(0.70710678118654752f, 0.70710678118654752f), e);
+        xmq[n2] = cmul(make_float2(X[N - k], -X[N + k]), em);
+      }
+    }
+  };
+  float2 u[R1];
+  row_inv_to_B<L, R1, R2, C::PADSH>(u, validA, r, n1, buf, true, rB, k2, tw, gather);
+  __syncthreads();  // every read of xrow is complete
+  // u[k1] = L (v[2m], v[2m+1]), m = k2 + R2 k1; x[perm(n)] = v[n]
+  {
+    float* vr = xrow[rB];
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int m = k2 + R2 * k1;
+      vr[2 * m] = u[k1].x;
+      vr[2 * m + 1] = u[k1].y;
+    }
+  }
+  __syncthreads();
+  // (x[2m], x[2m+1]) = (v[m], v[L-1-m])
+  float2* xo = reinterpret_cast<float2*>(x + (size_t)b * xs + (size_t)(row0 + rB) * L);
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const int m = k2 + R2 * k1;
+    xo[m] = make_float2(xrow[rB][m], xrow[rB][L - 1 - m]);
+  }
+}
+
+template <int L>
+struct LaunchDctRec {
+  static hipError_t rec(const float2* Qg, long long qs, const int* gfirst, int nspec, const float* low, long long ls,
+                        const LowThresh& lt, float* D, long long ds, const float* mask, float* x, long long xs,
+                        int nimg, const float2* tw, const float2* dtw, hipStream_t s, int step) {
+    constexpr int ct = ColTile<L>::narrow ? 4 : 8;
+    if (step == 0) {
+      SumTable gf{};
+      for (int i = 0; i <= nspec; ++i) gf.first[i] = gfirst[i];
+      hipLaunchKernelGGL((k_dct_rows_rec<L>), dim3(L / DctRowCfg<L>::ROWS, nspec + 1, nimg), dim3(DctRowCfg<L>::T),
+                         0, s, Qg, qs, gf, nspec, low, ls, lt, D, ds, tw, dtw);
+    } else if (step == 1) {
+      hipLaunchKernelGGL((k_dct_cols_rec<L, ct>), dim3(L / 2 / ct, nspec + 1, nimg), dim3(ColCfg<L, ct>::T), 0, s, D,
+                         ds, mask, tw, dtw);
+    } else {
+      hipLaunchKernelGGL((k_dct_rows_out<L>), dim3(L / DctRowCfg<L>::ROWS, 1, nimg), dim3(DctRowCfg<L>::T), 0, s, D,
+                         ds, nspec + 1, x, xs, tw, dtw);
+    }
+    return hipGetLastError();
+  }
+};
+
+hipError_t launch_dct_rec(int L, int step, const float2* Qg, long long qs, const int* gfirst, int nspec,
+                          const float* low, long long ls, const LowThresh& lt, float* D, long long ds,
+                          const float* mask, float* x, long long xs, int nimg, const float2* tw, const float2* dtw,
+                          hipStream_t s) {
+  if (step < 0 || step > 2 || nspec < 1 || nspec > SHX_MAX_SCALES) return hipErrorInvalidValue;
+  switch (L) {
+    case 64: return LaunchDctRec<64>::rec(Qg, qs, gfirst, nspec, low, ls, lt, D, ds, mask, x, xs, nimg, tw, dtw, s, step);
+    case 128: return LaunchDctRec<128>::rec(Qg, qs, gfirst, nspec, low, ls, lt, D, ds, mask, x, xs, nimg, tw, dtw, s, step);
+    case 256: return LaunchDctRec<256>::rec(Qg, qs, gfirst, nspec, low, ls, lt, D, ds, mask, x, xs, nimg, tw, dtw, s, step);
+    case 512: return LaunchDctRec<512>::rec(Qg, qs, gfirst, nspec, low, ls, lt, D, ds, mask, x, xs, nimg, tw, dtw, s, step);
+    case 1024: return LaunchDctRec<1024>::rec(Qg, qs, gfirst, nspec, low, ls, lt, D, ds, mask, x, xs, nimg, tw, dtw, s, step);
     default: return hipErrorInvalidValue;
   }
 }

@@ -1,14 +1,14 @@
-"""The DCT-domain decomposition front end against the pyramid front end and
-the FP64 oracle.
+"""The DCT-domain pyramid (decomposition front end and reconstruction back
+end) against the level-by-level pyramid and the FP64 oracle.
 
 With the half-sample symmetric extension of the a trous pyramid
-(atrousdec.m with symext.m, folded with period 2L) and the symmetric maxflat
-filters, every pyramid level is a diagonal mask on the 2D DCT-II
-coefficients.  The front end computes the lowpass and the scale spectra of all
-levels in three passes from those masks (shx_set_dec_front).  Both front ends
-must give the same coefficients to FP32 rounding, and the DCT front end's
-coefficients must meet the oracle bar of the pyramid front end (max and RMS
-relative error <= 1e-5 per band).
+(atrousdec.m / atrousrec.m with symext.m, folded with period 2L) and the
+symmetric maxflat filters, every pyramid level is a diagonal mask on the 2D
+DCT-II coefficients.  The front end computes the lowpass and the scale
+spectra of all levels in three passes from those masks, the back end the
+image from the scale partials and the lowpass (shx_set_plan_options).  Both
+forms must agree to FP32 rounding, and the DCT forms must meet the oracle bar
+of the pyramid forms (max and RMS relative error <= 1e-5 per band / image).
 """
 import numpy as np
 import pytest
@@ -163,4 +163,110 @@ def test_dct_front_needs_l64(gpu):
             api.set_dec_front(shear, True)
         assert not api.get_dec_front(shear)
     finally:
+        shear.close()
+
+
+def _rand_cells(api, shear, L, seed):
+    rng = np.random.default_rng(seed)
+    cells = []
+    for c in shear.alloc_coeffs():
+        cells.append(_gpu(rng.standard_normal(tuple(c.shape)) * (30.0 if not cells else 3.0)))
+    return cells
+
+
+@pytest.mark.parametrize("L,dcomp,dsize", [
+    (64, None, None), (128, None, None), (256, None, None), (512, None, None), (1024, None, None),
+    (256, (2, 3), (32, 16)),
+    (256, (3, 3, 4, 4, 4), (32, 32, 16, 16, 16)),
+    (128, (1,), (16,)),
+])
+def test_backs_agree(gpu, L, dcomp, dsize):
+    """inverse_shear (plain and thresholded) with the DCT back end against the
+    scale row pass + pyramid levels, on inconsistent random coefficients."""
+    from shearcuda_amd import api
+    print("case dct back vs pyramid L=%d dcomp=%s" % (L, dcomp), flush=True)
+    shear = api.make_dictionary(L) if dcomp is None else api.make_dictionary(L, dcomp=dcomp, dsize=dsize)
+    try:
+        cells = _rand_cells(api, shear, L, 31)
+        assert not api.get_plan_options(shear)["dct_back"]
+        ref = api.inverse_shear_cuda(cells, "maxflat", shear).cpu().numpy()
+        api.set_plan_options(shear, dct_back=True)
+        assert api.get_plan_options(shear) == {"dct_front": False, "dct_back": True}
+        got = api.inverse_shear_cuda(cells, "maxflat", shear).cpu().numpy()
+        mx, rms = _rel(got, ref)
+        print("L=%d inverse max-rel %.2e rms-rel %.2e" % (L, mx, rms), flush=True)
+        assert mx <= TOL and rms <= TOL
+    finally:
+        shear.close()
+
+
+@pytest.mark.parametrize("L,shift", [(256, "none"), (512, "none"), (256, "center")])
+def test_dct_back_matches_oracle(gpu, L, shift):
+    from shearcuda_amd import api
+    from shearcuda_amd.synth import noisy_ellipses
+    print("case dct back vs oracle L=%d shift=%s" % (L, shift), flush=True)
+    _, x = noisy_ellipses(L, sigma=20.0, seed=4)
+    H = o.make_dictionary(L, shift=shift)
+    ref_c = o.nsst_dec2(x, H)
+    rng = np.random.default_rng(9)
+    ref_c = [c + rng.standard_normal(c.shape) * 2.0 for c in ref_c]
+    shear = api.make_dictionary(L, shift=shift)
+    try:
+        api.set_plan_options(shear, dct_front=True, dct_back=True)
+        got = api.inverse_shear_cuda([_gpu(c) for c in ref_c], "maxflat", shear).cpu().numpy()
+        mx, rms = _rel(got, o.nsst_rec2(ref_c, H))
+        print("inverse max-rel %.3e rms-rel %.3e" % (mx, rms), flush=True)
+        assert mx <= TOL and rms <= TOL
+    finally:
+        shear.close()
+
+
+def test_dct_both_denoise_and_batch(gpu):
+    """Denoise with both DCT forms: against its own coefficients reconstructed
+    under its keep decisions by the pyramid path, and the batched concurrent
+    plan against the plain plan bit for bit."""
+    from shearcuda_amd import api
+    from shearcuda_amd.synth import noisy_ellipses, white_noise
+    L, n = 256, 3
+    print("case dct front + back denoise L=%d" % L, flush=True)
+    shear = api.make_dictionary(L)
+    b = api.Batch(shear, n)
+    try:
+        sc = api.nsst_scalars_cuda(shear, _gpu(white_noise(L)))
+        _, x = noisy_ellipses(L, sigma=20.0, seed=13)
+        xg = _gpu(x)
+        api.set_plan_options(shear, dct_front=True, dct_back=True)
+        got, C = api.denoise_cuda(xg, shear, 20.0, sc)
+        C = _cells(C)
+        th = api.band_thresholds(shear, 20.0, sc).astype(np.float32)
+        masked, k = [], 0
+        for c in C:
+            c3 = c.reshape(-1, L, L)
+            m = np.stack([np.where(np.abs(c3[i]) > th[k + i], c3[i], 0.0) for i in range(c3.shape[0])])
+            k += c3.shape[0]
+            masked.append(_gpu(m.reshape(c.shape)))
+        api.set_plan_options(shear, dct_front=False, dct_back=False)
+        rec = api.inverse_shear_cuda(masked, "maxflat", shear).cpu().numpy()
+        mx, rms = _rel(got.cpu().numpy(), rec)
+        print("denoise vs masked pyramid reconstruction max-rel %.2e rms-rel %.2e" % (mx, rms), flush=True)
+        assert mx <= TOL and rms <= TOL
+        api.set_plan_options(shear, dct_front=True, dct_back=True)
+        api.set_plan_options(shear, b, dct_front=True, dct_back=True)
+        sig = np.array([10.0, 20.0, 30.0])
+        xs = _gpu(np.stack([noisy_ellipses(L, sigma=float(s), seed=80 + i)[1] for i, s in enumerate(sig)]))
+        thb = api.batch_thresholds(shear, sig, sc)
+        out = torch.empty_like(xs)
+        b.denoise(xs, None, thb, out)
+        rt = torch.empty_like(xs)
+        cb = b.alloc_coeffs()
+        b.shear_trans(xs, cb)
+        b.inverse_shear(cb, rt)
+        torch.cuda.synchronize()
+        for i in range(n):
+            pd, _ = api.denoise_cuda(xs[i].contiguous(), shear, float(sig[i]), sc, coeffs=False)
+            assert np.array_equal(out[i].cpu().numpy(), pd.cpu().numpy()), i
+            mx, rms = _rel(rt[i].cpu().numpy(), xs[i].cpu().numpy())
+            assert mx <= TOL and rms <= TOL, (i, mx, rms)
+    finally:
+        b.close()
         shear.close()
