@@ -183,16 +183,22 @@ __device__ __forceinline__ void row_fwd_to_P(float2 (&v)[R2], bool validA, int r
   }
 }
 
-// P[k] and P[N-k] of one packed row for k = n1 + R1 n2, all loads issued
-// before any is used.
+// P[k] and P[N-k] of one packed row for k = n1 + R1 n2.  Only P[k] is
+// loaded: P[N-k] = P((R1 - n1) + R1 (R2 - 1 - n2)) is the partner lane's
+// value (lanes of a row are the R1 consecutive lanes n1 = 0 .. R1-1, all
+// active together), or for n1 = 0 this lane's own P(R1 ((R2 - n2) mod R2)).
 template <int N, int R1, int R2>
 __device__ __forceinline__ void gather_row(const float2* __restrict__ row, int n1, float2 (&x)[R2],
                                            float2 (&xm)[R2]) {
+  static_assert(64 % R1 == 0, "a row's lanes lie in one wave");
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) x[n2] = row[n1 + R1 * n2];
+  const int lane = threadIdx.x & 63;
+  const int src = (lane & ~(R1 - 1)) | ((R1 - n1) & (R1 - 1));
 #pragma unroll
   for (int n2 = 0; n2 < R2; ++n2) {
-    const int k = n1 + R1 * n2;
-    x[n2] = row[k];
-    xm[n2] = row[(N - k) & (N - 1)];
+    const float2 sh = shfl2(x[R2 - 1 - n2], src);
+    xm[n2] = n1 == 0 ? x[(R2 - n2) & (R2 - 1)] : sh;
   }
 }
 
