@@ -236,6 +236,48 @@ __device__ __forceinline__ void row_inv_to_B(float2 (&u)[R1], bool validA, int r
   }
 }
 
+// Twiddles of row_inv_to_B for pass-A index n1, loaded by the caller ahead of
+// time (when its gather reads LDS, so the table loads would otherwise start
+// only after it): w_L^n1 and the pass-A twiddles.
+template <int R2>
+struct RowInvTw {
+  PassTw<R2> pt;
+  float2 w0;
+};
+template <int L, int R1, int R2>
+__device__ __forceinline__ RowInvTw<R2> row_inv_tw(int n1, const float2* __restrict__ tw) {
+  RowInvTw<R2> t;
+  t.w0 = tw[n1];
+  passA_tw<R2, +1, 2>(t.pt, n1, tw);
+  return t;
+}
+
+// row_inv_to_B with the twiddles preloaded (row_inv_tw of this thread's n1).
+template <int L, int R1, int R2, int PADSH, class GATHER>
+__device__ __forceinline__ void row_inv_to_B_tw(float2 (&u)[R1], bool validA, int r, int n1, float2* buf,
+                                                bool validB, int rB, int k2, const RowInvTw<R2>& t, GATHER gather) {
+  constexpr int N = L / 2;
+  if (validA) {
+    float2 v[R2], xs[R2], xms[R2];
+    gather(r, n1, xs, xms);
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int k = n1 + R1 * n2;
+      const float2 x = xs[n2], xm = xms[n2];
+      v[n2] = (k == 0) ? make_float2(x.x + x.y, x.x - x.y) : c2r_pre(x, xm, ctw_mul<L / R1, -1>(t.w0, n2));
+    }
+    passA_apply<R2, +1>(v, t.pt);
+#pragma unroll
+    for (int k = 0; k < R2; ++k) buf[lds_ix<PADSH>(r * N + n1 + R1 * k)] = v[k];
+  }
+  __syncthreads();
+  if (validB) {
+#pragma unroll
+    for (int a = 0; a < R1; ++a) u[a] = buf[lds_ix<PADSH>(rB * N + a + R1 * k2)];
+    dft<R1, +1>(u);
+  }
+}
+
 // Row real-to-complex FFT (packed output), optional hard threshold on load.
 // grid: (L / ROWS, nplanes, nimg).
 template <int L>
@@ -1014,12 +1056,12 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct2_cols(const float* _
   const int c0 = tile * CT;
   const float2* src = reinterpret_cast<const float2*>(x + (size_t)b * xs) + c0;
   float2* dst = reinterpret_cast<float2*>(out + (size_t)b * os) + c0;
-  col_fwd_A_src<L, CT_, false>([&](int row, int c) { return src[(size_t)dct_perm(row, L) * NH + c]; }, buf, tw);
+  const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
+  const float2 w0 = dtw[k2];  // with the data loads, not after the transpose
+  col_fwd_A_src<L, CT_, false>([&](int row, int cc) { return src[(size_t)dct_perm(row, L) * NH + cc]; }, buf, tw);
   __syncthreads();
   float2 u[R1];
   col_fwd_B<L, CT_>(u, buf);
-  const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
-  const float2 w0 = dtw[k2];
   __syncthreads();  // pass-B reads of buf are complete
   {
     float2* row = buf + C::Lds::ix(k2, c);  // V at row k = k2 + R2 k1
@@ -1305,18 +1347,33 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_rec(const float2* 
   if (p < nspec) {
     const int q0 = gf.first[p], q1 = gf.first[p + 1];
     const float2* qb = Qg + (size_t)b * qs;
+    // the scale's partials summed two groups per round trip (an odd last
+    // group pairs with itself at weight 0); P[N-k] of the sum from the
+    // partner lane (as gather_row)
     auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
 #pragma unroll
-      for (int n2 = 0; n2 < R2; ++n2) x[n2] = xm[n2] = make_float2(0.f, 0.f);
-      for (int q = q0; q < q1; ++q) {
-        float2 a[R2], am[R2];
-        gather_row<N, R1, R2>(qb + (size_t)q * L * N + (size_t)(row0 + rr) * N, nn, a, am);
-        __builtin_amdgcn_sched_barrier(0);  // the group's loads in flight together
+      for (int n2 = 0; n2 < R2; ++n2) x[n2] = make_float2(0.f, 0.f);
+      for (int q = q0; q < q1; q += 2) {
+        const int q2 = q + 1 < q1 ? q + 1 : q;
+        const float wgt = q + 1 < q1 ? 1.0f : 0.0f;
+        const float2* ra = qb + (size_t)q * L * N + (size_t)(row0 + rr) * N;
+        const float2* rc = qb + (size_t)q2 * L * N + (size_t)(row0 + rr) * N;
+        float2 a[R2], c[R2];
 #pragma unroll
         for (int n2 = 0; n2 < R2; ++n2) {
-          x[n2] = cadd(x[n2], a[n2]);
-          xm[n2] = cadd(xm[n2], am[n2]);
+          a[n2] = ra[nn + R1 * n2];
+          c[n2] = rc[nn + R1 * n2];
         }
+        __builtin_amdgcn_sched_barrier(0);  // both groups' loads in flight together
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) x[n2] = cadd(cadd(x[n2], a[n2]), cscale(c[n2], wgt));
+      }
+      const int lane = threadIdx.x & 63;
+      const int src = (lane & ~(R1 - 1)) | ((R1 - nn) & (R1 - 1));
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        const float2 sh = shfl2(x[R2 - 1 - n2], src);
+        xm[n2] = nn == 0 ? x[(R2 - n2) & (R2 - 1)] : sh;
       }
     };
     float2 u[R1];
@@ -1432,30 +1489,41 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_out(const float* _
   const int r = tid / R1, n1 = tid % R1;
   const bool validA = r < ROWS;
   const int rB = tid / R2, k2 = tid % R2;
-  // the summed row X (thread (rB, k2): 16-byte chunks k2 + R2 i)
+  const float2 e1 = dtw[n1];
+  const RowInvTw<R2> itw = row_inv_tw<L, R1, R2>(n1, tw);
+  // the summed row X (thread (rB, k2): 16-byte chunks k2 + R2 i), three
+  // planes per round trip (past the last plane, the last again at weight 0)
   {
     constexpr int NC = L / 4 / R2;
     const float4* src = reinterpret_cast<const float4*>(D + (size_t)b * ds + (size_t)(row0 + rB) * L);
     float4 acc[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int pl = 0; pl < nplanes; ++pl) {
-      float4 a[NC];
+    for (int pl = 0; pl < nplanes; pl += 3) {
+      float4 a[3][NC];
 #pragma unroll
-      for (int i = 0; i < NC; ++i) a[i] = src[(size_t)pl * L * L / 4 + k2 + R2 * i];
+      for (int j = 0; j < 3; ++j) {
+        const int q = pl + j < nplanes ? pl + j : nplanes - 1;
 #pragma unroll
-      for (int i = 0; i < NC; ++i) {
-        acc[i].x += a[i].x;
-        acc[i].y += a[i].y;
-        acc[i].z += a[i].z;
-        acc[i].w += a[i].w;
+        for (int i = 0; i < NC; ++i) a[j][i] = src[(size_t)q * L * L / 4 + k2 + R2 * i];
+      }
+      __builtin_amdgcn_sched_barrier(0);  // the three planes' loads in flight together
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const float wgt = pl + j < nplanes ? 1.0f : 0.0f;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+          acc[i].x += wgt * a[j][i].x;
+          acc[i].y += wgt * a[j][i].y;
+          acc[i].z += wgt * a[j][i].z;
+          acc[i].w += wgt * a[j][i].w;
+        }
       }
     }
     float4* xr = reinterpret_cast<float4*>(&xrow[rB][0]);
 #pragma unroll
     for (int i = 0; i < NC; ++i) xr[k2 + R2 * i] = acc[i];
   }
-  const float2 e1 = dtw[n1];
   __syncthreads();
   // packed spectrum of v: P'[k] = e^{i t_k} (X[k] - i X[L-k]) (k >= 1),
   // P'[0] = (X[0], sqrt(2) X[L/2]); partners P'[N-k], e_{N-k} = e^{-i pi/4} conj(e_k)
@@ -1476,7 +1544,7 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_out(const float* _
     }
   };
   float2 u[R1];
-  row_inv_to_B<L, R1, R2, C::PADSH>(u, validA, r, n1, buf, true, rB, k2, tw, gather);
+  row_inv_to_B_tw<L, R1, R2, C::PADSH>(u, validA, r, n1, buf, true, rB, k2, itw, gather);
   __syncthreads();  // every read of xrow is complete
   // u[k1] = L (v[2m], v[2m+1]), m = k2 + R2 k1; x[perm(n)] = v[n]
   {
