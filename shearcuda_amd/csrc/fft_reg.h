@@ -263,6 +263,15 @@ __device__ __forceinline__ PassTw<RB> pin(PassTw<RB> t) {
   return t;
 }
 
+template <int R>
+__device__ __forceinline__ PairTw<R> pin(PairTw<R> t) {
+#pragma unroll
+  for (int i = 1; i < PairTw<R>::LO; ++i) asm volatile("" : "+v"(t.lo[i].x), "+v"(t.lo[i].y));
+#pragma unroll
+  for (int i = 0; i < PairTw<R>::NQ; ++i) asm volatile("" : "+v"(t.hi[i].x), "+v"(t.hi[i].y));
+  return t;
+}
+
 template <int RB, int DIR>
 __device__ __forceinline__ void passA_apply(float2 (&v)[RB], PassTw<RB> t) {
   constexpr int LO = PassTw<RB>::LO, NHI = PassTw<RB>::NHI;

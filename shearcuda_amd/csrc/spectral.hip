@@ -1194,6 +1194,223 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_dec(const float* _
   row_fwd_from_A<L, R1, R2, C::PADSH>(yz, validA, r, n1, buf, dst, true, rB, k2, ftw);
 }
 
+// ---------------------------------------------------------------------------
+// Lane-pair row transforms, for N = L/2 = R x R (L = 128, 512).  The two-pass
+// row FFT gives each length-R DFT to a lane pair (dft_pair: each lane half of
+// it), so a row takes 2R lanes holding H = R/2 values each instead of R lanes
+// holding R: half the arithmetic per lane on the short critical path of the
+// DCT-domain pyramid's row passes (fewer rows in flight than SIMDs).
+// Thread (r, p, h): row r of the block, pair p, lane h of the pair.
+//   forward pass A: pair p = n1, lane h holds z[n1 + R (2j + h)];
+//     after it, A[n1][k2] for k2 = k + H h, stored at n1 + R k2;
+//   forward pass B: pair p = k2, lane h reads A[2j + h][k2] and ends with
+//     Z[kk], kk = k2 + R (k + H h) (the "kk layout");
+//   inverse stage 1: from the kk layout, a decimation-in-frequency pair
+//     butterfly leaves lane h the odd/even half n1 = 2m + h, stored at n1 + R k2;
+//   inverse stage 2: pair p = n1, lane h reads k2 = 2j + h and ends with
+//     z[n], n = n1 + R (k + H h) (the "n layout").
+template <int L>
+struct PRow {
+  static constexpr int N = L / 2;
+  static constexpr int R = Split<N>::R1, H = R / 2;
+  static constexpr bool ok = Split<N>::R1 == Split<N>::R2 && H >= 2;
+  static constexpr int LANES = 2 * R;
+  static constexpr int ROWS = ok ? 64 / LANES : 1;
+  static constexpr int T = ROWS * LANES;
+  static constexpr int PADSH = ilog2(R);
+  static constexpr int LDS = lds_size<ROWS * N, PADSH>();
+};
+
+// Twiddles of a thread, loaded once: forward pass A (w_N^(n1 k2)), the r2c /
+// c2r factor w_L^kk (w0 = w_L^p times -i for the upper half), inverse stage 1
+// (w_N^(+2 p m) and the lane's extra w_N^(+p h)).
+template <int L>
+struct PRowTw {
+  PairTw<PRow<L>::H> fa;
+  PassTw<PRow<L>::H> ia;
+  float2 fh;  // w_N^(+p h)
+  float2 wh;  // w_L^(p + R H h)
+};
+template <int L>
+__device__ __forceinline__ PRowTw<L> prow_tw(int p, int h, const float2* __restrict__ tw) {
+  using C = PRow<L>;
+  PRowTw<L> t;
+  pair_tw<C::H, 2>(t.fa, p, h, tw);
+  passA_tw<C::H, +1, 4>(t.ia, p, tw);
+  const float2 w2p = tw[2 * p];  // w_N^p
+  t.fh = h ? cconj(w2p) : make_float2(1.f, 0.f);
+  const float2 w0 = tw[p];  // w_L^p; w_L^(R H) = w_L^(L/4) = -i
+  t.wh = h ? make_float2(w0.y, -w0.x) : w0;
+  return t;
+}
+
+// Forward pass A (registers z, n1 = p) then pass B: Z in the kk layout.
+// Starts by writing buf (the caller makes sure earlier reads of it are done).
+template <int L>
+__device__ __forceinline__ void prow_fwd(float2 (&z)[PRow<L>::H], int r, int p, int h, float2* buf,
+                                         const PRowTw<L>& t) {
+  using C = PRow<L>;
+  constexpr int N = C::N, R = C::R, H = C::H;
+  dft_pair<H, -1>(z, h);
+  pair_tw_apply<H, -1>(z, pin(t.fa));
+#pragma unroll
+  for (int k = 0; k < H; ++k) buf[lds_ix<C::PADSH>(r * N + p + R * (k + H * h))] = z[k];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < H; ++j) z[j] = buf[lds_ix<C::PADSH>(r * N + (2 * j + h) + R * p)];
+  dft_pair<H, -1>(z, h);
+}
+
+// Z[(N - kk) mod N] for the thread's kk-layout values (all lanes take part).
+template <int L>
+__device__ __forceinline__ void prow_partner(const float2 (&Z)[PRow<L>::H], int p, int h,
+                                             float2 (&Zm)[PRow<L>::H]) {
+  using C = PRow<L>;
+  constexpr int R = C::R, H = C::H;
+  const int lane = threadIdx.x & 63;
+  const int src = (lane & ~(C::LANES - 1)) + 2 * ((R - p) & (R - 1)) + (1 - h);
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const float2 sh1 = shfl2(Z[H - 1 - k], src);
+    const float2 sh0 = swap_pair(Z[(H - k) & (H - 1)]);
+    Zm[k] = p != 0 ? sh1 : (k == 0 ? Z[0] : sh0);
+  }
+}
+
+// Packed half spectrum of the real row from Z (kk layout), in place.
+template <int L>
+__device__ __forceinline__ void prow_r2c_post(float2 (&Z)[PRow<L>::H], int p, int h, const PRowTw<L>& t) {
+  using C = PRow<L>;
+  constexpr int R = C::R, H = C::H;
+  float2 Zm[H];
+  prow_partner<L>(Z, p, h, Zm);
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const bool k0 = k == 0 && p == 0 && h == 0;
+    const float2 z = Z[k];
+    Z[k] = k0 ? make_float2(z.x + z.y, z.x - z.y) : r2c_post(z, Zm[k], ctw_mul<L / R, -1>(t.wh, k));
+  }
+}
+
+// Inverse of a packed half spectrum P (kk layout) to z (n layout), times L.
+// Starts by writing buf (the caller makes sure earlier reads of it are done).
+template <int L>
+__device__ __forceinline__ void prow_inv(float2 (&P)[PRow<L>::H], int r, int p, int h, float2* buf,
+                                         const PRowTw<L>& t) {
+  using C = PRow<L>;
+  constexpr int N = C::N, R = C::R, H = C::H;
+  float2 Pm[H];
+  prow_partner<L>(P, p, h, Pm);
+  float2 a[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const bool k0 = k == 0 && p == 0 && h == 0;
+    const float2 x = P[k];
+    a[k] = k0 ? make_float2(x.x + x.y, x.x - x.y) : c2r_pre(x, Pm[k], ctw_mul<L / R, -1>(t.wh, k));
+  }
+  // decimation in frequency across the pair: lane 0 keeps x0 + x1, lane 1
+  // (x0 - x1) w_R^{-k}; each then takes its half of the outputs
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const float2 o = swap_pair(a[k]);
+    a[k] = h == 0 ? cadd(a[k], o) : ctw_mul<R, +1>(csub(o, a[k]), k);
+  }
+  passA_apply<H, +1>(a, pin(t.ia));
+#pragma unroll
+  for (int m = 0; m < H; ++m) buf[lds_ix<C::PADSH>(r * N + (2 * m + h) + R * p)] = cmul(a[m], t.fh);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < H; ++j) P[j] = buf[lds_ix<C::PADSH>(r * N + p + R * (2 * j + h))];
+  dft_pair<H, +1>(P, h);
+}
+
+// k_dct_rows_dec with lane-pair row transforms (same grid roles, rows per
+// block PRow<L>::ROWS).
+template <int L>
+__global__ void __launch_bounds__(PRow<L>::T) k_dct_rows_dec_p(const float* __restrict__ X, long long xs,
+                                                                float2* __restrict__ spec, long long ss, int nspec,
+                                                                float* __restrict__ low, long long ls,
+                                                                const float* __restrict__ mask,
+                                                                const float2* __restrict__ tw,
+                                                                const float2* __restrict__ dtw) {
+  using C = PRow<L>;
+  constexpr int N = C::N, R = C::R, H = C::H, ROWS = C::ROWS;
+  __shared__ float2 buf[C::LDS];
+  __shared__ float vrow[ROWS][L];
+  const int tid = threadIdx.x;
+  const int r = tid / C::LANES, l = tid % C::LANES, p = l >> 1, h = l & 1;
+  const int pl = blockIdx.y, b = blockIdx.z;
+  const int row = blockIdx.x * ROWS + r;
+  const float* xr = X + (size_t)b * xs + (size_t)row * L;
+  // z[n] = (x[perm(2n)], x[perm(2n+1)]) for n = p + R (2j + h)
+  float2 z[H];
+  {
+    const float4* x4 = reinterpret_cast<const float4*>(xr);
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int n = p + R * (2 * j + h);
+      if (j < H / 2) {
+        const float4 f = x4[n];
+        z[j] = make_float2(f.x, f.z);
+      } else {
+        const float4 f = x4[N - 1 - n];
+        z[j] = make_float2(f.w, f.y);
+      }
+    }
+  }
+  const PRowTw<L> t = prow_tw<L>(p, h, tw);
+  // e_kk = e^{-i pi kk / 2L}, kk = p + R (k + H h): dtw[p] times e^{-i pi h / 8}
+  // (R H = L/4) times the compile-time e^{-i pi R k / 2L}
+  float2 eh = dtw[p];
+  if (h) eh = cmul(eh, make_float2(0.92387953251128674f, -0.38268343236508977f));
+  const float* mrow = mask + (size_t)pl * L * L + (size_t)row * L;
+  float mk[H], mm[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const int kk = p + R * (k + H * h);
+    mk[k] = mrow[kk];
+    mm[k] = mrow[kk == 0 ? N : L - kk];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  prow_fwd<L>(z, r, p, h, buf, t);
+  prow_r2c_post<L>(z, p, h, t);
+  // mask and D2^-1 pre-twiddle (as k_dct_rows_dec)
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const bool k0 = k == 0 && p == 0 && h == 0;
+    const float2 e = ctw_mul<4 * L / R, -1>(eh, k);
+    const float2 W = cmul(e, z[k]);
+    z[k] = k0 ? make_float2(mk[k] * z[k].x, mm[k] * z[k].y) : cmulc(make_float2(mk[k] * W.x, mm[k] * W.y), e);
+  }
+  __syncthreads();  // pass-B reads of buf are complete
+  prow_inv<L>(z, r, p, h, buf, t);
+  // z[k] = L (v'[2n], v'[2n+1]), n = p + R (k + H h)
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const int n = p + R * (k + H * h);
+    vrow[r][2 * n] = z[k].x;
+    vrow[r][2 * n + 1] = z[k].y;
+  }
+  __syncthreads();
+  // y pairs (y[2n], y[2n+1]) = (v'[n], v'[L-1-n]) for n = p + R (2j + h)
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const int n = p + R * (2 * j + h);
+    z[j] = make_float2(vrow[r][n], vrow[r][L - 1 - n]);
+  }
+  if (pl == nspec) {
+    float2* lr = reinterpret_cast<float2*>(low + (size_t)b * ls + (size_t)row * L);
+#pragma unroll
+    for (int j = 0; j < H; ++j) lr[p + R * (2 * j + h)] = z[j];
+    return;
+  }
+  prow_fwd<L>(z, r, p, h, buf, t);
+  prow_r2c_post<L>(z, p, h, t);
+  float2* dst = spec + (size_t)b * ss + (size_t)pl * L * N + (size_t)row * N;
+#pragma unroll
+  for (int k = 0; k < H; ++k) dst[p + R * (k + H * h)] = z[k];
+}
+
 // Columns of the planes k_dct_rows_dec wrote: D2^-1, then (scale planes) the
 // forward column FFT, in place.  Plane p < nspec: spec plane p (complex
 // columns); p == nspec: the lowpass plane (pairs of real columns).
@@ -1260,8 +1477,12 @@ struct LaunchDct {
       hipLaunchKernelGGL((k_dct2_cols<L, ct1>), dim3(L / 2 / ct1, 1, nimg), dim3(ColCfg<L, ct1>::T), 0, s, x, xs,
                          xwork, ws, tw, dtw);
     } else if (step == 1) {
-      hipLaunchKernelGGL((k_dct_rows_dec<L>), dim3(L / DctRowCfg<L>::ROWS, nspec + 1, nimg), dim3(DctRowCfg<L>::T),
-                         0, s, xwork, ws, spec, ss, nspec, low, ls, mask, tw, dtw);
+      if constexpr (PRow<L>::ok)
+        hipLaunchKernelGGL((k_dct_rows_dec_p<L>), dim3(L / PRow<L>::ROWS, nspec + 1, nimg), dim3(PRow<L>::T), 0, s,
+                           xwork, ws, spec, ss, nspec, low, ls, mask, tw, dtw);
+      else
+        hipLaunchKernelGGL((k_dct_rows_dec<L>), dim3(L / DctRowCfg<L>::ROWS, nspec + 1, nimg),
+                           dim3(DctRowCfg<L>::T), 0, s, xwork, ws, spec, ss, nspec, low, ls, mask, tw, dtw);
     } else {
       hipLaunchKernelGGL((k_dct_cols_dec<L, ct>), dim3(L / 2 / ct, nspec + 1, nimg), dim3(ColCfg<L, ct>::T), 0, s,
                          spec, ss, nspec, low, ls, tw, dtw);
@@ -1566,6 +1787,173 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_out(const float* _
   }
 }
 
+// k_dct_rows_rec with lane-pair row transforms.
+template <int L>
+__global__ void __launch_bounds__(PRow<L>::T) k_dct_rows_rec_p(const float2* __restrict__ Qg, long long qs,
+                                                                SumTable gf, int nspec,
+                                                                const float* __restrict__ low, long long ls,
+                                                                LowThresh lt, float* __restrict__ D, long long ds,
+                                                                const float2* __restrict__ tw,
+                                                                const float2* __restrict__ dtw) {
+  using C = PRow<L>;
+  constexpr int N = C::N, R = C::R, H = C::H, ROWS = C::ROWS;
+  __shared__ float2 buf[C::LDS];
+  __shared__ float2 yrow[ROWS][N];
+  const int tid = threadIdx.x;
+  const int r = tid / C::LANES, l = tid % C::LANES, p = l >> 1, h = l & 1;
+  const int pl = blockIdx.y, b = blockIdx.z;
+  const int row = blockIdx.x * ROWS + r;
+  const PRowTw<L> t = prow_tw<L>(p, h, tw);
+  float2 eh = dtw[p];
+  if (h) eh = cmul(eh, make_float2(0.92387953251128674f, -0.38268343236508977f));
+  float2 z[H];
+  if (pl < nspec) {
+    // the scale's direction-group partials at kk = p + R (k + H h), two groups
+    // per round trip, then the inverse row transform: y pairs in the n layout
+    const int q0 = gf.first[pl], q1 = gf.first[pl + 1];
+    const float2* qrow = Qg + (size_t)b * qs + (size_t)row * N;
+#pragma unroll
+    for (int k = 0; k < H; ++k) z[k] = make_float2(0.f, 0.f);
+    for (int q = q0; q < q1; q += 2) {
+      const int q2 = q + 1 < q1 ? q + 1 : q;
+      const float wgt = q + 1 < q1 ? 1.0f : 0.0f;
+      float2 a[H], c[H];
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        const int kk = p + R * (k + H * h);
+        a[k] = qrow[(size_t)q * L * N + kk];
+        c[k] = qrow[(size_t)q2 * L * N + kk];
+      }
+      __builtin_amdgcn_sched_barrier(0);  // both groups' loads in flight together
+#pragma unroll
+      for (int k = 0; k < H; ++k) z[k] = cadd(cadd(z[k], a[k]), cscale(c[k], wgt));
+    }
+    prow_inv<L>(z, r, p, h, buf, t);
+#pragma unroll
+    for (int k = 0; k < H; ++k) yrow[r][p + R * (k + H * h)] = z[k];
+    __syncthreads();  // also: every read of buf is complete
+    // z[j] = (y[perm(2n)], y[perm(2n+1)]), n = p + R (2j + h)
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int n = p + R * (2 * j + h);
+      const int m = j < H / 2 ? n : N - 1 - n;
+      const float2 a = yrow[r][2 * m], c = yrow[r][2 * m + 1];
+      z[j] = j < H / 2 ? make_float2(a.x, c.x) : make_float2(c.y, a.y);
+    }
+  } else {
+    const float th = lt.mode == 0 ? -1.0f : (lt.mode == 1 ? lt.v : lt.dev[(size_t)b * lt.stride]);
+    const float4* x4 = reinterpret_cast<const float4*>(low + (size_t)b * ls + (size_t)row * L);
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const int n = p + R * (2 * j + h);
+      if (j < H / 2) {
+        const float4 f = x4[n];
+        z[j] = make_float2(hard_keep(f.x, th), hard_keep(f.z, th));
+      } else {
+        const float4 f = x4[N - 1 - n];
+        z[j] = make_float2(hard_keep(f.w, th), hard_keep(f.y, th));
+      }
+    }
+  }
+  prow_fwd<L>(z, r, p, h, buf, t);
+  prow_r2c_post<L>(z, p, h, t);
+  // real DCT-II values: X[kk] = Re(e_kk V[kk]), X[L-kk] = -Im(e_kk V[kk]),
+  // X[0] = V[0], X[L/2] = V[L/2] / sqrt(2)
+  float* drow = D + (size_t)b * ds + (size_t)pl * L * L + (size_t)row * L;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const int kk = p + R * (k + H * h);
+    if (kk == 0) {
+      drow[0] = z[k].x;
+      drow[N] = z[k].y * 0.70710678118654752f;
+    } else {
+      const float2 W = cmul(ctw_mul<4 * L / R, -1>(eh, k), z[k]);
+      drow[kk] = W.x;
+      drow[L - kk] = -W.y;
+    }
+  }
+}
+
+// k_dct_rows_out with lane-pair row transforms.
+template <int L>
+__global__ void __launch_bounds__(PRow<L>::T) k_dct_rows_out_p(const float* __restrict__ D, long long ds,
+                                                                int nplanes, float* __restrict__ x, long long xs,
+                                                                const float2* __restrict__ tw,
+                                                                const float2* __restrict__ dtw) {
+  using C = PRow<L>;
+  constexpr int N = C::N, R = C::R, H = C::H, ROWS = C::ROWS;
+  constexpr int NC = L / 4 / C::LANES;  // 16-byte chunks of a row per lane
+  static_assert(NC * 4 * C::LANES == L, "row chunking");
+  __shared__ float2 buf[C::LDS];
+  __shared__ float xrow[ROWS][L];
+  const int tid = threadIdx.x;
+  const int r = tid / C::LANES, l = tid % C::LANES, p = l >> 1, h = l & 1;
+  const int b = blockIdx.z;
+  const int row = blockIdx.x * ROWS + r;
+  const PRowTw<L> t = prow_tw<L>(p, h, tw);
+  float2 eh = dtw[p];
+  if (h) eh = cmul(eh, make_float2(0.92387953251128674f, -0.38268343236508977f));
+  // the summed row X, three planes per round trip (past the last plane, the
+  // last again at weight 0)
+  {
+    const float4* src = reinterpret_cast<const float4*>(D + (size_t)b * ds + (size_t)row * L);
+    float4 acc[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int q0 = 0; q0 < nplanes; q0 += 3) {
+      float4 a[3][NC];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int q = q0 + j < nplanes ? q0 + j : nplanes - 1;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) a[j][i] = src[(size_t)q * L * L / 4 + l + C::LANES * i];
+      }
+      __builtin_amdgcn_sched_barrier(0);  // the three planes' loads in flight together
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const float wgt = q0 + j < nplanes ? 1.0f : 0.0f;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+          acc[i].x += wgt * a[j][i].x;
+          acc[i].y += wgt * a[j][i].y;
+          acc[i].z += wgt * a[j][i].z;
+          acc[i].w += wgt * a[j][i].w;
+        }
+      }
+    }
+    float4* xr = reinterpret_cast<float4*>(&xrow[r][0]);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) xr[l + C::LANES * i] = acc[i];
+  }
+  __syncthreads();
+  // P'[kk] = e^{i t_kk} (X[kk] - i X[L-kk]) (kk >= 1), P'[0] = (X[0], sqrt(2) X[L/2])
+  float2 z[H];
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const int kk = p + R * (k + H * h);
+    const float* X = xrow[r];
+    z[k] = kk == 0 ? make_float2(X[0], 1.41421356237309505f * X[N])
+                   : cmulc(make_float2(X[kk], -X[L - kk]), ctw_mul<4 * L / R, -1>(eh, k));
+  }
+  prow_inv<L>(z, r, p, h, buf, t);
+  __syncthreads();  // every read of xrow is complete
+  // z[k] = L (v[2n], v[2n+1]), n = p + R (k + H h); x[perm(n)] = v[n]
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const int n = p + R * (k + H * h);
+    xrow[r][2 * n] = z[k].x;
+    xrow[r][2 * n + 1] = z[k].y;
+  }
+  __syncthreads();
+  // (x[2m], x[2m+1]) = (v[m], v[L-1-m])
+  float2* xo = reinterpret_cast<float2*>(x + (size_t)b * xs + (size_t)row * L);
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const int m = p + R * (k + H * h);
+    xo[m] = make_float2(xrow[r][m], xrow[r][L - 1 - m]);
+  }
+}
+
 template <int L>
 struct LaunchDctRec {
   static hipError_t rec(const float2* Qg, long long qs, const int* gfirst, int nspec, const float* low, long long ls,
@@ -1575,14 +1963,22 @@ struct LaunchDctRec {
     if (step == 0) {
       SumTable gf{};
       for (int i = 0; i <= nspec; ++i) gf.first[i] = gfirst[i];
-      hipLaunchKernelGGL((k_dct_rows_rec<L>), dim3(L / DctRowCfg<L>::ROWS, nspec + 1, nimg), dim3(DctRowCfg<L>::T),
-                         0, s, Qg, qs, gf, nspec, low, ls, lt, D, ds, tw, dtw);
+      if constexpr (PRow<L>::ok)
+        hipLaunchKernelGGL((k_dct_rows_rec_p<L>), dim3(L / PRow<L>::ROWS, nspec + 1, nimg), dim3(PRow<L>::T), 0, s,
+                           Qg, qs, gf, nspec, low, ls, lt, D, ds, tw, dtw);
+      else
+        hipLaunchKernelGGL((k_dct_rows_rec<L>), dim3(L / DctRowCfg<L>::ROWS, nspec + 1, nimg),
+                           dim3(DctRowCfg<L>::T), 0, s, Qg, qs, gf, nspec, low, ls, lt, D, ds, tw, dtw);
     } else if (step == 1) {
       hipLaunchKernelGGL((k_dct_cols_rec<L, ct>), dim3(L / 2 / ct, nspec + 1, nimg), dim3(ColCfg<L, ct>::T), 0, s, D,
                          ds, mask, tw, dtw);
     } else {
-      hipLaunchKernelGGL((k_dct_rows_out<L>), dim3(L / DctRowCfg<L>::ROWS, 1, nimg), dim3(DctRowCfg<L>::T), 0, s, D,
-                         ds, nspec + 1, x, xs, tw, dtw);
+      if constexpr (PRow<L>::ok)
+        hipLaunchKernelGGL((k_dct_rows_out_p<L>), dim3(L / PRow<L>::ROWS, 1, nimg), dim3(PRow<L>::T), 0, s, D, ds,
+                           nspec + 1, x, xs, tw, dtw);
+      else
+        hipLaunchKernelGGL((k_dct_rows_out<L>), dim3(L / DctRowCfg<L>::ROWS, 1, nimg), dim3(DctRowCfg<L>::T), 0, s,
+                           D, ds, nspec + 1, x, xs, tw, dtw);
     }
     return hipGetLastError();
   }
