@@ -181,8 +181,8 @@ def _rand_cells(api, shear, L, seed):
     (128, (1,), (16,)),
 ])
 def test_backs_agree(gpu, L, dcomp, dsize):
-    """inverse_shear (plain and thresholded) with the DCT back end against the
-    scale row pass + pyramid levels, on inconsistent random coefficients."""
+    """inverse_shear with the DCT back end against the scale row pass +
+    pyramid levels, on inconsistent random coefficients."""
     from shearcuda_amd import api
     print("case dct back vs pyramid L=%d dcomp=%s" % (L, dcomp), flush=True)
     shear = api.make_dictionary(L) if dcomp is None else api.make_dictionary(L, dcomp=dcomp, dsize=dsize)
@@ -267,6 +267,42 @@ def test_dct_both_denoise_and_batch(gpu):
             assert np.array_equal(out[i].cpu().numpy(), pd.cpu().numpy()), i
             mx, rms = _rel(rt[i].cpu().numpy(), xs[i].cpu().numpy())
             assert mx <= TOL and rms <= TOL, (i, mx, rms)
+    finally:
+        b.close()
+        shear.close()
+
+
+def test_dct_back_thresholds(gpu):
+    """Thresholded reconstruction (host thresholds, and the batch's device
+    table) with a nonzero lowpass threshold: the DCT back end applies the
+    lowpass and band keep rules as the level-by-level path does."""
+    from shearcuda_amd import api
+    L, n = 256, 2
+    print("case dct back thresholds L=%d" % L, flush=True)
+    shear = api.make_dictionary(L)
+    b = api.Batch(shear, n)
+    try:
+        cells = _rand_cells(api, shear, L, 41)
+        th = np.full(shear.nbands, 2.0)
+        th[0] = 25.0  # lowpass threshold that removes part of the lowpass
+        ref = api.inverse_shear_thresh_cuda(cells, th, shear).cpu().numpy()
+        api.set_plan_options(shear, dct_back=True)
+        got = api.inverse_shear_thresh_cuda(cells, th, shear).cpu().numpy()
+        mx, rms = _rel(got, ref)
+        print("host thresholds max-rel %.2e rms-rel %.2e" % (mx, rms), flush=True)
+        assert mx <= TOL and rms <= TOL
+        stack = torch.stack([torch.cat([c.reshape(-1, L, L) for c in _rand_cells(api, shear, L, 50 + i)])
+                             for i in range(n)]).contiguous()
+        thb = torch.tensor(np.stack([th * (1 + 0.5 * i) for i in range(n)]), dtype=torch.float32, device="cuda")
+        outs = []
+        for dct in (False, True):
+            api.set_plan_options(shear, b, dct_back=dct)
+            o = torch.empty((n, L, L), device="cuda")
+            b.inverse_shear(stack, o, thb)
+            outs.append(o.cpu().numpy())
+        mx, rms = _rel(outs[1], outs[0])
+        print("device thresholds max-rel %.2e rms-rel %.2e" % (mx, rms), flush=True)
+        assert mx <= TOL and rms <= TOL
     finally:
         b.close()
         shear.close()
