@@ -361,8 +361,9 @@ class Shearcuda {
   }
 
   // Noise scalars of every band: transform of a unit-variance noise image
-  // (seed 4321), median absolute deviation / 0.6745 per band on the host
-  // (shearcuda.cpp:205-271).
+  // (seed 4321), median absolute deviation / 0.6745 per band in the
+  // reference's float arithmetic (shearcuda.cpp:205-271), computed on the
+  // device by shx_band_mad (exact medians, the same values as the host loop).
   bool computeShearNorm(GpuShearDictionary& shearDict, GpuCellData& temp) const {
     const int L = shearDict.filterLen();
     GpuImage x(gpuFLOAT, L, L);
@@ -371,16 +372,21 @@ class Shearcuda {
     GpuCellData Ct;
     if (!shearTrans(x, shearDict, temp, Ct)) return false;
     const int ns = Ct.numElem();
-    std::vector<float> im((size_t)L * L);
+    int maxd = 1;
+    for (int s = 0; s < ns; ++s) maxd = std::max(maxd, Ct.elem(s)->depth());
+    GpuImage mad(gpuDOUBLE, maxd, 1);
+    std::vector<double> host(maxd);
     double* E = shearDict.norm();
-    for (int s = 0; s < ns; ++s)
-      for (int k = 0; k < Ct.elem(s)->depth(); ++k) {
-        hipMemcpy(im.data(), (float*)Ct.elem(s)->getPtr() + (size_t)k * L * L, sizeof(float) * L * L,
-                  hipMemcpyDeviceToHost);
-        const float med = median(im);
-        for (float& v : im) v = std::fabs(v - med);
-        E[s + k * ns] = median(im) / .6745;
-      }
+    for (int s = 0; s < ns; ++s) {
+      const int depth = Ct.elem(s)->depth();
+      if (!shxOk(shx_band_mad((const float*)Ct.elem(s)->getPtr(), (long long)L * L, (long long)L * L, depth,
+                              SHX_MAD_FP32, (double*)mad.getPtr(), nullptr),
+                 "Shearcuda::computeShearNorm()"))
+        return false;
+      if (hipMemcpy(host.data(), mad.getPtr(), sizeof(double) * depth, hipMemcpyDeviceToHost) != hipSuccess)
+        return false;
+      for (int k = 0; k < depth; ++k) E[s + k * ns] = host[k];
+    }
     return true;
   }
 
@@ -478,14 +484,6 @@ class Shearcuda {
   float elapsedMs() const { return m_ms; }
 
  private:
-  static float median(std::vector<float> v) {
-    const size_t n = v.size();
-    std::nth_element(v.begin(), v.begin() + n / 2, v.end());
-    const float hi = v[n / 2];
-    if (n % 2) return hi;
-    std::nth_element(v.begin(), v.begin() + (n / 2 - 1), v.end());
-    return (hi + v[n / 2 - 1]) / 2;
-  }
   static bool allocCoeff(const GpuShearDictionary& d, GpuCellData& coeff, std::vector<float*>& cells) {
     const int S = d.numScales(), L = d.filterLen();
     if (coeff.numElem() != S + 1) coeff.setNumElem(S + 1);

@@ -246,6 +246,31 @@ int shx_hard_bands_backward(const float* x, const float* gy, float* gx, int nimg
  * norm_cuda's reduction (shearcuda/src_cuda/norm_cuda.cpp:172-215). */
 int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream);
 
+/* Median absolute deviation / 0.6745 of each of nplanes planes (plane i at
+ * planes + i*plane_stride, plane_elems finite floats) into mad[i] (device
+ * FP64), on the device: the per-band step of the noise scalars, which the
+ * reference takes on the host (shearlet_toolbox_1/nsst_scalars.m:36-44;
+ * demo_denoise/shearcuda/shearcuda.cpp:241-268 with its median at :165-201).
+ * The medians are exact order statistics (mean of the two middle values for
+ * an even count).  arith SHX_MAD_FP64: values widened to double (MATLAB /
+ * numpy on the double transform); SHX_MAD_FP32: float arithmetic as the demo,
+ * final division in double.  Each call is bit-identical to that host
+ * computation. */
+#define SHX_MAD_FP64 0
+#define SHX_MAD_FP32 1
+int shx_band_mad(const float* planes, long long plane_elems, long long plane_stride, int nplanes, int arith,
+                 double* mad, void* stream);
+
+/* Noise scalars of a dictionary: the transform of the given noise image
+ * (L x L device FP32, normally unit white noise), then shx_band_mad of every
+ * band, into scalars[0 .. nbands] on the host (lowpass first, then the bands
+ * coarse -> fine); synchronises the stream.  Replaces nsst_scalars(L, shear,
+ * 'maxflat') (shearlet_toolbox_1/nsst_scalars.m:28-45; arith SHX_MAD_FP64) and
+ * the per-band loop of Shearcuda::computeShearNorm
+ * (demo_denoise/shearcuda/shearcuda.cpp:237-268; SHX_MAD_FP32).  Allocates a
+ * coefficient stack for the call. */
+int shx_nsst_scalars(shx_dict* dict, const float* noise, int arith, double* scalars, void* stream);
+
 /* In-place soft threshold sign(v)*max(|v|-t, 0) (softth_cuda.cpp:49-112). */
 int shx_softth(float* x, int64_t n, double t, void* stream);
 

@@ -402,21 +402,48 @@ def com_norm(pfilt, n, shear):
     return E
 
 
-def nsst_scalars_cuda(shear, noise):
+MAD_ARITH = {"fp64": 0, "fp32": 1}
+
+
+def band_mad_cuda(planes, arith="fp64"):
+    """Median absolute deviation / 0.6745 of each plane of a contiguous device
+    stack (first dimension indexes the planes), computed on the device with
+    exact medians (shx_band_mad).  arith "fp64": values widened to double, as
+    numpy / MATLAB on the double transform (nsst_scalars.m:36-44); "fp32":
+    float arithmetic as the demo (demo_denoise/shearcuda/shearcuda.cpp:246-252).
+    Returns a float64 device tensor with one entry per plane."""
+    _require_gpu(planes, "planes")
+    if arith not in MAD_ARITH:
+        raise ValueError("arith must be 'fp64' or 'fp32'")
+    if planes.dtype != torch.float32 or not planes.is_contiguous():
+        raise ValueError("planes must be a contiguous float32 tensor")
+    n = planes.shape[0] if planes.dim() > 0 else 1
+    pe = planes.numel() // max(n, 1)
+    out = torch.empty(n, device=planes.device, dtype=torch.float64)
+    if n:
+        check(lib.shx_band_mad(ctypes.c_void_p(planes.data_ptr()), pe, pe, n, MAD_ARITH[arith],
+                               ctypes.c_void_p(out.data_ptr()), _stream()))
+    return out
+
+
+def nsst_scalars_cuda(shear, noise, arith="fp64"):
     """Per-band MAD/0.6745 noise scalars of the transform of a unit white-noise
-    image (shearlet_toolbox_1/nsst_scalars.m:28-45); the median runs on the
-    host, as demo_denoise/shearcuda/shearcuda.cpp:241-268 does.
+    image (shearlet_toolbox_1/nsst_scalars.m:28-45), all on the device: the
+    transform, then exact medians per band (shx_nsst_scalars).  arith "fp32"
+    gives demo_denoise/shearcuda/shearcuda.cpp:241-268's float medians.
     Returns [s_low, array(D_1), ..., array(D_S)]."""
-    C = shear_trans_cuda(noise, "maxflat", shear)
-
-    def mad(v):
-        v = v.ravel().astype(np.float64)
-        return np.median(np.abs(v - np.median(v))) / 0.6745
-
-    out = [mad(C[0].cpu().numpy())]
-    for c in C[1:]:
-        a = c.cpu().numpy()
-        out.append(np.array([mad(a[k]) for k in range(a.shape[0])]))
+    _require_gpu(noise, "noise")
+    if arith not in MAD_ARITH:
+        raise ValueError("arith must be 'fp64' or 'fp32'")
+    if tuple(noise.shape) != (shear.L, shear.L) or noise.dtype != torch.float32 or not noise.is_contiguous():
+        raise ValueError("noise must be a contiguous %dx%d float32 image" % (shear.L, shear.L))
+    sc = np.zeros(shear.nbands, dtype=np.float64)
+    check(lib.shx_nsst_scalars(shear.handle, ctypes.c_void_p(noise.data_ptr()), MAD_ARITH[arith],
+                               sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _stream()))
+    out, k = [float(sc[0])], 1
+    for d in shear.ndir:
+        out.append(sc[k:k + d].copy())
+        k += d
     return out
 
 

@@ -1253,6 +1253,36 @@ int shx_shear_trans(shx_dict* d, const float* x, float* const* coeff, void* stre
   return run_stages(st);
 }
 
+int shx_nsst_scalars(shx_dict* d, const float* noise, int arith, double* scalars, void* stream) {
+  if (int r = check_dict(d)) return r;
+  if (!noise || !scalars) return fail(SHX_ERR_ARG, "null noise image or output");
+  if (arith != SHX_MAD_FP64 && arith != SHX_MAD_FP32) return fail(SHX_ERR_ARG, "unknown arithmetic %d", arith);
+  const size_t pl = d->plane(), np = 1 + (size_t)d->nbands;
+  hipStream_t s = (hipStream_t)stream;
+  float* stack = nullptr;
+  double* mad = nullptr;
+  hipError_t e = hipMalloc(&stack, sizeof(float) * pl * np);
+  if (e == hipSuccess) e = hipMalloc(&mad, sizeof(double) * np);
+  if (e != hipSuccess) {
+    if (stack) (void)hipFree(stack);
+    return fail(SHX_ERR_ALLOC, "scalar workspace allocation failed: %s", hipGetErrorString(e));
+  }
+  std::vector<float*> cells(d->S + 1);
+  cells[0] = stack;
+  for (int j = 0; j < d->S; ++j) cells[j + 1] = stack + (1 + (size_t)d->map.first[j]) * pl;
+  std::vector<Stage> st;
+  stages_dec(d, d->w, 1, noise, 0, cell_coeffs(d, cells.data()), s, st);
+  int r = run_stages(st);
+  if (r == SHX_OK) r = shx_band_mad(stack, (long long)pl, (long long)pl, (int)np, arith, mad, stream);
+  if (r == SHX_OK && ((e = hipMemcpyAsync(scalars, mad, sizeof(double) * np, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+                      (e = hipStreamSynchronize(s)) != hipSuccess))
+    r = fail(SHX_ERR_HIP, "scalar download failed: %s", hipGetErrorString(e));
+  if (r != SHX_OK) (void)hipStreamSynchronize(s);
+  (void)hipFree(stack);
+  (void)hipFree(mad);
+  return r;
+}
+
 int shx_inverse_shear(shx_dict* d, const float* const* coeff, float* x, void* stream) {
   if (int r = check_dict(d)) return r;
   if (!x) return fail(SHX_ERR_ARG, "null output");

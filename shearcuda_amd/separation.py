@@ -131,17 +131,10 @@ def unrolled_separation(x, th_sh, th_w, plan, h, levels=3, iters=10):
 
 
 def wavelet_scalars(noise, h, levels):
-    """MAD/0.6745 of each wavelet band of a unit white-noise image (median on
-    the host); order: yl, then [lh hl hh] per level."""
+    """MAD/0.6745 of each wavelet band of a unit white-noise image (exact
+    medians on the device, shx_band_mad); order: yl, then [lh hl hh] per level."""
     yl, yh = api.mrdwt_cuda(noise, h, levels)
     m, n = noise.shape[-2:]
-
-    def mad(v):
-        v = v.ravel().astype(np.float64)
-        return np.median(np.abs(v - np.median(v))) / 0.6745
-
-    out = [mad(yl.cpu().numpy())]
-    a = yh.cpu().numpy().reshape(m, 3 * levels, n)
-    for k in range(3 * levels):
-        out.append(mad(a[:, k, :]))
-    return np.array(out)
+    bands = yh.reshape(m, 3 * levels, n).permute(1, 0, 2).contiguous()
+    return np.concatenate([api.band_mad_cuda(yl.reshape(1, m, n).contiguous()).cpu().numpy(),
+                           api.band_mad_cuda(bands).cpu().numpy()])

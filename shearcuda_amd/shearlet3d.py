@@ -237,28 +237,21 @@ def sh_rec(bands_level):
     return bands_level.sum(dim=1)
 
 
-def _mad(v):
-    v = np.asarray(v, dtype=np.float64).ravel()
-    return float(np.median(np.abs(v - np.median(v))) / 0.6745)
-
-
 def nsst_scalars_3d(shear, noise):
     """NsstScalars (DenoiseDemo/NsstScalars.m:1-19) with the noise volume passed
-    in: (bandEst in band order, lowPassEst).  Bands come from the GPU; the
-    medians are taken on the host."""
+    in: (bandEst in band order, lowPassEst).  Bands and their exact medians
+    (MAD/0.6745, shx_band_mad) are computed on the device."""
+    from . import api
     plan = Plan3D(shear, 1, group=8)
     try:
         subs = plan.pyr_dec(noise.reshape(1, *noise.shape).contiguous())
         bands = plan.analysis(subs)
         per_level = shear.split_bands(bands)
-        est = []
-        for lvl in per_level:
-            host = lvl[0].cpu().numpy()
-            est.extend(_mad(b) for b in host)
-        low = _mad(shear.split_subs(subs)[-1][0].cpu().numpy())
+        est = [api.band_mad_cuda(lvl[0].contiguous()).cpu().numpy() for lvl in per_level]
+        low = float(api.band_mad_cuda(shear.split_subs(subs)[-1][0].reshape(1, -1).contiguous()).cpu()[0])
     finally:
         plan.close()
-    return np.array(est), low
+    return np.concatenate(est), low
 
 
 def thresholds_3d(shear, band_est, sigmas, T):
