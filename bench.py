@@ -76,6 +76,24 @@ def plain_setup(L, sigma, seed=0):
     return shear, xg, sc
 
 
+def dictionary_build_times(L):
+    """Seconds to build the default dictionary (spatial filters, spectra,
+    pyramid masks, workspaces) with the spectra computed on the device and on
+    the host."""
+    import torch
+
+    from shearcuda_amd import api
+    out = {}
+    for where in ("device", "host"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        shear = api.make_dictionary(L, spectra=where)
+        torch.cuda.synchronize()
+        out[where] = round(time.perf_counter() - t0, 3)
+        shear.close()
+    return out
+
+
 def profile_plain(shear, xg, sc, sigma, reps, coeffs=False):
     """Per-kernel mean durations of one plain denoise step (HIP events)."""
     import numpy as np
@@ -399,6 +417,7 @@ def main():
         res["plain_1024"] = {"ms_per_image": round(ms1k, 5), "Mpixel_s": round(1024 * 1024 / (ms1k * 1e-3) / 1e6, 2),
                              "Mpixel_s_with_coeffs": round(1024 * 1024 / (ms1kc * 1e-3) / 1e6, 2),
                              "path_roofline_frac": round(400.0 * 1024 * 1024 / (ms1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        res["plain_1024"]["dictionary_build_s"] = dictionary_build_times(1024)
     if not args.no_batched:
         res["batched"] = bench_batched(max(5, args.steps // 20), 2)
     if not args.no_adjoint:

@@ -46,7 +46,8 @@ const char* shx_last_error(void);
 int shx_version(void);
 
 /* Shearlet dictionary: normalised FP32 spectra of the directional filters of
- * every scale (computed in FP64 on the host), FFT twiddles and workspaces.
+ * every scale (computed in FP64 on the device from the spatial filters), FFT
+ * twiddles, the DCT-domain pyramid masks and workspaces.
  * Replaces prepare_shearing_filters_cuda(w_s, L)
  * (shearcuda/src_cuda/prepare_shearing_filters_cuda.cpp:61-249).
  *   L        image side, power of two in [32, 1024]
@@ -57,6 +58,12 @@ int shx_version(void);
  *   shift    SHX_SHIFT_NONE or SHX_SHIFT_CENTER */
 int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s,
                                  int shift, shx_dict** out);
+/* The same with flags.  SHX_DICT_HOST_SPECTRA: spectra by an FP64 FFT on the
+ * host and uploaded, instead of the FP64 direct transform on the device (both
+ * round the same FP64 values once; they agree to FP32 rounding). */
+#define SHX_DICT_HOST_SPECTRA 1
+int shx_prepare_shearing_filters_ex(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s,
+                                    int shift, int flags, shx_dict** out);
 /* Frees a dictionary (clear_shearing_filters_cuda.cpp:51-96). NULL is a no-op. */
 int shx_clear_shearing_filters(shx_dict* dict);
 /* Dictionary shape: L, S and ndir[0..S-1] (ndir may be NULL). */

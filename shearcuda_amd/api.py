@@ -89,15 +89,19 @@ class ShearDict:
             torch.empty((d, L, L), device=dev, dtype=torch.float32) for d in self.ndir]
 
 
-def prepare_shearing_filters_cuda(w_s, L, shift="none"):
+def prepare_shearing_filters_cuda(w_s, L, shift="none", spectra="device"):
     """shear = prepare_shearing_filters_cuda(w_s, L)
     (shearcuda/src_cuda/prepare_shearing_filters_cuda.cpp:61-249).
 
     w_s: list over scales of (D_j, n1_j, n1_j) FP64 arrays.
     shift: 'none' (toolbox convention, nsst_dec2.m:50) or 'center' (the MEX GPU
-    path's circular n1/2 shift, prepare_shearing_filters_cuda.cpp:254)."""
+    path's circular n1/2 shift, prepare_shearing_filters_cuda.cpp:254).
+    spectra: 'device' (FP64 direct transform on the GPU) or 'host' (FP64 FFT
+    on the host, uploaded)."""
     if shift not in _SHIFTS:
         raise ValueError("shift must be 'none' or 'center'")
+    if spectra not in ("device", "host"):
+        raise ValueError("spectra must be 'device' or 'host'")
     w_s = [np.ascontiguousarray(np.asarray(w, dtype=np.float64)) for w in w_s]
     for w in w_s:
         if w.ndim != 3 or w.shape[1] != w.shape[2]:
@@ -109,7 +113,8 @@ def prepare_shearing_filters_cuda(w_s, L, shift="none"):
         *[w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for w in w_s])
     handle = ctypes.c_void_p()
     torch.cuda.init()
-    check(lib.shx_prepare_shearing_filters(int(L), S, ndir, dsize, ptrs, _SHIFTS[shift], ctypes.byref(handle)))
+    check(lib.shx_prepare_shearing_filters_ex(int(L), S, ndir, dsize, ptrs, _SHIFTS[shift],
+                                              1 if spectra == "host" else 0, ctypes.byref(handle)))
     return ShearDict(handle, int(L), list(ndir), list(dsize), shift, w_s, torch.device("cuda", torch.cuda.current_device()))
 
 
@@ -447,9 +452,9 @@ def nsst_scalars_cuda(shear, noise, arith="fp64"):
     return out
 
 
-def make_dictionary(L, dcomp=DEFAULT_DCOMP, dsize=DEFAULT_DSIZE, shift="none"):
+def make_dictionary(L, dcomp=DEFAULT_DCOMP, dsize=DEFAULT_DSIZE, shift="none", spectra="device"):
     """Shortcut: spatial Meyer filters + prepare_shearing_filters_cuda."""
-    return prepare_shearing_filters_cuda(dictionary_filters(dcomp, dsize), L, shift)
+    return prepare_shearing_filters_cuda(dictionary_filters(dcomp, dsize), L, shift, spectra)
 
 
 # ---- HIP graphs ----------------------------------------------------------------

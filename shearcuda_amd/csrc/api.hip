@@ -13,6 +13,7 @@
 
 #include "../../include/shx.h"
 #include "api_err.h"
+#include "dict_dev.h"
 #include "dict_host.h"
 #include "pyramid.h"
 #include "rdwt.h"
@@ -1128,7 +1129,14 @@ int shx_maxflat_filter(int which, double* out, int* len) {
 
 int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s,
                                  int shift, shx_dict** out) {
+  return shx_prepare_shearing_filters_ex(L, nscales, ndir, dsize, w_s, shift, 0, out);
+}
+
+int shx_prepare_shearing_filters_ex(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s,
+                                    int shift, int flags, shx_dict** out) {
   if (!out) return fail(SHX_ERR_ARG, "null output pointer");
+  if (flags & ~SHX_DICT_HOST_SPECTRA) return fail(SHX_ERR_ARG, "unknown dictionary flags 0x%x", flags);
+  const bool host_spectra = (flags & SHX_DICT_HOST_SPECTRA) != 0;
   *out = nullptr;
   if (!is_pow2(L) || L < 32 || L > 1024)
     return fail(SHX_ERR_UNSUPPORTED, "image size %d not supported (power of two in [32, 1024])", L);
@@ -1168,8 +1176,9 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
     return fail(SHX_ERR_HIP, "no HIP device");
   }
   const int N = L / 2;
-  std::vector<float> hm((size_t)nb * L * N * 2), hb((size_t)nb * L * 2), tw((size_t)L * 2);
-  for (int j = 0; j < nscales; ++j) {
+  std::vector<float> hm(host_spectra ? (size_t)nb * L * N * 2 : 0), hb(host_spectra ? (size_t)nb * L * 2 : 0);
+  std::vector<float> tw((size_t)L * 2);
+  for (int j = 0; host_spectra && j < nscales; ++j) {
     std::vector<std::complex<double>> H;
     shx::scale_spectra_fp64(L, ndir[j], dsize[j], w_s[j], shift, H);
     for (size_t i = 0; i < H.size(); ++i)
@@ -1191,7 +1200,7 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
     shx::dct_pyramid_masks(L, nscales, 1, rmask.data());
     shx::dct_twiddles_fp32(L, dtw.data());
   }
-  const size_t bytes_hm = hm.size() * sizeof(float), bytes_hb = hb.size() * sizeof(float);
+  const size_t bytes_hm = sizeof(float2) * (size_t)nb * L * N, bytes_hb = sizeof(float2) * (size_t)nb * L;
   const size_t bytes_dm = dmask.size() * sizeof(float), bytes_dt = dtw.size() * sizeof(float);
   hipError_t e = hipSuccess;
   if ((e = hipMalloc(&d->Hmain, bytes_hm)) != hipSuccess || (e = hipMalloc(&d->Hbeta, bytes_hb)) != hipSuccess ||
@@ -1207,8 +1216,14 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
     return r;
   }
   d->bytes = bytes_hm + bytes_hb + tw.size() * sizeof(float) + 2 * bytes_dm + bytes_dt + d->w.bytes;
-  if ((e = hipMemcpy(d->Hmain, hm.data(), bytes_hm, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(d->Hbeta, hb.data(), bytes_hb, hipMemcpyHostToDevice)) != hipSuccess ||
+  for (int j = 0; !host_spectra && j < nscales; ++j)
+    if (int r = shx::spectra_device(L, ndir[j], dsize[j], w_s[j], shift, d->Hmain + (size_t)d->map.first[j] * L * N,
+                                    d->Hbeta + (size_t)d->map.first[j] * L, j)) {
+      shx_clear_shearing_filters(d);
+      return r;
+    }
+  if ((host_spectra && ((e = hipMemcpy(d->Hmain, hm.data(), bytes_hm, hipMemcpyHostToDevice)) != hipSuccess ||
+                        (e = hipMemcpy(d->Hbeta, hb.data(), bytes_hb, hipMemcpyHostToDevice)) != hipSuccess)) ||
       (e = hipMemcpy(d->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
       (bytes_dm && ((e = hipMemcpy(d->dmask, dmask.data(), bytes_dm, hipMemcpyHostToDevice)) != hipSuccess ||
                     (e = hipMemcpy(d->rmask, rmask.data(), bytes_dm, hipMemcpyHostToDevice)) != hipSuccess ||
