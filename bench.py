@@ -197,6 +197,32 @@ def bench_batched(steps, warmup, mix=BATCH_MIX):
             "schedules": {str(p["L"]): p["sched"] for p in plans}}
 
 
+def bench_separate(L=512, reps=3):
+    """The shipped separation loop (separate.m) with the parameters of
+    separation_demo01_512_gpu.m:37,78: filters 80x80, directions [8 8 16 16],
+    4 wavelet levels, 10 iterations, stop 3, gamma 3, weights [.1 .1 1.5 1.5];
+    seconds per call, setup included."""
+    import numpy as np
+    import torch
+
+    from shearcuda_amd import api, separation
+    from shearcuda_amd.synth import separation_input
+    shear = api.make_dictionary(L, dcomp=(3, 3, 4, 4), dsize=(80, 80, 80, 80))
+    img = torch.from_numpy(separation_input(L, seed=1, sigma=10.0)[2].astype(np.float32)).cuda()
+    run = lambda: separation.separate(img, 4, 10, 3.0, 3.0, [0.1, 0.1, 1.5, 1.5], shear)
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    torch.cuda.synchronize()
+    s = (time.perf_counter() - t0) / reps
+    shear.close()
+    return {"workload": "separate(img, 4, 10, 3, 3, [.1 .1 1.5 1.5], shear) on one 512x512 FP32 image, filters "
+                        "80x80, directions [8,8,16,16]", "s_per_call": round(s, 4),
+            "ms_per_iteration": round(s * 1e3 / 10, 3)}
+
+
 def bench_adjoint(iters=10, nimg=8, L=512, reps=3):
     """Unrolled separation, 8 x 512^2 images: ms per unrolled iteration of the
     forward pass and of forward + backward (adjoint kernels)."""
@@ -422,6 +448,7 @@ def main():
         res["batched"] = bench_batched(max(5, args.steps // 20), 2)
     if not args.no_adjoint:
         res["adjoint"] = bench_adjoint()
+        res["separate"] = bench_separate()
     if not args.no_3d:
         res["volume"] = bench_3d(steps=args.vol_steps, group=args.vol_group, cpu=args.cpu_budget > 0)
     if args.cpu_budget > 0:

@@ -7,6 +7,9 @@ Restates the Rice Wavelet Toolbox code the reference ships:
 * ``mirdwt`` shearcuda/src/mirdwt_r.h:92-218 (bpconv + MIRDWT)
 * ``make_on_filter`` shearcuda/MakeONFilter.m:55-57,111-114,174-177 (Haar,
   Daubechies 4, Symmlet 4), normalised as MakeONFilter.m:270.
+* ``mdwt1``  the decimated transform at one level, shearcuda/src/mdwt_r.h:80-176
+  (fpsconv + MDWT with L = 1), which separate.m:89-101 uses for its noise
+  estimate.
 
 Arrays are indexed like the MATLAB matrices (row, column); yh is m x 3nL with
 level l in columns [3n(l-1), 3nl) as [lh hl hh].  The loops follow the C
@@ -14,7 +17,9 @@ code: each level splits a row (column) into sample_f interleaved
 subsequences and filters each periodically with the undilated filter.
 
 Pinned by: perfect reconstruction mirdwt(mrdwt(x)) = x, the Haar one-level
-closed form, and orthonormality of the filters (tests/test_oracle_rdwt.py).
+closed form, and orthonormality of the filters (tests/test_oracle_rdwt.py);
+mdwt1 by the Haar closed form and energy preservation of the orthonormal
+decimated transform.
 """
 import numpy as np
 
@@ -114,3 +119,34 @@ def mirdwt(yl, yh, h, L):
             x[:, sl] = _bpconv(X[:, sl], XH[:, sl], g0, g1)
         sample_f //= 2
     return x
+
+
+def _fpsconv(x, h0, h1):
+    """fpsconv (mdwt_r.h:80-99): periodic filtering kept at even positions,
+    out[i/2] = sum_j x[i+j] f[lh-1-j] for i = 0, 2, ...; last axis."""
+    lh = len(h0)
+    lx = x.shape[-1]
+    i = np.arange(0, lx, 2)
+    lo = np.zeros(x.shape[:-1] + (lx // 2,))
+    hi = np.zeros_like(lo)
+    for j in range(lh):
+        xs = np.take(x, (i + j) % lx, axis=-1)
+        lo += xs * h0[lh - 1 - j]
+        hi += xs * h1[lh - 1 - j]
+    return lo, hi
+
+
+def mdwt1(x, h):
+    """wc = mdwt(x, h, 1) for a 2D x (MDWT, mdwt_r.h:102-176 with L = 1): rows
+    split into [low | high] halves, then columns into [low ; high]; the
+    finest diagonal band is wc[m/2:, n/2:]."""
+    x = np.asarray(x, dtype=np.float64)
+    m, n = x.shape
+    lh = len(h)
+    h0 = np.array([h[lh - 1 - i] for i in range(lh)])
+    h1 = np.array(h, dtype=np.float64).copy()
+    h1[0::2] = -h1[0::2]
+    lo, hi = _fpsconv(x, h0, h1)
+    y = np.concatenate([lo, hi], axis=1)
+    lo, hi = _fpsconv(y.T, h0, h1)
+    return np.concatenate([lo, hi], axis=1).T

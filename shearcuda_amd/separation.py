@@ -138,3 +138,72 @@ def wavelet_scalars(noise, h, levels):
     bands = yh.reshape(m, 3 * levels, n).permute(1, 0, 2).contiguous()
     return np.concatenate([api.band_mad_cuda(yl.reshape(1, m, n).contiguous()).cpu().numpy(),
                            api.band_mad_cuda(bands).cpu().numpy()])
+
+
+def tv_correction_cuda(x, gamma, haar=None):
+    """TVCorrection_cuda (shearcuda/separate.m:272-278): one-level undecimated
+    Haar transform, soft threshold of its detail bands, inverse."""
+    from .wavelets import MakeONFilter
+    haar = MakeONFilter("Haar") if haar is None else haar
+    ll, wc = api.mrdwt_cuda(x, haar, 1)
+    api.softth_cuda(wc, gamma)
+    return api.mirdwt_cuda(ll, wc, haar, 1)
+
+
+def separate(img, level2, itermax, stop, gamma, coeff, shear, trace=None, info=None):
+    """[C, P] = separate(img, level2, itermax, stop, gamma, coeff, 0, shear)
+    (shearcuda/separate.m:1-257, GPU branch): curve part C by hard-thresholded
+    shearlets with a Haar TV correction, point part P by the hard-thresholded
+    undecimated Symmlet-4 wavelet, on an exponentially decreasing threshold.
+
+    Every transform, threshold and reduction is a shx call; torch supplies
+    the buffers and the elementwise sums of the iteration.  The noise level
+    sigma is the MAD of the finest diagonal band of a one-level decimated D4
+    transform (separate.m:86-104, computed on the host in the reference); that
+    band is the undecimated band at even rows and columns, so it is taken from
+    shx_mrdwt and its median from shx_band_mad.
+
+    trace, if given, is called after every iteration with a dict holding the
+    iteration's inputs (part1_in, part2_in, delta) and results (part1, part2),
+    the thresholded shearlet cells (coeff) and wavelet details (yh).  info, if
+    a dict, receives the schedule: sigma, deltamax, lambda and the two
+    maxima (buff) deltamax is the smaller of.  Returns (C, P)."""
+    from .wavelets import MakeONFilter
+    L = shear.L
+    if tuple(img.shape) != (L, L):
+        raise ValueError("image must be %dx%d" % (L, L))
+    if itermax < 2:
+        raise ValueError("itermax must be at least 2 (the schedule divides by 1 - itermax)")
+    qmf = MakeONFilter("Symmlet", 4)
+    haar = MakeONFilter("Haar")
+    ones = np.ones(shear.nscales + 1)
+    E = api.com_norm("maxflat", (L, L), shear)                                   # :69
+    x = api.re_weight_band_cuda(img.contiguous(), coeff, shear)                  # :74
+    _, yh1 = api.mrdwt_cuda(x, MakeONFilter("Daubechies", 4), 1)                 # :81-101
+    hh = yh1[0::2, 2 * L::2].contiguous().reshape(1, -1)
+    sigma = float(api.band_mad_cuda(hh).cpu()[0])                                # :103-104
+    stop_t = stop * sigma                                                        # :105
+    Ct = api.shear_trans_cuda(x, "maxflat", shear)                               # :116-117
+    b1 = api.thresh_cuda(Ct, 0.0, 2, E, ones, 1)
+    Ctw = list(api.mrdwt_cuda(x, qmf, level2))                                   # :127-128
+    b2 = api.thresh_cuda(Ctw, 0.0, 1, E, np.zeros(1), 1)                         # :131, max |Ctw{2}|
+    delta = min(b1, b2)                                                          # :133
+    lam = (delta / stop_t) ** (1.0 / (1 - itermax))                              # :135
+    part1 = torch.zeros_like(x)
+    part2 = torch.zeros_like(x)
+    for it in range(itermax):
+        residual = x - part1 - part2                                             # :164-166
+        api.shear_trans_cuda(part1 + residual, "maxflat", shear, Ct)             # :175-177
+        api.thresh_cuda(Ct, 1.4 * delta, 2, E, ones, 0)
+        p1 = tv_correction_cuda(api.inverse_shear_cuda(Ct, "maxflat", shear), gamma, haar)  # :187-191
+        yl, yh = api.mrdwt_cuda(part2 + residual, qmf, level2)                   # :206-212
+        api.thresh_cuda([yl, yh], 1.4 * delta, 1, E, np.zeros(1), 0)
+        p2 = api.mirdwt_cuda(yl, yh, qmf, level2)
+        if trace is not None:
+            trace({"iter": it, "delta": delta, "part1_in": part1, "part2_in": part2, "part1": p1, "part2": p2,
+                   "coeff": [c.clone() for c in Ct], "yh": yh})
+        part1, part2 = p1, p2
+        delta *= lam                                                             # :252
+    if info is not None:
+        info.update({"sigma": sigma, "deltamax": min(b1, b2), "lambda": lam, "buff": (b1, b2)})
+    return part1, part2

@@ -1,0 +1,67 @@
+"""The shipped separation loop (separation.separate, shearcuda/separate.m) on
+the GPU against the FP64 oracle restatement (oracle/separation.py).
+
+Setup: the noise level, the starting threshold and the schedule must agree to
+1e-5.  Iterations: each traced GPU iteration is replayed by the oracle from the
+GPU's own input state (parts and delta), under the GPU's hard-threshold
+decisions (its kept coefficients), as the thresholded-output rule asks; the
+parts must then agree to 1e-5 of max |x| (the parts are pieces of the
+re-weighted image x, which fixes their scale; early parts can be near zero),
+and the number of decisions the oracle would have taken differently is
+reported.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import nsst as o
+from oracle import separation as osep
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.mark.parametrize("L,dsize,level2,itermax", [(256, (80, 80, 80, 80), 4, 4), (128, (32, 32, 16, 16), 3, 3)])
+def test_separate_matches_oracle_under_gpu_masks(gpu, L, dsize, level2, itermax):
+    from shearcuda_amd import api, separation
+    from shearcuda_amd.synth import separation_input
+    print("case separate L=%d dsize=%s level2=%d itermax=%d" % (L, dsize, level2, itermax), flush=True)
+    stop, gamma, coeff, dcomp = 3.0, 3.0, [0.1, 0.1, 1.5, 1.5], (3, 3, 4, 4)
+    strict = L >= 256
+    _, _, img = separation_input(L, seed=9, sigma=10.0)
+    img = img.astype(np.float32)
+    shear = api.make_dictionary(L, dcomp=dcomp, dsize=dsize)
+    H = o.make_dictionary(L, dcomp=dcomp, dsize=dsize)
+    steps, info = [], {}
+
+    def keep(d):
+        steps.append({"delta": d["delta"], "p1_in": d["part1_in"].cpu().numpy(), "p2_in": d["part2_in"].cpu().numpy(),
+                      "p1": d["part1"].cpu().numpy(), "p2": d["part2"].cpu().numpy(),
+                      "keep_sh": [c.cpu().numpy() != 0 for c in d["coeff"]], "keep_w": d["yh"].cpu().numpy() != 0})
+
+    try:
+        C, P = separation.separate(torch.from_numpy(img).cuda(), level2, itermax, stop, gamma, coeff, shear,
+                                   trace=keep, info=info)
+    finally:
+        shear.close()
+    st = osep.separate_setup(img.astype(np.float64), level2, itermax, stop, coeff, H, strict=strict)
+    for k in ("sigma", "deltamax", "lambda"):
+        e = abs(info[k] / st[k] - 1)
+        print("%s gpu %.9g oracle %.9g rel %.1e" % (k, info[k], st[k], e), flush=True)
+        assert e < TOL, k
+    assert len(steps) == itermax
+    x = st["x"]
+    scale = np.abs(x).max()
+    total_flips = 0
+    for i, s in enumerate(steps):
+        p1, p2, flips = osep.separate_iter(x, s["p1_in"].astype(np.float64), s["p2_in"].astype(np.float64), s["delta"],
+                                           st["E"], H, level2, gamma, s["keep_sh"], s["keep_w"], strict=strict)
+        e1 = np.abs(s["p1"] - p1).max() / scale
+        e2 = np.abs(s["p2"] - p2).max() / scale
+        total_flips += flips
+        print("iter %d: curve part %.2e, point part %.2e of max|x|, kept %d + %d, flips %d" %
+              (i, e1, e2, sum(int(k.sum()) for k in s["keep_sh"]), int(s["keep_w"].sum()), flips), flush=True)
+        assert e1 < TOL and e2 < TOL, (i, e1, e2)
+    assert np.array_equal(C.cpu().numpy(), steps[-1]["p1"]) and np.array_equal(P.cpu().numpy(), steps[-1]["p2"])
+    print("threshold decisions that differ from the oracle's own: %d" % total_flips, flush=True)

@@ -45,3 +45,30 @@ def test_product_filters_match_oracle():
     from shearcuda_amd.wavelets import MakeONFilter
     for name, par in [("Haar", 0), ("Daubechies", 4), ("Symmlet", 4)]:
         assert np.abs(MakeONFilter(name, par) - w.make_on_filter(name, par)).max() < 1e-15
+
+
+@pytest.mark.parametrize("name,par", [("Haar", 0), ("Daubechies", 4), ("Symmlet", 4)])
+def test_mdwt1_orthonormal(name, par):
+    """The one-level decimated transform of an orthonormal filter preserves
+    energy (a wrong phase, index or sign breaks it)."""
+    x = np.random.default_rng(3).standard_normal((32, 16))
+    wc = w.mdwt1(x, w.make_on_filter(name, par))
+    assert wc.shape == x.shape
+    assert abs(np.sum(wc * wc) / np.sum(x * x) - 1) < 1e-10
+
+
+def test_mdwt1_haar_closed_form():
+    x = np.random.default_rng(4).standard_normal((8, 8))
+    wc = w.mdwt1(x, w.make_on_filter("Haar"))
+    a, b, c, d = x[0::2, 0::2], x[0::2, 1::2], x[1::2, 0::2], x[1::2, 1::2]
+    assert np.allclose(wc[:4, :4], (a + b + c + d) / 2, atol=1e-15)
+    assert np.allclose(wc[4:, 4:], (a - b - c + d) / 2, atol=1e-15)
+
+
+def test_mdwt1_is_mrdwt_at_even_samples():
+    """The decimated finest diagonal band is the undecimated one at even rows
+    and columns (what the GPU separation uses for its noise level)."""
+    x = np.random.default_rng(5).standard_normal((16, 16))
+    h = w.make_on_filter("Daubechies", 4)
+    _, yh = w.mrdwt(x, h, 1)
+    assert np.array_equal(w.mdwt1(x, h)[8:, 8:], yh[0::2, 32::2])
