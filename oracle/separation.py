@@ -68,14 +68,18 @@ def tv_correction(x, gamma):
     return w.mirdwt(ll, soft(wc, gamma), haar, 1)
 
 
-def separate_setup(img, level2, itermax, stop, coeff, H, strict=True):
+def separate_setup(img, level2, itermax, stop, coeff, H, strict=True, wavelet=("Symmlet", 4), delta_max=0.0):
     """separate.m:56-133: the pre-emphasised image, the band norms E, the
-    noise level sigma and the threshold schedule (deltamax, lambda)."""
+    noise level sigma and the threshold schedule (deltamax, lambda).
+    delta_max > 0 is separateHairlineCrack.m:63-66's fixed start (sigma 2.5)."""
     img = np.asarray(img, dtype=np.float64)
     N = img.shape[0]
-    qmf = w.make_on_filter("Symmlet", 4)
+    qmf = w.make_on_filter(*wavelet)
     E = o.com_norm(H, N, strict=strict)                                # :69
     x = o.re_weight_band(img, coeff, strict=strict)                    # :76
+    if delta_max > 0:
+        lam = (delta_max / (stop * 2.5)) ** (1.0 / (1 - itermax))
+        return {"x": x, "E": E, "sigma": 2.5, "deltamax": delta_max, "lambda": lam, "buff": (None, None)}
     wc = w.mdwt1(x, w.make_on_filter("Daubechies", 4))                 # :81-101
     sigma = mad(wc[N // 2:N // 2 + N // 2, N // 2:N // 2 + N // 2])    # :103-104
     stop_t = stop * sigma                                              # :105
@@ -88,14 +92,15 @@ def separate_setup(img, level2, itermax, stop, coeff, H, strict=True):
     return {"x": x, "E": E, "sigma": sigma, "deltamax": dmax, "lambda": lam, "buff": (b1, b2)}
 
 
-def separate_iter(x, part1, part2, delta, E, H, level2, gamma, keep_sh=None, keep_w=None, strict=True):
+def separate_iter(x, part1, part2, delta, E, H, level2, gamma, keep_sh=None, keep_w=None, strict=True,
+                  wavelet=("Symmlet", 4)):
     """One outer iteration of separate.m:162-252: the residual is taken once,
     then the curve part (k = 1) and the wavelet part (k = 2) are re-estimated
     from it.  keep_sh (list of boolean arrays shaped like the coefficient
     cells) and keep_w (boolean, shaped like yh) replace the hard-threshold
     decisions when given.  Returns (part1, part2, flips) with flips the
     number of decisions that differ from this function's own."""
-    qmf = w.make_on_filter("Symmlet", 4)
+    qmf = w.make_on_filter(*wavelet)
     residual = x - (part1 + part2)                                     # :166
     Ra = part1 + residual                                              # :172
     Ct = o.nsst_dec2(Ra, H, strict=strict)                             # :183

@@ -168,29 +168,56 @@ def separate(img, level2, itermax, stop, gamma, coeff, shear, trace=None, info=N
     the thresholded shearlet cells (coeff) and wavelet details (yh).  info, if
     a dict, receives the schedule: sigma, deltamax, lambda and the two
     maxima (buff) deltamax is the smaller of.  Returns (C, P)."""
+    C, P, _, _ = _separate(img, level2, itermax, stop, gamma, coeff, shear, ("Symmlet", 4), 0.0, None, trace, info)
+    return C, P
+
+
+def separate_hairline_crack(img, level2, param, shear, E=None, trace=None, info=None):
+    """[C, P, residual, Ct] = separateHairlineCrack(img, level2, param, 0,
+    shear, 0, E) (crack_detection/separateHairlineCrack.m): separate() with
+    param.numIter / stop / gamma / coeff, an optional wavelet
+    (param.wavelet_name / wavelet_val, default Symmlet 4), an optional fixed
+    starting threshold param.deltaMax > 0 (then sigma is taken as 2.5 and no
+    transform estimates it), and band norms E passed in.  residual is the last
+    iteration's residual and Ct its thresholded shearlet cells."""
+    wav = (param.get("wavelet_name", "Symmlet"), param.get("wavelet_val", 4))
+    return _separate(img, level2, int(param["numIter"]), float(param["stop"]), float(param["gamma"]),
+                     param["coeff"], shear, wav, float(param.get("deltaMax", 0.0)), E, trace, info)
+
+
+def _separate(img, level2, itermax, stop, gamma, coeff, shear, wavelet, delta_max, E, trace, info):
     from .wavelets import MakeONFilter
     L = shear.L
     if tuple(img.shape) != (L, L):
         raise ValueError("image must be %dx%d" % (L, L))
     if itermax < 2:
         raise ValueError("itermax must be at least 2 (the schedule divides by 1 - itermax)")
-    qmf = MakeONFilter("Symmlet", 4)
+    qmf = MakeONFilter(*wavelet)
     haar = MakeONFilter("Haar")
     ones = np.ones(shear.nscales + 1)
-    E = api.com_norm("maxflat", (L, L), shear)                                   # :69
+    if E is None:
+        E = api.com_norm("maxflat", (L, L), shear)                               # separate.m:69
     x = api.re_weight_band_cuda(img.contiguous(), coeff, shear)                  # :74
-    _, yh1 = api.mrdwt_cuda(x, MakeONFilter("Daubechies", 4), 1)                 # :81-101
-    hh = yh1[0::2, 2 * L::2].contiguous().reshape(1, -1)
-    sigma = float(api.band_mad_cuda(hh).cpu()[0])                                # :103-104
-    stop_t = stop * sigma                                                        # :105
-    Ct = api.shear_trans_cuda(x, "maxflat", shear)                               # :116-117
-    b1 = api.thresh_cuda(Ct, 0.0, 2, E, ones, 1)
-    Ctw = list(api.mrdwt_cuda(x, qmf, level2))                                   # :127-128
-    b2 = api.thresh_cuda(Ctw, 0.0, 1, E, np.zeros(1), 1)                         # :131, max |Ctw{2}|
-    delta = min(b1, b2)                                                          # :133
+    Ct = shear.alloc_coeffs()
+    if delta_max > 0:                                                            # separateHairlineCrack.m:63-66
+        sigma, b1, b2 = 2.5, None, None
+        stop_t = stop * sigma
+        delta = delta_max
+    else:
+        _, yh1 = api.mrdwt_cuda(x, MakeONFilter("Daubechies", 4), 1)             # separate.m:81-101
+        hh = yh1[0::2, 2 * L::2].contiguous().reshape(1, -1)
+        sigma = float(api.band_mad_cuda(hh).cpu()[0])                            # :103-104
+        stop_t = stop * sigma                                                    # :105
+        api.shear_trans_cuda(x, "maxflat", shear, Ct)                            # :116-117
+        b1 = api.thresh_cuda(Ct, 0.0, 2, E, ones, 1)
+        Ctw = list(api.mrdwt_cuda(x, qmf, level2))                               # :127-128
+        b2 = api.thresh_cuda(Ctw, 0.0, 1, E, np.zeros(1), 1)                     # :131, max |Ctw{2}|
+        delta = min(b1, b2)                                                      # :133
+    dmax = delta
     lam = (delta / stop_t) ** (1.0 / (1 - itermax))                              # :135
     part1 = torch.zeros_like(x)
     part2 = torch.zeros_like(x)
+    residual = x
     for it in range(itermax):
         residual = x - part1 - part2                                             # :164-166
         api.shear_trans_cuda(part1 + residual, "maxflat", shear, Ct)             # :175-177
@@ -205,5 +232,5 @@ def separate(img, level2, itermax, stop, gamma, coeff, shear, trace=None, info=N
         part1, part2 = p1, p2
         delta *= lam                                                             # :252
     if info is not None:
-        info.update({"sigma": sigma, "deltamax": min(b1, b2), "lambda": lam, "buff": (b1, b2)})
-    return part1, part2
+        info.update({"sigma": sigma, "deltamax": dmax, "lambda": lam, "buff": (b1, b2)})
+    return part1, part2, residual, Ct

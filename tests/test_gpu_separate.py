@@ -65,3 +65,54 @@ def test_separate_matches_oracle_under_gpu_masks(gpu, L, dsize, level2, itermax)
         assert e1 < TOL and e2 < TOL, (i, e1, e2)
     assert np.array_equal(C.cpu().numpy(), steps[-1]["p1"]) and np.array_equal(P.cpu().numpy(), steps[-1]["p2"])
     print("threshold decisions that differ from the oracle's own: %d" % total_flips, flush=True)
+
+
+def test_hairline_crack_variant_matches_oracle(gpu):
+    """separateHairlineCrack: a fixed starting threshold (param.deltaMax, sigma
+    taken as 2.5), the Daubechies-4 wavelet, band norms passed in; same
+    replay under the GPU's decisions."""
+    from shearcuda_amd import api, separation
+    from shearcuda_amd.synth import separation_input
+    L, level2 = 128, 3
+    param = {"numIter": 3, "stop": 3.0, "gamma": 2.0, "coeff": [0.1, 1.0, 1.5], "deltaMax": 300.0,
+             "wavelet_name": "Daubechies", "wavelet_val": 4}
+    print("case hairline crack variant L=%d" % L, flush=True)
+    _, _, img = separation_input(L, seed=4, sigma=10.0)
+    img = img.astype(np.float32)
+    shear = api.make_dictionary(L)
+    H = o.make_dictionary(L)
+    steps, info = [], {}
+
+    def keep(d):
+        steps.append({"delta": d["delta"], "p1_in": d["part1_in"].cpu().numpy(), "p2_in": d["part2_in"].cpu().numpy(),
+                      "p1": d["part1"].cpu().numpy(), "p2": d["part2"].cpu().numpy(),
+                      "keep_sh": [c.cpu().numpy() != 0 for c in d["coeff"]], "keep_w": d["yh"].cpu().numpy() != 0})
+
+    try:
+        E = api.com_norm("maxflat", (L, L), shear)
+        C, P, residual, Ct = separation.separate_hairline_crack(torch.from_numpy(img).cuda(), level2, param, shear,
+                                                                E=E, trace=keep, info=info)
+        Ct = [c.cpu().numpy() for c in Ct]
+    finally:
+        shear.close()
+    wav = ("Daubechies", 4)
+    st = osep.separate_setup(img.astype(np.float64), level2, 3, 3.0, param["coeff"], H, strict=False, wavelet=wav,
+                             delta_max=300.0)
+    assert info["sigma"] == 2.5 and info["deltamax"] == 300.0
+    assert abs(info["lambda"] / st["lambda"] - 1) < 1e-12
+    x = st["x"]
+    scale = np.abs(x).max()
+    for i, s in enumerate(steps):
+        p1, p2, flips = osep.separate_iter(x, s["p1_in"].astype(np.float64), s["p2_in"].astype(np.float64), s["delta"],
+                                           st["E"], H, level2, param["gamma"], s["keep_sh"], s["keep_w"],
+                                           strict=False, wavelet=wav)
+        e1 = np.abs(s["p1"] - p1).max() / scale
+        e2 = np.abs(s["p2"] - p2).max() / scale
+        print("iter %d: %.2e %.2e flips %d" % (i, e1, e2, flips), flush=True)
+        assert e1 < TOL and e2 < TOL, (i, e1, e2)
+    last = steps[-1]
+    assert np.array_equal(C.cpu().numpy(), last["p1"]) and np.array_equal(P.cpu().numpy(), last["p2"])
+    # residual of the last iteration and its thresholded cells
+    res_o = x - (last["p1_in"].astype(np.float64) + last["p2_in"].astype(np.float64))
+    assert np.abs(residual.cpu().numpy() - res_o).max() / scale < TOL
+    assert all(np.array_equal(a != 0, b) for a, b in zip(Ct, last["keep_sh"]))
