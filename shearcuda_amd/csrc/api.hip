@@ -109,7 +109,20 @@ struct ThreshTable {
   float invE[64];
 };
 
-__global__ void k_hard_thresh(ThreshTable t, int opt, unsigned int* maxbits) {
+// Sum (or max) over a 256-thread block, returned to every thread: one wave
+// shuffle tree, then the four wave results through LDS.  Reductions below
+// combine one value per block with an atomic, not one per wave, so a call's
+// atomics to its single accumulator stay few.
+template <class T, class Op>
+__device__ __forceinline__ T block_reduce256(T v, Op op) {
+  __shared__ T part[4];
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return op(op(part[0], part[1]), op(part[2], part[3]));
+}
+
+__global__ void __launch_bounds__(256) k_hard_thresh(ThreshTable t, int opt, unsigned int* maxbits) {
   const int e = blockIdx.y;
   float* p = t.ptr[e];
   const long long n = t.len[e];
@@ -121,9 +134,8 @@ __global__ void k_hard_thresh(ThreshTable t, int opt, unsigned int* maxbits) {
     m = fmaxf(m, fabsf(v));
   }
   if (opt) {
-    m *= t.invE[e];
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, __float_as_uint(m));
+    m = block_reduce256(m * t.invE[e], [](float a, float b) { return fmaxf(a, b); });
+    if (threadIdx.x == 0) atomicMax(maxbits, __float_as_uint(m));
   }
 }
 
@@ -135,26 +147,25 @@ __global__ void k_softth(float* x, long long n, float t) {
   }
 }
 
-__global__ void k_norm_pow(const float* x, long long n, float p, double* out) {
+__global__ void __launch_bounds__(256) k_norm_pow(const float* x, long long n, float p, double* out) {
   float s = 0.f;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const float a = fabsf(x[i]);
     s += (p == 2.f) ? a * a : (p == 1.f ? a : powf(a, p));
   }
-  double d = s;
-  for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
-  if ((threadIdx.x & 63) == 0) atomicAdd(out, d);
+  const double d = block_reduce256((double)s, [](double u, double v) { return u + v; });
+  if (threadIdx.x == 0) atomicAdd(out, d);
 }
 
-__global__ void k_norm_pow_diff(const float* a, const float* b, long long n, float p, double* out) {
+__global__ void __launch_bounds__(256) k_norm_pow_diff(const float* a, const float* b, long long n, float p,
+                                                       double* out) {
   float s = 0.f;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const float v = fabsf(a[i] - b[i]);
     s += (p == 2.f) ? v * v : (p == 1.f ? v : powf(v, p));
   }
-  double d = s;
-  for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
-  if ((threadIdx.x & 63) == 0) atomicAdd(out, d);
+  const double d = block_reduce256((double)s, [](double u, double v) { return u + v; });
+  if (threadIdx.x == 0) atomicAdd(out, d);
 }
 
 // Counter-based Gaussian noise: element i draws two uniforms from a
@@ -1621,7 +1632,8 @@ int shx_thresh(float* const* cells, int ncells, const int* cell_planes, const in
       t.invE[i] = invE[b + i];
       mx = std::max(mx, len[b + i]);
     }
-    const int blocks = (int)std::max<long long>(1, std::min<long long>(1024, (mx + 255) / 256));
+    // about 16 elements per thread: with opt, one atomic per block
+    const int blocks = (int)std::max<long long>(1, std::min<long long>(1024, (mx + 4095) / 4096));
     hipLaunchKernelGGL(k_hard_thresh, dim3(blocks, t.n), dim3(256), 0, s, t, opt == 1 ? 1 : 0, dmax);
     SHX_HIP(hipGetLastError());
   }
@@ -1848,7 +1860,7 @@ int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream)
   double* acc = nullptr;
   SHX_HIP(hipMalloc(&acc, sizeof(double)));
   SHX_HIP(hipMemsetAsync(acc, 0, sizeof(double), s));
-  const int blocks = (int)std::max<long long>(1, std::min<long long>(2048, (n + 255) / 256));
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(1024, (n + 4095) / 4096));
   if (n > 0) hipLaunchKernelGGL(k_norm_pow, dim3(blocks), dim3(256), 0, s, x, (long long)n, (float)p, acc);
   SHX_HIP(hipGetLastError());
   SHX_HIP(hipMemcpyAsync(out, acc, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1863,7 +1875,7 @@ int shx_norm_pow_diff(const float* a, const float* b, int64_t n, double p, doubl
   double* acc = nullptr;
   SHX_HIP(hipMalloc(&acc, sizeof(double)));
   SHX_HIP(hipMemsetAsync(acc, 0, sizeof(double), s));
-  const int blocks = (int)std::max<long long>(1, std::min<long long>(2048, (n + 255) / 256));
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(1024, (n + 4095) / 4096));
   if (n > 0) hipLaunchKernelGGL(k_norm_pow_diff, dim3(blocks), dim3(256), 0, s, a, b, (long long)n, (float)p, acc);
   SHX_HIP(hipGetLastError());
   SHX_HIP(hipMemcpyAsync(out, acc, sizeof(double), hipMemcpyDeviceToHost, s));
