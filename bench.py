@@ -224,11 +224,23 @@ def bench_separate(L=512, reps=10):
         run()
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
+    native = lambda: separation.separate_native(img, 4, 10, 3.0, 3.0, [0.1, 0.1, 1.5, 1.5], shear)
+    for _ in range(2):
+        native()
+    tn = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        native()
+        torch.cuda.synchronize()
+        tn.append(time.perf_counter() - t0)
     shear.close()
     s = statistics.median(times)
+    sn = statistics.median(tn)
     return {"workload": "separate(img, 4, 10, 3, 3, [.1 .1 1.5 1.5], shear) on one 512x512 FP32 image, filters "
                         "80x80, directions [8,8,16,16]", "s_per_call": round(s, 4),
-            "s_per_call_min": round(min(times), 4), "ms_per_iteration": round(s * 1e3 / 10, 3)}
+            "s_per_call_min": round(min(times), 4), "ms_per_iteration": round(s * 1e3 / 10, 3),
+            "native_s_per_call": round(sn, 4), "native_ms_per_iteration": round(sn * 1e3 / 10, 3)}
 
 
 def bench_adjoint(iters=10, nimg=8, L=512, reps=3):

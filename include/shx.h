@@ -278,6 +278,27 @@ int shx_band_mad(const float* planes, long long plane_elems, long long plane_str
  * coefficient stack for the call. */
 int shx_nsst_scalars(shx_dict* dict, const float* noise, int arith, double* scalars, void* stream);
 
+/* The shipped image separation loop, [C, P] = separate(img, level2, itermax,
+ * stop, gamma, coeff, 0, shear) (shearcuda/separate.m:41-257, GPU branch),
+ * run natively over one workspace allocated per call:
+ *   img       L x L device image (L = the dictionary's size)
+ *   level2    levels of the undecimated wavelet of the point part
+ *   coeff     ncoeff re_weight_band weights (ncoeff - 1 pyramid levels)
+ *   qmf       lqmf orthonormal taps of that wavelet (host FP64; separate.m
+ *             uses Symmlet 4)
+ *   delta_max > 0: fixed starting threshold with sigma = 2.5
+ *             (crack_detection/separateHairlineCrack.m:63-66); else the noise
+ *             level and deltamax are estimated as separate.m:81-133 does
+ *   E         band norms, column-major (S+1) x E_cols host FP64, or NULL to
+ *             compute them as com_norm.m:38-53 does
+ *   C, P      L x L device outputs (curve part, point part)
+ *   info      NULL or 3 doubles: sigma, deltamax, lambda.
+ * The same library calls in the same order as the Python mirror
+ * separation.separate; synchronises the stream before returning. */
+int shx_separate(shx_dict* dict, const float* img, int level2, int itermax, double stop, double gamma,
+                 const double* coeff, int ncoeff, const double* qmf, int lqmf, double delta_max, const double* E,
+                 int E_cols, float* C, float* P, double* info, void* stream);
+
 /* In-place soft threshold sign(v)*max(|v|-t, 0) (softth_cuda.cpp:49-112). */
 int shx_softth(float* x, int64_t n, double t, void* stream);
 

@@ -139,6 +139,27 @@ __global__ void __launch_bounds__(256) k_hard_thresh(ThreshTable t, int opt, uns
   }
 }
 
+// Residual step of the separation loop: r = (x - a) - b, ra = a + r, rb = b + r.
+__global__ void k_sep_residual(const float* __restrict__ x, const float* __restrict__ a, const float* __restrict__ b,
+                               float* __restrict__ r, float* __restrict__ ra, float* __restrict__ rb, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float v = (x[i] - a[i]) - b[i];
+    r[i] = v;
+    ra[i] = a[i] + v;
+    rb[i] = b[i] + v;
+  }
+}
+
+// The finest diagonal band of a one-level undecimated transform (yh: L rows
+// of [lh hl hh], 3L columns) at even rows and columns: the decimated band.
+__global__ void k_even_hh(const float* __restrict__ yh, int L, float* __restrict__ out) {
+  const int h = L / 2;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < h * h; i += gridDim.x * blockDim.x) {
+    const int r = i / h, c = i - r * h;
+    out[i] = yh[(size_t)(2 * r) * 3 * L + 2 * L + 2 * c];
+  }
+}
+
 __global__ void k_softth(float* x, long long n, float t) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const float v = x[i];
@@ -1972,3 +1993,154 @@ int shx_softth(float* x, int64_t n, double t, void* stream) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The shipped separation loop, shearcuda/separate.m:41-257 (GPU branch) and
+// crack_detection/separateHairlineCrack.m, run natively: the same sequence of
+// library calls as the Python mirror, over one workspace allocated per call.
+namespace {
+
+std::vector<double> on_filter(const double* f, int n) {
+  double ss = 0.0;
+  for (int i = 0; i < n; ++i) ss += f[i] * f[i];
+  const double r = std::sqrt(ss);
+  std::vector<double> o(n);
+  for (int i = 0; i < n; ++i) o[i] = f[i] / r;
+  return o;
+}
+
+}  // namespace
+
+extern "C" int shx_separate(shx_dict* d, const float* img, int level2, int itermax, double stop, double gamma,
+                            const double* coeff, int ncoeff, const double* qmf, int lqmf, double delta_max,
+                            const double* E, int E_cols, float* C, float* P, double* info, void* stream) {
+  if (int r = check_dict(d)) return r;
+  if (!img || !coeff || !qmf || !C || !P) return fail(SHX_ERR_ARG, "null image, weights, filter or output");
+  const int L = d->L, S = d->S, nb = d->nbands;
+  if (itermax < 2) return fail(SHX_ERR_ARG, "itermax must be at least 2 (the schedule divides by 1 - itermax)");
+  if (level2 < 1 || L % (1 << level2) != 0) return fail(SHX_ERR_ARG, "invalid number of wavelet levels %d", level2);
+  if (lqmf < 2 || lqmf % 2) return fail(SHX_ERR_ARG, "wavelet filter length %d must be even", lqmf);
+  if (ncoeff <= 1 || ncoeff > 10) return fail(SHX_ERR_ARG, "need 2..10 band weights");
+  int maxd = 1;
+  for (int j = 0; j < S; ++j) maxd = std::max(maxd, d->ndir[j]);
+  if (E && E_cols < maxd) return fail(SHX_ERR_ARG, "E has %d columns, the dictionary needs %d", E_cols, maxd);
+  hipStream_t s = (hipStream_t)stream;
+  const size_t pl = (size_t)L * L, nyh = (size_t)3 * L * level2 * L;
+  // workspace: x r ra rb t yl ll | stack (1+nb planes) | yh | wc (3 planes) | work | hh
+  const size_t nwork = std::max<size_t>(2 * pl, (size_t)(ncoeff + 2) * pl);
+  const size_t total = 7 * pl + (1 + (size_t)nb) * pl + nyh + 3 * pl + nwork + pl / 4;
+  float* ws = nullptr;
+  double* dmad = nullptr;
+  SHX_HIP(hipMalloc(&ws, sizeof(float) * total));
+  if (hipMalloc(&dmad, sizeof(double)) != hipSuccess) {
+    (void)hipFree(ws);
+    return fail(SHX_ERR_ALLOC, "separation workspace allocation failed");
+  }
+  float *x = ws, *r = ws + pl, *ra = ws + 2 * pl, *rb = ws + 3 * pl, *t = ws + 4 * pl, *yl = ws + 5 * pl,
+        *ll = ws + 6 * pl, *stack = ws + 7 * pl;
+  float* yh = stack + (1 + (size_t)nb) * pl;
+  float* wc = yh + nyh;
+  float* work = wc + 3 * pl;
+  float* hh = work + nwork;
+  std::vector<float*> cells(S + 1);
+  std::vector<int> cplanes(S + 1);
+  std::vector<int64_t> celems(S + 1, (int64_t)pl);
+  cells[0] = stack;
+  cplanes[0] = 1;
+  for (int j = 0; j < S; ++j) {
+    cells[j + 1] = stack + (1 + (size_t)d->map.first[j]) * pl;
+    cplanes[j + 1] = d->ndir[j];
+  }
+  float* wcells[2] = {yl, yh};
+  int wplanes[2] = {1, 1};
+  int64_t welems[2] = {(int64_t)pl, (int64_t)nyh};
+  const std::vector<double> ones(S + 1, 1.0);
+  const double zero = 0.0;
+  const double haar_raw[2] = {1.0, 1.0}, d4_raw[4] = {.482962913145, .836516303738, .224143868042, -.129409522551};
+  const std::vector<double> haar = on_filter(haar_raw, 2), d4 = on_filter(d4_raw, 4);
+  std::vector<double> Eh;
+  int rc = SHX_OK;
+  auto ok = [&](int v) { return (rc = v) == SHX_OK; };
+  const int eb = (int)std::min<size_t>(4096, (pl + 255) / 256);
+  double sigma = 0.0, dmax = 0.0, lam = 0.0;
+  do {
+    // band norms (com_norm.m:38-53): transform of an impulse of height L at the centre
+    if (!E) {
+      Eh.assign((size_t)(S + 1) * maxd, 0.0);
+      std::vector<float> imp(pl, 0.f);
+      imp[(size_t)(L / 2) * L + L / 2] = (float)L;
+      if (hipMemcpyAsync(t, imp.data(), sizeof(float) * pl, hipMemcpyHostToDevice, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        rc = fail(SHX_ERR_HIP, "impulse upload failed");
+        break;
+      }
+      if (!ok(shx_shear_trans(d, t, cells.data(), stream))) break;
+      for (int c = 0; c <= S && rc == SHX_OK; ++c)
+        for (int w = 0; w < cplanes[c]; ++w) {
+          double ss = 0.0;
+          if (!ok(shx_norm_pow(cells[c] + (size_t)w * pl, (int64_t)pl, 2.0, &ss, stream))) break;
+          Eh[c + (size_t)w * (S + 1)] = std::sqrt(ss) / std::sqrt((double)pl);
+        }
+      if (rc != SHX_OK) break;
+      E = Eh.data();
+      E_cols = maxd;
+    }
+    if (!ok(shx_re_weight_band(img, L, coeff, ncoeff, x, work, stream))) break;     // separate.m:74
+    if (delta_max > 0) {                                                              // separateHairlineCrack.m:63-66
+      sigma = 2.5;
+      dmax = delta_max;
+    } else {
+      if (!ok(shx_mrdwt(x, L, L, d4.data(), 4, 1, t, yh, work, 1, stream))) break;   // separate.m:81-101
+      hipLaunchKernelGGL(k_even_hh, dim3((unsigned)std::min<size_t>(1024, (pl / 4 + 255) / 256)), dim3(256), 0, s,
+                         yh, L, hh);
+      if (!ok(shx_band_mad(hh, (long long)(pl / 4), (long long)(pl / 4), 1, SHX_MAD_FP64, dmad, stream))) break;
+      if (hipMemcpyAsync(&sigma, dmad, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        rc = fail(SHX_ERR_HIP, "noise level download failed");
+        break;
+      }
+      double b1 = 0.0, b2 = 0.0;
+      if (!ok(shx_shear_trans(d, x, cells.data(), stream))) break;                   // :116-117
+      if (!ok(shx_thresh(cells.data(), S + 1, cplanes.data(), celems.data(), 0.0, 2, E, S + 1, E_cols, ones.data(),
+                         S + 1, 1, &b1, stream)))
+        break;
+      if (!ok(shx_mrdwt(x, L, L, qmf, lqmf, level2, yl, yh, work, 1, stream))) break; // :127-131
+      if (!ok(shx_thresh(wcells, 2, wplanes, welems, 0.0, 1, E, S + 1, E_cols, &zero, 1, 1, &b2, stream))) break;
+      dmax = std::min(b1, b2);                                                        // :133
+    }
+    const double stop_t = stop * sigma;
+    lam = std::pow(dmax / stop_t, 1.0 / (1 - itermax));                              // :135
+    double delta = dmax;
+    if (hipMemsetAsync(C, 0, sizeof(float) * pl, s) != hipSuccess ||
+        hipMemsetAsync(P, 0, sizeof(float) * pl, s) != hipSuccess) {
+      rc = fail(SHX_ERR_HIP, "clearing the parts failed");
+      break;
+    }
+    for (int it = 0; it < itermax && rc == SHX_OK; ++it) {
+      hipLaunchKernelGGL(k_sep_residual, dim3(eb), dim3(256), 0, s, x, C, P, r, ra, rb, (long long)pl);  // :164-172
+      if (!ok(shx_shear_trans(d, ra, cells.data(), stream))) break;                  // :175-177
+      if (!ok(shx_thresh(cells.data(), S + 1, cplanes.data(), celems.data(), 1.4 * delta, 2, E, S + 1, E_cols,
+                         ones.data(), S + 1, 0, nullptr, stream)))
+        break;
+      if (!ok(shx_inverse_shear(d, cells.data(), t, stream))) break;                 // :187
+      if (!ok(shx_mrdwt(t, L, L, haar.data(), 2, 1, ll, wc, work, 1, stream))) break; // :189, TVCorrection_cuda
+      if (!ok(shx_softth(wc, (int64_t)(3 * pl), gamma, stream))) break;
+      if (!ok(shx_mirdwt(ll, wc, L, L, haar.data(), 2, 1, C, work, 1, stream))) break;
+      if (!ok(shx_mrdwt(rb, L, L, qmf, lqmf, level2, yl, yh, work, 1, stream))) break; // :206-212
+      if (!ok(shx_thresh(wcells, 2, wplanes, welems, 1.4 * delta, 1, E, S + 1, E_cols, &zero, 1, 0, nullptr, stream)))
+        break;
+      if (!ok(shx_mirdwt(yl, yh, L, L, qmf, lqmf, level2, P, work, 1, stream))) break;
+      delta *= lam;                                                                    // :252
+    }
+  } while (false);
+  if (rc == SHX_OK && hipGetLastError() != hipSuccess) rc = fail(SHX_ERR_HIP, "separation kernel launch failed");
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(ws);
+  (void)hipFree(dmad);
+  if (rc == SHX_OK && info) {
+    info[0] = sigma;
+    info[1] = dmax;
+    info[2] = lam;
+  }
+  return rc;
+}

@@ -234,3 +234,34 @@ def _separate(img, level2, itermax, stop, gamma, coeff, shear, wavelet, delta_ma
     if info is not None:
         info.update({"sigma": sigma, "deltamax": dmax, "lambda": lam, "buff": (b1, b2)})
     return part1, part2, residual, Ct
+
+
+def separate_native(img, level2, itermax, stop, gamma, coeff, shear, wavelet=("Symmlet", 4), delta_max=0.0, E=None,
+                    info=None):
+    """The same loop as separate() / separate_hairline_crack() run inside the
+    library (shx_separate): one call, one workspace, no per-step host work.
+    Returns (C, P); info, if a dict, receives sigma, deltamax and lambda."""
+    from .wavelets import MakeONFilter
+    api._require_gpu(img, "img")
+    L = shear.L
+    if tuple(img.shape) != (L, L) or img.dtype != torch.float32:
+        raise ValueError("image must be a %dx%d float32 tensor" % (L, L))
+    img = img.contiguous()
+    qmf = np.ascontiguousarray(MakeONFilter(*wavelet), dtype=np.float64)
+    coeff = np.ascontiguousarray(np.asarray(coeff, dtype=np.float64).ravel())
+    dp = ctypes.POINTER(ctypes.c_double)
+    if E is not None:
+        Ef = np.asfortranarray(np.asarray(E, dtype=np.float64))
+        Eflat = np.ascontiguousarray(Ef.ravel(order="F"))
+        eptr, ecols = Eflat.ctypes.data_as(dp), Ef.shape[1]
+    else:
+        Eflat, eptr, ecols = None, None, 0
+    C = torch.empty_like(img)
+    P = torch.empty_like(img)
+    inf = np.zeros(3)
+    check(lib.shx_separate(shear.handle, _vp(img), int(level2), int(itermax), float(stop), float(gamma),
+                           coeff.ctypes.data_as(dp), coeff.size, qmf.ctypes.data_as(dp), qmf.size, float(delta_max),
+                           eptr, ecols, _vp(C), _vp(P), inf.ctypes.data_as(dp), _stream()))
+    if info is not None:
+        info.update({"sigma": float(inf[0]), "deltamax": float(inf[1]), "lambda": float(inf[2])})
+    return C, P

@@ -116,3 +116,45 @@ def test_hairline_crack_variant_matches_oracle(gpu):
     res_o = x - (last["p1_in"].astype(np.float64) + last["p2_in"].astype(np.float64))
     assert np.abs(residual.cpu().numpy() - res_o).max() / scale < TOL
     assert all(np.array_equal(a != 0, b) for a, b in zip(Ct, last["keep_sh"]))
+
+
+@pytest.mark.parametrize("variant", ["separate", "hairline"])
+def test_native_loop_equals_python_loop(gpu, variant):
+    """shx_separate (the loop inside the library) against the Python mirror:
+    the same calls in the same order, so the parts are bit-identical when
+    both use the same band norms; the norms it computes itself match
+    com_norm's to FP32 reduction rounding."""
+    from shearcuda_amd import api, separation
+    from shearcuda_amd.synth import separation_input
+    L = 256
+    print("case native separation loop %s L=%d" % (variant, L), flush=True)
+    _, _, img = separation_input(L, seed=6, sigma=10.0)
+    img = torch.from_numpy(img.astype(np.float32)).cuda()
+    shear = api.make_dictionary(L, dcomp=(3, 3, 4, 4), dsize=(32, 32, 32, 32))
+    try:
+        E = api.com_norm("maxflat", (L, L), shear)
+        ip, inn = {}, {}
+        if variant == "separate":
+            Cp, Pp = separation.separate(img, 3, 4, 3.0, 3.0, [0.1, 0.1, 1.5, 1.5], shear, info=ip)
+            Cn, Pn = separation.separate_native(img, 3, 4, 3.0, 3.0, [0.1, 0.1, 1.5, 1.5], shear, E=E, info=inn)
+        else:
+            param = {"numIter": 3, "stop": 3.0, "gamma": 2.0, "coeff": [0.1, 1.0, 1.5], "deltaMax": 300.0,
+                     "wavelet_name": "Daubechies", "wavelet_val": 4}
+            Cp, Pp, _, _ = separation.separate_hairline_crack(img, 3, param, shear, E=E, info=ip)
+            Cn, Pn = separation.separate_native(img, 3, 3, 3.0, 2.0, param["coeff"], shear, wavelet=("Daubechies", 4),
+                                                delta_max=300.0, E=E, info=inn)
+        for k in ("sigma", "deltamax", "lambda"):
+            assert ip[k] == inn[k], (k, ip[k], inn[k])
+        assert np.array_equal(Cp.cpu().numpy(), Cn.cpu().numpy())
+        assert np.array_equal(Pp.cpu().numpy(), Pn.cpu().numpy())
+        print("%s: bit-identical with shared norms" % variant, flush=True)
+        if variant == "separate":
+            # norms computed inside the library: the same reductions, summed in
+            # another order, so at most a rare threshold decision can differ
+            Cs, Ps = separation.separate_native(img, 3, 4, 3.0, 3.0, [0.1, 0.1, 1.5, 1.5], shear)
+            e = max(np.abs(Cs.cpu().numpy() - Cp.cpu().numpy()).max(),
+                    np.abs(Ps.cpu().numpy() - Pp.cpu().numpy()).max())
+            print("own norms: max diff %.2e" % e, flush=True)
+            assert e <= 1e-2 * float(img.abs().max())
+    finally:
+        shear.close()
