@@ -150,6 +150,17 @@ __global__ void k_sep_residual(const float* __restrict__ x, const float* __restr
   }
 }
 
+// Sum of squares of each plane (one block per plane, FP64 accumulation):
+// the band norms of com_norm.m in one launch, in a fixed order.
+__global__ void __launch_bounds__(256) k_plane_sumsq(const float* __restrict__ planes, long long pl,
+                                                     double* __restrict__ out) {
+  const float* p = planes + (size_t)blockIdx.x * pl;
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < pl; i += 256) s += (double)p[i] * (double)p[i];
+  const double d = block_reduce256(s, [](double u, double v) { return u + v; });
+  if (threadIdx.x == 0) out[blockIdx.x] = d;
+}
+
 // The finest diagonal band of a one-level undecimated transform (yh: L rows
 // of [lh hl hh], 3L columns) at even rows and columns: the decimated band.
 __global__ void k_even_hh(const float* __restrict__ yh, int L, float* __restrict__ out) {
@@ -2075,13 +2086,17 @@ extern "C" int shx_separate(shx_dict* d, const float* img, int level2, int iterm
         break;
       }
       if (!ok(shx_shear_trans(d, t, cells.data(), stream))) break;
-      for (int c = 0; c <= S && rc == SHX_OK; ++c)
-        for (int w = 0; w < cplanes[c]; ++w) {
-          double ss = 0.0;
-          if (!ok(shx_norm_pow(cells[c] + (size_t)w * pl, (int64_t)pl, 2.0, &ss, stream))) break;
-          Eh[c + (size_t)w * (S + 1)] = std::sqrt(ss) / std::sqrt((double)pl);
-        }
-      if (rc != SHX_OK) break;
+      // the stack is contiguous: lowpass, then the bands scale by scale
+      double* dsq = reinterpret_cast<double*>(wc);  // 3 planes of floats hold 1 + nb doubles
+      hipLaunchKernelGGL(k_plane_sumsq, dim3(1 + nb), dim3(256), 0, s, stack, (long long)pl, dsq);
+      std::vector<double> sq(1 + (size_t)nb);
+      if (hipMemcpyAsync(sq.data(), dsq, sizeof(double) * sq.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        rc = fail(SHX_ERR_HIP, "band norm download failed");
+        break;
+      }
+      for (int c = 0, k = 0; c <= S; ++c)
+        for (int w = 0; w < cplanes[c]; ++w, ++k) Eh[c + (size_t)w * (S + 1)] = std::sqrt(sq[k]) / std::sqrt((double)pl);
       E = Eh.data();
       E_cols = maxd;
     }

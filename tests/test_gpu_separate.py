@@ -149,12 +149,13 @@ def test_native_loop_equals_python_loop(gpu, variant):
         assert np.array_equal(Pp.cpu().numpy(), Pn.cpu().numpy())
         print("%s: bit-identical with shared norms" % variant, flush=True)
         if variant == "separate":
-            # norms computed inside the library: the same reductions, summed in
-            # another order, so at most a rare threshold decision can differ
+            # norms computed inside the library (one FP64 reduction per band,
+            # where com_norm sums FP32 partials): the norms differ by rounding,
+            # so at most a rare threshold decision can differ
             Cs, Ps = separation.separate_native(img, 3, 4, 3.0, 3.0, [0.1, 0.1, 1.5, 1.5], shear)
-            e = max(np.abs(Cs.cpu().numpy() - Cp.cpu().numpy()).max(),
-                    np.abs(Ps.cpu().numpy() - Pp.cpu().numpy()).max())
-            print("own norms: max diff %.2e" % e, flush=True)
-            assert e <= 1e-2 * float(img.abs().max())
+            e = max(np.linalg.norm(Cs.cpu().numpy() - Cp.cpu().numpy()) / np.linalg.norm(Cp.cpu().numpy()),
+                    np.linalg.norm(Ps.cpu().numpy() - Pp.cpu().numpy()) / np.linalg.norm(Pp.cpu().numpy()))
+            print("own norms: relative l2 diff %.2e" % e, flush=True)
+            assert e < 1e-3
     finally:
         shear.close()
