@@ -48,7 +48,21 @@ def cpu_baseline(L, sigma, budget_s):
             break
     return {"value": round(L * L * n / el / 1e6, 4), "unit": "Mpixel/s", "cores": 1, "kind": "port",
             "sample": "%d image(s) of %dx%d through the FP64 numpy oracle (dec + nsst_HT + rec), %.1f s" % (
-                n, L, L, el)}
+                n, L, L, el), "host": host_cpu()}
+
+
+def host_cpu():
+    """The host the CPU baseline ran on: logical CPUs and the model name."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu": model}
 
 
 def _timed(fn, steps, warmup):
