@@ -97,6 +97,8 @@ def dictionary_build_times(L):
     import torch
 
     from shearcuda_amd import api
+    for where in ("device", "host"):  # warm-up: module loading and first allocations
+        api.make_dictionary(64, spectra=where).close()
     out = {}
     for where in ("device", "host"):
         torch.cuda.synchronize()

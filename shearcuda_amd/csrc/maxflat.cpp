@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <vector>
 #include <algorithm>
 
@@ -266,14 +267,21 @@ void dct_pyramid_masks(int L, int nscales, int pair, float* out) {
   const Full& f1 = M.full[pair == 0 ? 1 : 3];  // h1 / g1
   const size_t pe = (size_t)L * L;
   const double inv = 1.0 / ((double)L * L);
+  // the 2S level responses are independent: one host thread each
+  std::vector<std::vector<double>> H0(nscales), H1(nscales);
+  {
+    std::vector<std::thread> th;
+    for (int lev = 0; lev < nscales; ++lev) {
+      th.emplace_back([&, lev] { H1[lev] = dct_response(f1, 1 << lev, L); });
+      th.emplace_back([&, lev] { H0[lev] = dct_response(f0, 1 << lev, L); });
+    }
+    for (auto& t : th) t.join();
+  }
   std::vector<double> P(pe, 1.0);
   for (int lev = 0; lev < nscales; ++lev) {
-    const int m = 1 << lev;
-    const std::vector<double> H1 = dct_response(f1, m, L);
     float* o = out + (size_t)(nscales - 1 - lev) * pe;
-    for (size_t i = 0; i < pe; ++i) o[i] = (float)(H1[i] * P[i] * inv);
-    const std::vector<double> H0 = dct_response(f0, m, L);
-    for (size_t i = 0; i < pe; ++i) P[i] *= H0[i];
+    for (size_t i = 0; i < pe; ++i) o[i] = (float)(H1[lev][i] * P[i] * inv);
+    for (size_t i = 0; i < pe; ++i) P[i] *= H0[lev][i];
   }
   float* o = out + (size_t)nscales * pe;
   for (size_t i = 0; i < pe; ++i) o[i] = (float)(P[i] * inv);
