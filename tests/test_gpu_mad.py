@@ -70,7 +70,7 @@ def test_band_mad_unaligned_planes(gpu):
     assert np.array_equal(got, np.array([_mad64(r) for r in a]))
 
 
-@pytest.mark.parametrize("L", [64, 256, 512])
+@pytest.mark.parametrize("L", [64, 256, 512, 1024])
 def test_nsst_scalars_on_device(gpu, L):
     """shx_nsst_scalars: the transform of the noise image, then exact medians,
     equal to numpy on the same coefficients; close to the FP64 oracle's."""
