@@ -26,6 +26,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def path_flops_per_px(L):
+    """Fixed FLOP model of the 2D denoise per pixel (maxflat level pyramid
+    8480, 104 real FFTs at 2.5 log2(L^2) each, spectral products 289,
+    threshold and accumulate 100): the VALU counterpart of the 400 B/px byte
+    model.  Implementation-independent, so the DCT-domain pyramid is priced
+    as the level pyramid it replaces."""
+    import math
+    return 8480 + 104 * 2.5 * math.log2(L * L) + 289 + 100
+
+
+def valu_frac(L, npx, seconds):
+    return round(path_flops_per_px(L) * npx / seconds / 1e12 / FP32_PEAK_TFLOPS, 4)
 METRIC = "NSST dec+thresh+rec Mpixel/s @512² FP32, batched vs plain; adjoint ms/iter"
 
 
@@ -210,6 +225,8 @@ def bench_batched(steps, warmup, mix=BATCH_MIX):
             "Mpixel_s_with_coeffs": round(npx * steps / elc / 1e6, 2),
             "plain_image_by_image_Mpixel_s": round(plain, 2), "speedup_vs_plain": round(batched / plain, 3),
             "path_roofline_frac": round(400.0 * npx / (el / steps) / 1e9 / HBM_PEAK_GBS, 4),
+            "path_valu_frac": round(sum(path_flops_per_px(L) * L * L * n for L, n in mix) / (el / steps) / 1e12
+                                    / FP32_PEAK_TFLOPS, 4),
             "schedules": {str(p["L"]): p["sched"] for p in plans}}
 
 
@@ -472,13 +489,17 @@ def main():
         "roofline": roof,
         "path_roofline": {"bytes_per_step": path_bytes, "achieved": round(path_gbs, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
+        "path_valu_roofline": {"flop_per_step": round(path_flops_per_px(L) * L * L),
+                               "achieved": round(path_flops_per_px(L) * L * L / (ms * 1e-3) / 1e12, 2),
+                               "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": valu_frac(L, L * L, ms * 1e-3)},
         "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()} for s in stages],
     }
     if L == 512 and not args.no_plain_1024:
         ms1k, ms1kc, _, _ = bench_plain(1024, sigma, max(20, args.steps // 4), 5)
         res["plain_1024"] = {"ms_per_image": round(ms1k, 5), "Mpixel_s": round(1024 * 1024 / (ms1k * 1e-3) / 1e6, 2),
                              "Mpixel_s_with_coeffs": round(1024 * 1024 / (ms1kc * 1e-3) / 1e6, 2),
-                             "path_roofline_frac": round(400.0 * 1024 * 1024 / (ms1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                             "path_roofline_frac": round(400.0 * 1024 * 1024 / (ms1k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "path_valu_frac": valu_frac(1024, 1024 * 1024, ms1k * 1e-3)}
         res["plain_1024"]["dictionary_build_s"] = dictionary_build_times(1024)
     if not args.no_batched:
         res["batched"] = bench_batched(max(5, args.steps // 20), 2)
