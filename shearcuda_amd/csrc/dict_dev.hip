@@ -14,8 +14,8 @@
 // frequencies, then a column DFT of those n1 rows at every row frequency, both
 // against an FP64 twiddle table indexed by (frequency * position) mod L.  The
 // normalisation needs every direction of a bin, so each thread of the column
-// pass owns one column frequency of UB row frequencies and evaluates their
-// directions twice (norm, then output) rather than holding them.
+// pass owns one bin and evaluates its directions twice (norm, then output)
+// rather than holding them.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -48,26 +48,17 @@ __global__ void __launch_bounds__(ST) k_spec_rows(const double* __restrict__ w, 
   G[((size_t)k * n1 + r) * NC + v] = make_double2(re, im);
 }
 
-// Row frequencies per block of the column pass: each block reads the row
-// DFTs G once for UB row frequencies (G is D x n1 x (L/2+1), up to a few MB
-// per scale, and every block needs all of it).
-constexpr int UB = 8;
-
-// H_k(u0 + j, v) for j < UB: sum_r e^{-2 pi i (u0+j) rr / L} G[k][r][v].
-__device__ __forceinline__ void col_dft(const double2* __restrict__ Gk, int n1, int NC, int v, int u0, int L, int s,
-                                        const double2* __restrict__ tw, double2 (&h)[UB]) {
-#pragma unroll
-  for (int j = 0; j < UB; ++j) h[j] = make_double2(0.0, 0.0);
+__device__ __forceinline__ double2 col_dft(const double2* __restrict__ Gk, int n1, int NC, int v, int u, int L, int s,
+                                           const double2* __restrict__ tw) {
+  double re = 0.0, im = 0.0;
   for (int r = 0; r < n1; ++r) {
     const long long rr = (r - s + L) & (L - 1);
+    const double2 t = tw[(u * rr) & (L - 1)];
     const double2 g = Gk[(size_t)r * NC + v];
-#pragma unroll
-    for (int j = 0; j < UB; ++j) {
-      const double2 t = tw[((u0 + j) * rr) & (L - 1)];
-      h[j].x += t.x * g.x - t.y * g.y;
-      h[j].y += t.x * g.y + t.y * g.x;
-    }
+    re += t.x * g.x - t.y * g.y;
+    im += t.x * g.y + t.y * g.x;
   }
+  return make_double2(re, im);
 }
 
 __global__ void __launch_bounds__(ST) k_spec_cols(const double2* __restrict__ G, int D, int n1, int L, int s,
@@ -75,50 +66,38 @@ __global__ void __launch_bounds__(ST) k_spec_cols(const double2* __restrict__ G,
                                                   float2* __restrict__ beta_out, int* __restrict__ bad) {
   const int N = L / 2, NC = N + 1;
   const int v = blockIdx.x * ST + threadIdx.x;  // packed slot; slot 0 also carries v = N
-  const int u0 = blockIdx.y * UB;
+  const int u = blockIdx.y;
   if (v >= N) return;
   const size_t gk = (size_t)n1 * NC;
-  double d0[UB], dn[UB];
-#pragma unroll
-  for (int j = 0; j < UB; ++j) d0[j] = dn[j] = 0.0;
-  double2 h[UB], g[UB];
+  double d0 = 0.0, dn = 0.0;
   for (int k = 0; k < D; ++k) {
-    col_dft(G + k * gk, n1, NC, v, u0, L, s, tw, h);
-#pragma unroll
-    for (int j = 0; j < UB; ++j) d0[j] += h[j].x * h[j].x + h[j].y * h[j].y;
+    const double2 h = col_dft(G + k * gk, n1, NC, v, u, L, s, tw);
+    d0 += h.x * h.x + h.y * h.y;
     if (v == 0) {
-      col_dft(G + k * gk, n1, NC, N, u0, L, s, tw, g);
-#pragma unroll
-      for (int j = 0; j < UB; ++j) dn[j] += g[j].x * g[j].x + g[j].y * g[j].y;
+      const double2 g = col_dft(G + k * gk, n1, NC, N, u, L, s, tw);
+      dn += g.x * g.x + g.y * g.y;
     }
   }
-#pragma unroll
-  for (int j = 0; j < UB; ++j) {
-    d0[j] = sqrt(d0[j]);
-    dn[j] = sqrt(dn[j]);
-  }
+  d0 = sqrt(d0);
+  dn = sqrt(dn);
   const double sc = 1.0 / ((double)L * (double)L);
   bool ok = true;
   for (int k = 0; k < D; ++k) {
-    col_dft(G + k * gk, n1, NC, v, u0, L, s, tw, h);
-    if (v == 0) col_dft(G + k * gk, n1, NC, N, u0, L, s, tw, g);
-#pragma unroll
-    for (int j = 0; j < UB; ++j) {
-      const int u = u0 + j;
-      const double hx = h[j].x / d0[j], hy = h[j].y / d0[j];
-      float2* mk = main_out + (size_t)k * L * N + (size_t)u * N;
-      if (v == 0) {
-        const double gx = g[j].x / dn[j], gy = g[j].y / dn[j];
-        const double ax = 0.5 * (hx + gx) * sc, ay = 0.5 * (hy + gy) * sc;
-        const double bx = 0.5 * (hx - gx) * sc, by = 0.5 * (hy - gy) * sc;
-        ok = ok && isfinite(ax) && isfinite(ay) && isfinite(bx) && isfinite(by);
-        mk[0] = make_float2((float)ax, (float)ay);
-        beta_out[(size_t)k * L + u] = make_float2((float)bx, (float)by);
-      } else {
-        const double px = hx * sc, py = hy * sc;
-        ok = ok && isfinite(px) && isfinite(py);
-        mk[v] = make_float2((float)px, (float)py);
-      }
+    const double2 h = col_dft(G + k * gk, n1, NC, v, u, L, s, tw);
+    const double hx = h.x / d0, hy = h.y / d0;
+    float2* mk = main_out + (size_t)k * L * N + (size_t)u * N;
+    if (v == 0) {
+      const double2 g = col_dft(G + k * gk, n1, NC, N, u, L, s, tw);
+      const double gx = g.x / dn, gy = g.y / dn;
+      const double ax = 0.5 * (hx + gx) * sc, ay = 0.5 * (hy + gy) * sc;
+      const double bx = 0.5 * (hx - gx) * sc, by = 0.5 * (hy - gy) * sc;
+      ok = ok && isfinite(ax) && isfinite(ay) && isfinite(bx) && isfinite(by);
+      mk[0] = make_float2((float)ax, (float)ay);
+      beta_out[(size_t)k * L + u] = make_float2((float)bx, (float)by);
+    } else {
+      const double px = hx * sc, py = hy * sc;
+      ok = ok && isfinite(px) && isfinite(py);
+      mk[v] = make_float2((float)px, (float)py);
     }
   }
   if (!ok) bad[0] = 1;
@@ -152,8 +131,8 @@ int spectra_device(int L, int D, int n1, const double* w_host, int shift, float2
     r = fail(SHX_ERR_HIP, "spectra upload failed: %s", hipGetErrorString(e));
   if (r == SHX_OK) {
     hipLaunchKernelGGL(k_spec_rows, dim3((NC + ST - 1) / ST, n1, D), dim3(ST), 0, nullptr, w, n1, L, s, twd, G);
-    hipLaunchKernelGGL(k_spec_cols, dim3((N + ST - 1) / ST, L / UB), dim3(ST), 0, nullptr, G, D, n1, L, s, twd,
-                       main_out, beta_out, bad);
+    hipLaunchKernelGGL(k_spec_cols, dim3((N + ST - 1) / ST, L), dim3(ST), 0, nullptr, G, D, n1, L, s, twd, main_out,
+                       beta_out, bad);
     if ((e = hipGetLastError()) != hipSuccess ||
         (e = hipMemcpy(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
       r = fail(SHX_ERR_HIP, "spectra kernels failed: %s", hipGetErrorString(e));
