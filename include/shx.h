@@ -191,7 +191,9 @@ int shx_denoise_graph(shx_dict* dict, const float* x, float* const* coeff, doubl
  *   option 1 (wavelets): only cells[1], th = lambda.
  *   option 2 (shearlets): th(s,w) = sc[s] * lambda * E[s + w*E_rows]
  *       (E column-major E_rows x E_cols doubles, E_rows = ncells).
- *   opt 1: *maxi = max over thresholded bands of max|c|/E (else 0). */
+ *   opt 1: *maxi = max over thresholded bands of max|c|/E (else 0); this
+ *       allocates a device word and synchronises the stream, so an opt 1
+ *       call cannot be recorded into a graph (opt 0 can). */
 int shx_thresh(float* const* cells, int ncells, const int* cell_planes, const int64_t* plane_elems, double lambda,
                int option, const double* E, int E_rows, int E_cols, const double* sc, int n_sc, int opt,
                double* maxi, void* stream);
@@ -250,7 +252,8 @@ int shx_hard_bands_backward(const float* x, const float* gy, float* gx, int nimg
                             int64_t seglen, int band_off, const float* th_dev, void* stream);
 
 /* Sum |x|^p over n elements, result (p-th root not taken) in *out (host).
- * norm_cuda's reduction (shearcuda/src_cuda/norm_cuda.cpp:172-215). */
+ * norm_cuda's reduction (shearcuda/src_cuda/norm_cuda.cpp:172-215).
+ * Allocates a device word and synchronises the stream (not capturable). */
 int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream);
 
 /* Median absolute deviation / 0.6745 of each of nplanes planes (plane i at
@@ -289,21 +292,23 @@ int shx_nsst_scalars(shx_dict* dict, const float* noise, int arith, double* scal
  *   delta_max > 0: fixed starting threshold with sigma = 2.5
  *             (crack_detection/separateHairlineCrack.m:63-66); else the noise
  *             level and deltamax are estimated as separate.m:81-133 does
- *   E         band norms, column-major (S+1) x E_cols host FP64, or NULL to
- *             compute them as com_norm.m:38-53 does
+ *   E         band norms, column-major E_rows x E_cols host FP64 with
+ *             E_rows = S+1 and E_cols >= max ndir, or NULL to compute them as
+ *             com_norm.m:38-53 does (E_rows, E_cols ignored)
  *   C, P      L x L device outputs (curve part, point part)
  *   info      NULL or 3 doubles: sigma, deltamax, lambda.
  * The same library calls in the same order as the Python mirror
  * separation.separate; synchronises the stream before returning. */
 int shx_separate(shx_dict* dict, const float* img, int level2, int itermax, double stop, double gamma,
                  const double* coeff, int ncoeff, const double* qmf, int lqmf, double delta_max, const double* E,
-                 int E_cols, float* C, float* P, double* info, void* stream);
+                 int E_rows, int E_cols, float* C, float* P, double* info, void* stream);
 
 /* In-place soft threshold sign(v)*max(|v|-t, 0) (softth_cuda.cpp:49-112). */
 int shx_softth(float* x, int64_t n, double t, void* stream);
 
 /* Sum |a - b|^p over n elements into *out (host): the reduction of
- * Shearcuda::getNormErr (demo_denoise/shearcuda/shearcuda.cpp:672-717). */
+ * Shearcuda::getNormErr (demo_denoise/shearcuda/shearcuda.cpp:672-717).
+ * Allocates a device word and synchronises the stream (not capturable). */
 int shx_norm_pow_diff(const float* a, const float* b, int64_t n, double p, double* out, void* stream);
 
 /* x += N(0, sigma^2) noise from a counter-based generator (splitmix64 of

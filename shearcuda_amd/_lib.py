@@ -85,7 +85,7 @@ _SIGNATURES = [
     ("shx_band_mad", c_int, [c_void_p, ctypes.c_longlong, ctypes.c_longlong, c_int, c_int, c_void_p, c_void_p]),
     ("shx_nsst_scalars", c_int, [c_void_p, c_void_p, c_int, P(c_double), c_void_p]),
     ("shx_separate", c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_double, P(c_double), c_int, P(c_double),
-                             c_int, c_double, P(c_double), c_int, c_void_p, c_void_p, P(c_double), c_void_p]),
+                             c_int, c_double, P(c_double), c_int, c_int, c_void_p, c_void_p, P(c_double), c_void_p]),
     ("shx_maxflat_factor_error", c_double, []),
     ("shx_maxflat_filter", c_int, [c_int, P(c_double), P(c_int)]),
     ("shx_norm_pow_diff", c_int, [c_void_p, c_void_p, c_int64, c_double, P(c_double), c_void_p]),

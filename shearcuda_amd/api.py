@@ -38,6 +38,46 @@ def _require_gpu(x, name="x"):
         raise ValueError("%s must be contiguous" % name)
 
 
+def _require_shape(x, shape, name="x"):
+    """_require_gpu plus an exact shape; the C ABI trusts buffer sizes, so
+    every tensor is checked before anything is launched."""
+    _require_gpu(x, name)
+    if tuple(x.shape) != tuple(shape):
+        raise ValueError("%s must have shape %s, got %s" % (name, tuple(shape), tuple(x.shape)))
+
+
+def _require_cells(Ct, shear, name="Ct"):
+    """Coefficient cells of a dictionary: Ct{1} (L, L), Ct{j+1} (D_j, L, L)."""
+    if len(Ct) != shear.nscales + 1:
+        raise ValueError("Incorrect number of cell elements")
+    L = shear.L
+    _require_shape(Ct[0], (L, L), "%s{1}" % name)
+    for j, d in enumerate(shear.ndir):
+        _require_shape(Ct[j + 1], (d, L, L), "%s{%d}" % (name, j + 2))
+
+
+def _flat_scalars(shear, scalars):
+    """[s_low, array(D_1), ..., array(D_S)] -> 1 + sum(D) FP64 values."""
+    if len(scalars) != shear.nscales + 1:
+        raise ValueError("scalars must hold the lowpass and one array per scale (%d entries)" % (shear.nscales + 1))
+    if np.size(scalars[0]) != 1:
+        raise ValueError("scalars[0] (lowpass) must be one value")
+    parts = [np.asarray(scalars[0], dtype=np.float64).reshape(1)]
+    for j, d in enumerate(shear.ndir):
+        a = np.asarray(scalars[j + 1], dtype=np.float64).ravel()
+        if a.size != d:
+            raise ValueError("scalars[%d] must hold %d values, got %d" % (j + 1, d, a.size))
+        parts.append(a)
+    return np.ascontiguousarray(np.concatenate(parts))
+
+
+def _tscalars(Tscalars):
+    T = np.ascontiguousarray(np.asarray(Tscalars, dtype=np.float64).ravel())
+    if T.size != 3:
+        raise ValueError("Tscalars must hold 3 values (lowpass, coarser scales, finest scale)")
+    return T
+
+
 class ShearDict:
     """A prepared shearlet dictionary (the ``shear`` struct of
     prepare_shearing_filters_cuda.cpp:234-248): FP32 spectra on the GPU plus
@@ -150,10 +190,7 @@ def shear_trans_cuda(x, pfilt, shear, Ct=None):
         raise ValueError("Input data size does not match dictionary size")
     if Ct is None:
         Ct = shear.alloc_coeffs()
-    elif len(Ct) != shear.nscales + 1:
-        raise ValueError("Incorrect number of cell elements")
-    for i, c in enumerate(Ct):
-        _require_gpu(c, "Ct{%d}" % (i + 1))
+    _require_cells(Ct, shear)
     check(lib.shx_shear_trans(shear.handle, ctypes.c_void_p(x.data_ptr()),
                               ptr_array([c.data_ptr() for c in Ct]), _stream()))
     return Ct
@@ -163,12 +200,10 @@ def inverse_shear_cuda(Ct, pfilt, shear, out=None):
     """x = inverse_shear_cuda(Ct, 'maxflat', shear)
     (shearcuda/src_cuda/inverse_shear_cuda.cpp:57-219)."""
     _check_pfilt(pfilt)
-    if len(Ct) != shear.nscales + 1:
-        raise ValueError("Incorrect number of cell elements")
-    for i, c in enumerate(Ct):
-        _require_gpu(c, "Ct{%d}" % (i + 1))
+    _require_cells(Ct, shear)
     if out is None:
         out = torch.empty((shear.L, shear.L), device=Ct[0].device, dtype=torch.float32)
+    _require_shape(out, (shear.L, shear.L), "out")
     check(lib.shx_inverse_shear(shear.handle, ptr_array([c.data_ptr() for c in Ct]),
                                 ctypes.c_void_p(out.data_ptr()), _stream()))
     return out
@@ -176,12 +211,10 @@ def inverse_shear_cuda(Ct, pfilt, shear, out=None):
 
 def shear_trans_adjoint_cuda(Ct_bar, shear, out=None):
     """Transpose of shear_trans_cuda: coefficient gradients -> image gradient."""
-    if len(Ct_bar) != shear.nscales + 1:
-        raise ValueError("Incorrect number of cell elements")
-    for i, c in enumerate(Ct_bar):
-        _require_gpu(c, "Ct_bar{%d}" % (i + 1))
+    _require_cells(Ct_bar, shear, "Ct_bar")
     if out is None:
         out = torch.empty((shear.L, shear.L), device=Ct_bar[0].device, dtype=torch.float32)
+    _require_shape(out, (shear.L, shear.L), "out")
     check(lib.shx_shear_trans_adjoint(shear.handle, ptr_array([c.data_ptr() for c in Ct_bar]),
                                       ctypes.c_void_p(out.data_ptr()), _stream()))
     return out
@@ -189,9 +222,10 @@ def shear_trans_adjoint_cuda(Ct_bar, shear, out=None):
 
 def inverse_shear_adjoint_cuda(x_bar, shear, Ct_bar=None):
     """Transpose of inverse_shear_cuda: image gradient -> coefficient gradients."""
-    _require_gpu(x_bar, "x_bar")
+    _require_shape(x_bar, (shear.L, shear.L), "x_bar")
     if Ct_bar is None:
         Ct_bar = shear.alloc_coeffs()
+    _require_cells(Ct_bar, shear, "Ct_bar")
     check(lib.shx_inverse_shear_adjoint(shear.handle, ctypes.c_void_p(x_bar.data_ptr()),
                                         ptr_array([c.data_ptr() for c in Ct_bar]), _stream()))
     return Ct_bar
@@ -201,11 +235,13 @@ def inverse_shear_thresh_cuda(Ct, th, shear, out=None):
     """Inverse NSST of hard-thresholded bands (thresholds fused into the
     reconstruction's first read; Ct is not modified).  th: 1 + sum(D_j)
     thresholds, lowpass first."""
-    th = np.ascontiguousarray(np.asarray(th, dtype=np.float32))
+    _require_cells(Ct, shear)
+    th = np.ascontiguousarray(np.asarray(th, dtype=np.float32).ravel())
     if th.size != shear.nbands:
         raise ValueError("need %d thresholds" % shear.nbands)
     if out is None:
         out = torch.empty((shear.L, shear.L), device=Ct[0].device, dtype=torch.float32)
+    _require_shape(out, (shear.L, shear.L), "out")
     check(lib.shx_inverse_shear_thresh(shear.handle, ptr_array([c.data_ptr() for c in Ct]),
                                        th.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                        ctypes.c_void_p(out.data_ptr()), _stream()))
@@ -216,10 +252,13 @@ def band_thresholds(shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS):
     """nsst_HT thresholds T*sigma*s per band (shearlet_toolbox_1/nsst_HT.m:32-47),
     lowpass first.  scalars: [s_low, array(D_1), ..., array(D_S)]."""
     S = shear.nscales
-    th = [Tscalars[0] * sigma * float(scalars[0])]
-    for j in range(S):
-        T = Tscalars[1] if j < S - 1 else Tscalars[2]
-        th.extend(T * sigma * np.asarray(scalars[j + 1], dtype=np.float64))
+    sc = _flat_scalars(shear, scalars)
+    T = _tscalars(Tscalars)
+    th = [T[0] * sigma * sc[0]]
+    k = 1
+    for j, d in enumerate(shear.ndir):
+        th.extend((T[1] if j < S - 1 else T[2]) * sigma * sc[k:k + d])
+        k += d
     return np.array(th, dtype=np.float64)
 
 
@@ -228,16 +267,19 @@ def denoise_cuda(x, shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS, Ct=None, o
     the threshold fused into the inverse.  Returns (x_hat, Ct); with
     coeffs=False the coefficients are neither returned nor written
     (x_hat, None)."""
-    _require_gpu(x)
+    L = shear.L
+    _require_shape(x, (L, L))
     if not coeffs:
         Ct = None
     elif Ct is None:
         Ct = shear.alloc_coeffs()
+    if Ct is not None:
+        _require_cells(Ct, shear)
     if out is None:
         out = torch.empty_like(x)
-    sc = np.concatenate([[float(scalars[0])]] + [np.asarray(s, dtype=np.float64) for s in scalars[1:]])
-    sc = np.ascontiguousarray(sc, dtype=np.float64)
-    T = np.ascontiguousarray(np.asarray(Tscalars, dtype=np.float64))
+    _require_shape(out, (L, L), "out")
+    sc = _flat_scalars(shear, scalars)
+    T = _tscalars(Tscalars)
     cells = ptr_array([c.data_ptr() for c in Ct]) if Ct is not None else None
     check(lib.shx_denoise(shear.handle, ctypes.c_void_p(x.data_ptr()), cells, float(sigma), T.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                           sc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.c_void_p(out.data_ptr()),
@@ -250,9 +292,13 @@ def thresh_cuda(Ct, lam, option, E, sc, opt):
     (shearcuda/src_cuda/thresh_cuda.cpp:57-217)."""
     for i, c in enumerate(Ct):
         _require_gpu(c, "Ct{%d}" % (i + 1))
+        if c.dim() not in (2, 3):
+            raise ValueError("Ct{%d} must be a plane or a stack of planes" % (i + 1))
     E = np.asarray(E, dtype=np.float64)
     if E.ndim == 1:
         E = E[:, None]
+    if E.ndim != 2:
+        raise ValueError("E must be a matrix")
     sc = np.ascontiguousarray(np.asarray(sc, dtype=np.float64).ravel())
     Ecm = np.asfortranarray(E)
     planes = (ctypes.c_int * len(Ct))(*[c.shape[0] if c.dim() == 3 else 1 for c in Ct])
@@ -271,6 +317,8 @@ def atrousdec_cuda(x, pfilt, nlevels):
     (shearcuda/src_cuda/atrousdec.cpp:41-131): [lowpass, coarse ... fine]."""
     _check_pfilt(pfilt)
     _require_gpu(x)
+    if x.dim() != 2 or x.shape[0] != x.shape[1]:
+        raise ValueError("Input data should be a square")
     L = x.shape[0]
     y = [torch.empty_like(x) for _ in range(nlevels + 1)]
     check(lib.shx_atrousdec(ctypes.c_void_p(x.data_ptr()), L, int(nlevels), ptr_array([t.data_ptr() for t in y]),
@@ -281,8 +329,13 @@ def atrousdec_cuda(x, pfilt, nlevels):
 def atrousrec_cuda(y, pfilt):
     """x = atrousrec(y, 'maxflat') on the GPU (shearcuda/src_cuda/atrousrec.cpp:42-123)."""
     _check_pfilt(pfilt)
-    for t in y:
-        _require_gpu(t, "y")
+    if len(y) < 2:
+        raise ValueError("y must hold the lowpass and at least one level")
+    _require_gpu(y[0], "y{1}")
+    if y[0].dim() != 2 or y[0].shape[0] != y[0].shape[1]:
+        raise ValueError("Input data should be a square")
+    for i, t in enumerate(y):
+        _require_shape(t, y[0].shape, "y{%d}" % (i + 1))
     out = torch.empty_like(y[0])
     check(lib.shx_atrousrec(ptr_array([t.data_ptr() for t in y]), y[0].shape[0], len(y) - 1,
                             ctypes.c_void_p(out.data_ptr()), None, _stream()))
@@ -295,9 +348,10 @@ def convolution_cuda(y, shear, Ct=None):
     Ct[0] is a copy of y[0]."""
     if len(y) != shear.nscales + 1:
         raise ValueError("y must hold the lowpass and one level per scale")
-    for t in y:
-        _require_gpu(t, "y")
+    for i, t in enumerate(y):
+        _require_shape(t, (shear.L, shear.L), "y{%d}" % (i + 1))
     Ct = shear.alloc_coeffs() if Ct is None else Ct
+    _require_cells(Ct, shear)
     check(lib.shx_convolution(shear.handle, ptr_array([t.data_ptr() for t in y]),
                               ptr_array([c.data_ptr() for c in Ct]), _stream()))
     return Ct
@@ -306,9 +360,8 @@ def convolution_cuda(y, shear, Ct=None):
 def applyfilters_cuda(Ct, shear):
     """Conjugate-filtered band sums per scale (Shearcuda::applyfilters,
     demo_denoise/shearcuda/shearcuda.cpp:719-800): [None, level 1 ... S]."""
-    for c in Ct:
-        _require_gpu(c, "Ct")
-    y = [None] + [torch.empty((shear.L, shear.L), device="cuda") for _ in range(shear.nscales)]
+    _require_cells(Ct, shear)
+    y = [None] + [torch.empty((shear.L, shear.L), device=Ct[0].device) for _ in range(shear.nscales)]
     check(lib.shx_applyfilters(shear.handle, ptr_array([c.data_ptr() for c in Ct]),
                                ptr_array([0] + [t.data_ptr() for t in y[1:]]), _stream()))
     return y
@@ -545,46 +598,55 @@ class Batch:
         L = self.shear.L
         return torch.empty((self.nimg, self.shear.nbands, L, L), device=self.shear.device, dtype=torch.float32)
 
-    def _chk(self, *ts):
-        for t in ts:
-            _require_gpu(t)
-        if ts[0].shape[0] != self.nimg:
-            raise ValueError("batch holds %d images, got %d" % (self.nimg, ts[0].shape[0]))
+    def _img(self, t, name):
+        L = self.shear.L
+        _require_shape(t, (self.nimg, L, L), name)
+
+    def _coef(self, t, name):
+        L = self.shear.L
+        _require_shape(t, (self.nimg, self.shear.nbands, L, L), name)
+
+    def _th(self, th):
+        _require_shape(th, (self.nimg, self.shear.nbands), "th")
 
     def shear_trans(self, x, coeff):
-        self._chk(x, coeff)
+        self._img(x, "x")
+        self._coef(coeff, "coeff")
         check(lib.shx_batch_shear_trans(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(coeff.data_ptr()),
                                         _stream()))
         return coeff
 
     def inverse_shear(self, coeff, out, th=None):
-        self._chk(coeff, out)
+        self._coef(coeff, "coeff")
+        self._img(out, "out")
+        if th is not None:
+            self._th(th)
         thp = ctypes.c_void_p(th.data_ptr()) if th is not None else None
         check(lib.shx_batch_inverse_shear(self._h, ctypes.c_void_p(coeff.data_ptr()), thp,
                                           ctypes.c_void_p(out.data_ptr()), _stream()))
         return out
 
     def shear_trans_adjoint(self, coeff_bar, x_bar):
-        self._chk(coeff_bar, x_bar)
+        self._coef(coeff_bar, "coeff_bar")
+        self._img(x_bar, "x_bar")
         check(lib.shx_batch_shear_trans_adjoint(self._h, ctypes.c_void_p(coeff_bar.data_ptr()),
                                                 ctypes.c_void_p(x_bar.data_ptr()), _stream()))
         return x_bar
 
     def inverse_shear_adjoint(self, x_bar, coeff_bar):
-        self._chk(x_bar, coeff_bar)
+        self._img(x_bar, "x_bar")
+        self._coef(coeff_bar, "coeff_bar")
         check(lib.shx_batch_inverse_shear_adjoint(self._h, ctypes.c_void_p(x_bar.data_ptr()),
                                                   ctypes.c_void_p(coeff_bar.data_ptr()), _stream()))
         return coeff_bar
 
     def denoise(self, x, coeff, th, out):
         """coeff None: the coefficients are neither returned nor written."""
-        if coeff is None:
-            self._chk(x, out)
-        else:
-            self._chk(x, coeff, out)
-        _require_gpu(th, "th")
-        if tuple(th.shape) != (self.nimg, self.shear.nbands):
-            raise ValueError("thresholds must be (%d, %d)" % (self.nimg, self.shear.nbands))
+        self._img(x, "x")
+        self._img(out, "out")
+        if coeff is not None:
+            self._coef(coeff, "coeff")
+        self._th(th)
         cptr = ctypes.c_void_p(coeff.data_ptr()) if coeff is not None else None
         check(lib.shx_batch_denoise(self._h, ctypes.c_void_p(x.data_ptr()), cptr,
                                     ctypes.c_void_p(th.data_ptr()), ctypes.c_void_p(out.data_ptr()), _stream()))
@@ -674,8 +736,8 @@ def mrdwt_cuda(x, h, L):
 def mirdwt_cuda(yl, yh, h, L):
     """x = mirdwt_cuda(yl, yh, h, L) (shearcuda/src_cuda/mirdwt_cuda.cpp:58-255)."""
     _require_gpu(yl, "yl")
-    _require_gpu(yh, "yh")
     nimg, m, n = _rdwt_dims(yl)
+    _require_shape(yh, yl.shape[:-1] + (3 * n * L,), "yh")
     h, hp = _h_arg(h)
     x = torch.empty_like(yl)
     check(lib.shx_mirdwt(ctypes.c_void_p(yl.data_ptr()), ctypes.c_void_p(yh.data_ptr()), m, n, hp, h.size, int(L),
@@ -686,8 +748,8 @@ def mirdwt_cuda(yl, yh, h, L):
 def mrdwt_adjoint_cuda(yl_bar, yh_bar, h, L):
     """Transpose of mrdwt: (yl_bar, yh_bar) -> x_bar."""
     _require_gpu(yl_bar, "yl_bar")
-    _require_gpu(yh_bar, "yh_bar")
     nimg, m, n = _rdwt_dims(yl_bar)
+    _require_shape(yh_bar, yl_bar.shape[:-1] + (3 * n * L,), "yh_bar")
     h, hp = _h_arg(h)
     xb = torch.empty_like(yl_bar)
     check(lib.shx_mrdwt_adjoint(ctypes.c_void_p(yl_bar.data_ptr()), ctypes.c_void_p(yh_bar.data_ptr()), m, n, hp,

@@ -42,6 +42,21 @@ namespace {
 using shx::fail;
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Device allocation owned by a scope: freed on every return path.
+struct DevMem {
+  void* p = nullptr;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
 }  // namespace
 
 // Per-image intermediate buffers of the NSST path.
@@ -1648,9 +1663,11 @@ int shx_thresh(float* const* cells, int ncells, const int* cell_planes, const in
       }
     }
   }
+  DevMem dmax_mem;
   unsigned int* dmax = nullptr;
   if (opt == 1) {
-    SHX_HIP(hipMalloc(&dmax, sizeof(unsigned int)));
+    SHX_HIP(hipMalloc(&dmax_mem.p, sizeof(unsigned int)));
+    dmax = dmax_mem.as<unsigned int>();
     SHX_HIP(hipMemsetAsync(dmax, 0, sizeof(unsigned int), s));
   }
   for (size_t b = 0; b < ptr.size(); b += 64) {
@@ -1674,7 +1691,6 @@ int shx_thresh(float* const* cells, int ncells, const int* cell_planes, const in
     unsigned int bits = 0;
     SHX_HIP(hipMemcpyAsync(&bits, dmax, sizeof bits, hipMemcpyDeviceToHost, s));
     SHX_HIP(hipStreamSynchronize(s));
-    hipFree(dmax);
     float f;
     memcpy(&f, &bits, sizeof f);
     if (maxi) *maxi = f;
@@ -1889,30 +1905,30 @@ int shx_mirdwt_adjoint(const float* x_bar, int m, int n, const double* h, int lh
 int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream) {
   if (!x || !out) return fail(SHX_ERR_ARG, "null input or output");
   hipStream_t s = (hipStream_t)stream;
-  double* acc = nullptr;
-  SHX_HIP(hipMalloc(&acc, sizeof(double)));
+  DevMem acc_mem;
+  SHX_HIP(hipMalloc(&acc_mem.p, sizeof(double)));
+  double* acc = acc_mem.as<double>();
   SHX_HIP(hipMemsetAsync(acc, 0, sizeof(double), s));
   const int blocks = (int)std::max<long long>(1, std::min<long long>(1024, (n + 4095) / 4096));
   if (n > 0) hipLaunchKernelGGL(k_norm_pow, dim3(blocks), dim3(256), 0, s, x, (long long)n, (float)p, acc);
   SHX_HIP(hipGetLastError());
   SHX_HIP(hipMemcpyAsync(out, acc, sizeof(double), hipMemcpyDeviceToHost, s));
   SHX_HIP(hipStreamSynchronize(s));
-  hipFree(acc);
   return SHX_OK;
 }
 
 int shx_norm_pow_diff(const float* a, const float* b, int64_t n, double p, double* out, void* stream) {
   if (!a || !b || !out) return fail(SHX_ERR_ARG, "null input or output");
   hipStream_t s = (hipStream_t)stream;
-  double* acc = nullptr;
-  SHX_HIP(hipMalloc(&acc, sizeof(double)));
+  DevMem acc_mem;
+  SHX_HIP(hipMalloc(&acc_mem.p, sizeof(double)));
+  double* acc = acc_mem.as<double>();
   SHX_HIP(hipMemsetAsync(acc, 0, sizeof(double), s));
   const int blocks = (int)std::max<long long>(1, std::min<long long>(1024, (n + 4095) / 4096));
   if (n > 0) hipLaunchKernelGGL(k_norm_pow_diff, dim3(blocks), dim3(256), 0, s, a, b, (long long)n, (float)p, acc);
   SHX_HIP(hipGetLastError());
   SHX_HIP(hipMemcpyAsync(out, acc, sizeof(double), hipMemcpyDeviceToHost, s));
   SHX_HIP(hipStreamSynchronize(s));
-  hipFree(acc);
   return SHX_OK;
 }
 
@@ -2024,7 +2040,8 @@ std::vector<double> on_filter(const double* f, int n) {
 
 extern "C" int shx_separate(shx_dict* d, const float* img, int level2, int itermax, double stop, double gamma,
                             const double* coeff, int ncoeff, const double* qmf, int lqmf, double delta_max,
-                            const double* E, int E_cols, float* C, float* P, double* info, void* stream) {
+                            const double* E, int E_rows, int E_cols, float* C, float* P, double* info,
+                            void* stream) {
   if (int r = check_dict(d)) return r;
   if (!img || !coeff || !qmf || !C || !P) return fail(SHX_ERR_ARG, "null image, weights, filter or output");
   const int L = d->L, S = d->S, nb = d->nbands;
@@ -2034,6 +2051,7 @@ extern "C" int shx_separate(shx_dict* d, const float* img, int level2, int iterm
   if (ncoeff <= 1 || ncoeff > 10) return fail(SHX_ERR_ARG, "need 2..10 band weights");
   int maxd = 1;
   for (int j = 0; j < S; ++j) maxd = std::max(maxd, d->ndir[j]);
+  if (E && E_rows != S + 1) return fail(SHX_ERR_ARG, "E has %d rows, the dictionary needs %d", E_rows, S + 1);
   if (E && E_cols < maxd) return fail(SHX_ERR_ARG, "E has %d columns, the dictionary needs %d", E_cols, maxd);
   hipStream_t s = (hipStream_t)stream;
   const size_t pl = (size_t)L * L, nyh = (size_t)3 * L * level2 * L;

@@ -29,14 +29,34 @@ def _vp(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _check_layout(x, th, nimg, img_elems, rowlen, seglen, band_off, *same):
+    """Buffers of the per-band threshold kernels: x holds nimg images of
+    img_elems elements; element e of an image belongs to band
+    band_off + (e mod rowlen) // seglen, whose threshold th holds (one table
+    for all images); each tensor in `same` has x's shape."""
+    api._require_gpu(x, "x")
+    api._require_gpu(th, "th")
+    if nimg < 1 or img_elems < 1 or rowlen < 1 or seglen < 1 or img_elems % rowlen or rowlen % seglen:
+        raise ValueError("invalid band layout")
+    if x.numel() != nimg * img_elems:
+        raise ValueError("x holds %d elements, the layout %d" % (x.numel(), nimg * img_elems))
+    if band_off < 0 or th.numel() < band_off + rowlen // seglen:
+        raise ValueError("th must hold at least %d thresholds" % (band_off + rowlen // seglen))
+    for t in same:
+        api._require_shape(t, x.shape, "buffer")
+
+
 def soft_bands(x, th, nimg, img_elems, rowlen, seglen, band_off=0, out=None):
     out = torch.empty_like(x) if out is None else out
+    _check_layout(x, th, nimg, img_elems, rowlen, seglen, band_off, out)
     check(lib.shx_soft_bands(_vp(x), _vp(out), nimg, img_elems, rowlen, seglen, band_off, _vp(th), _stream()))
     return out
 
 
 def soft_bands_backward(x, gy, th, gth, nimg, img_elems, rowlen, seglen, band_off=0):
     gx = torch.empty_like(x)
+    _check_layout(x, th, nimg, img_elems, rowlen, seglen, band_off, gy)
+    api._require_shape(gth, th.shape, "gth")
     check(lib.shx_soft_bands_backward(_vp(x), _vp(gy), _vp(gx), _vp(gth), nimg, img_elems, rowlen, seglen, band_off,
                                       _vp(th), _stream()))
     return gx
@@ -44,12 +64,14 @@ def soft_bands_backward(x, gy, th, gth, nimg, img_elems, rowlen, seglen, band_of
 
 def hard_bands(x, th, nimg, img_elems, rowlen, seglen, band_off=0, out=None):
     out = torch.empty_like(x) if out is None else out
+    _check_layout(x, th, nimg, img_elems, rowlen, seglen, band_off, out)
     check(lib.shx_hard_bands(_vp(x), _vp(out), nimg, img_elems, rowlen, seglen, band_off, _vp(th), _stream()))
     return out
 
 
 def hard_bands_backward(x, gy, th, nimg, img_elems, rowlen, seglen, band_off=0):
     gx = torch.empty_like(x)
+    _check_layout(x, th, nimg, img_elems, rowlen, seglen, band_off, gy)
     check(lib.shx_hard_bands_backward(_vp(x), _vp(gy), _vp(gx), nimg, img_elems, rowlen, seglen, band_off, _vp(th),
                                       _stream()))
     return gx
@@ -253,15 +275,17 @@ def separate_native(img, level2, itermax, stop, gamma, coeff, shear, wavelet=("S
     if E is not None:
         Ef = np.asfortranarray(np.asarray(E, dtype=np.float64))
         Eflat = np.ascontiguousarray(Ef.ravel(order="F"))
-        eptr, ecols = Eflat.ctypes.data_as(dp), Ef.shape[1]
+        if Ef.ndim != 2 or Ef.shape[0] != shear.nscales + 1:
+            raise ValueError("E must have %d rows (lowpass and one per scale)" % (shear.nscales + 1))
+        eptr, erows, ecols = Eflat.ctypes.data_as(dp), Ef.shape[0], Ef.shape[1]
     else:
-        Eflat, eptr, ecols = None, None, 0
+        Eflat, eptr, erows, ecols = None, None, 0, 0
     C = torch.empty_like(img)
     P = torch.empty_like(img)
     inf = np.zeros(3)
     check(lib.shx_separate(shear.handle, _vp(img), int(level2), int(itermax), float(stop), float(gamma),
                            coeff.ctypes.data_as(dp), coeff.size, qmf.ctypes.data_as(dp), qmf.size, float(delta_max),
-                           eptr, ecols, _vp(C), _vp(P), inf.ctypes.data_as(dp), _stream()))
+                           eptr, erows, ecols, _vp(C), _vp(P), inf.ctypes.data_as(dp), _stream()))
     if info is not None:
         info.update({"sigma": float(inf[0]), "deltamax": float(inf[1]), "lambda": float(inf[2])})
     return C, P

@@ -70,8 +70,16 @@ def test_band_thresholds_match_nsst_HT():
 
     class D:
         nscales = 3
+        ndir = [2, 1, 2]
     sc = [2.0, np.array([1.0, 2.0]), np.array([3.0]), np.array([0.5, 0.25])]
     th = band_thresholds(D, 10.0, sc, (0.0, 3.0, 4.0))
     assert np.allclose(th, [0.0, 30.0, 60.0, 90.0, 20.0, 10.0])
     ref = o.nsst_thresholds(10.0, (0.0, 3.0, 4.0), sc)
     assert np.allclose(th, ref)
+    # scalar tables of the wrong shape are rejected (the C ABI reads 1 + sum D
+    # scalars and 3 T values from the host pointers it is given)
+    for bad in (sc[:-1], [2.0, np.array([1.0]), sc[2], sc[3]], [np.array([1.0, 2.0])] + sc[1:]):
+        with pytest.raises(ValueError):
+            band_thresholds(D, 10.0, bad, (0.0, 3.0, 4.0))
+    with pytest.raises(ValueError):
+        band_thresholds(D, 10.0, sc, (0.0, 3.0))
