@@ -204,6 +204,18 @@ int shx_thresh(float* const* cells, int ncells, const int* cell_planes, const in
  * atrousrec.cpp:42-123; demo_denoise/shearcuda/shearcuda.cpp:401-546).
  * work: 2 planes of scratch (L*L floats each) or NULL (allocated per call). */
 int shx_atrousdec(const float* x, int L, int nlevels, float* const* y, float* work, void* stream);
+/* shx_atrousdec without any allocation: the two level temporaries live in a
+ * static device array of 2 * SHX_NOALLOC_MAX_L^2 floats (8 MiB) that every
+ * device the library runs on holds until the process ends; L above
+ * SHX_NOALLOC_MAX_L is rejected (SHX_ERR_ARG).  The results equal
+ * shx_atrousdec's bit for bit.  It calls no hipMalloc/hipFree and does not
+ * synchronise.  All calls on one device share that device's array, so calls
+ * that overlap on one device (concurrent streams or host threads) race; serialise
+ * them.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). */
+#define SHX_NOALLOC_MAX_L 1024
+int shx_atrousdec_noalloc(const float* x, int L, int nlevels, float* const* y, void* stream);
 int shx_atrousrec(const float* const* y, int L, int nlevels, float* x, float* work, void* stream);
 
 /* re_weight_band_cuda(x, coeff): atrousrec(coeff(j) .* atrousdec(x)) with

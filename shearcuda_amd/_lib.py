@@ -64,6 +64,7 @@ _SIGNATURES = [
     ("shx_thresh", c_int, [P(c_void_p), c_int, P(c_int), P(c_int64), c_double, c_int, P(c_double), c_int, c_int,
                            P(c_double), c_int, c_int, P(c_double), c_void_p]),
     ("shx_atrousdec", c_int, [c_void_p, c_int, c_int, P(c_void_p), c_void_p, c_void_p]),
+    ("shx_atrousdec_noalloc", c_int, [c_void_p, c_int, c_int, P(c_void_p), c_void_p]),
     ("shx_atrousrec", c_int, [P(c_void_p), c_int, c_int, c_void_p, c_void_p, c_void_p]),
     ("shx_re_weight_band", c_int, [c_void_p, c_int, P(c_double), c_int, c_void_p, c_void_p, c_void_p]),
     ("shx_mrdwt", c_int, [c_void_p, c_int, c_int, P(c_double), c_int, c_int, c_void_p, c_void_p, c_void_p, c_int,

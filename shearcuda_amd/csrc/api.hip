@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1713,6 +1714,61 @@ int shx_atrousdec(const float* x, int L, int nlevels, float* const* y, float* wo
     hipFree(w);
   }
   SHX_HIP(e);
+  return SHX_OK;
+}
+
+}  // extern "C"
+
+// ---- allocation-free pyramid decomposition ---------------------------------------
+// Level temporaries of shx_atrousdec_noalloc: two planes of at most
+// SHX_NOALLOC_MAX_L^2 floats in a file-scope device array.  Every device the
+// library runs on holds its own copy until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in): the
+// device address of each device's copy comes from hipGetSymbolAddress, taken
+// once per device with that device current.
+static __device__ __align__(256) float scratch[2 * (size_t)SHX_NOALLOC_MAX_L * SHX_NOALLOC_MAX_L];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's copy of `scratch`.
+int scratch_address(float** out) {
+  int dev = -1;
+  SHX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SHX_ERR_UNSUPPORTED, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SHX_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SHX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int shx_atrousdec_noalloc(const float* x, int L, int nlevels, float* const* y, void* stream) {
+  if (!x || !y) return fail(SHX_ERR_ARG, "null input or output");
+  if (!is_pow2(L) || L < 32) return fail(SHX_ERR_UNSUPPORTED, "image size %d not supported", L);
+  if (L > SHX_NOALLOC_MAX_L)
+    return fail(SHX_ERR_ARG, "image size %d exceeds the %d x %d planes of the static scratch (use shx_atrousdec)", L,
+                SHX_NOALLOC_MAX_L, SHX_NOALLOC_MAX_L);
+  if (nlevels < 1 || nlevels > 9 || L % (1 << (nlevels - 1)) != 0)
+    return fail(SHX_ERR_ARG, "invalid number of levels %d", nlevels);
+  for (int j = 0; j <= nlevels; ++j)
+    if (!y[j]) return fail(SHX_ERR_ARG, "output plane %d is null", j);
+  float* w = nullptr;
+  if (int r = scratch_address(&w)) return r;
+  // every level writes its whole temporary before the next level reads it, so
+  // the scratch needs no clearing
+  SHX_HIP(pyr_dec(x, L, nlevels, y, w, w + (size_t)L * L, (hipStream_t)stream));
+  SHX_HIP(hipGetLastError());
   return SHX_OK;
 }
 
