@@ -64,6 +64,24 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
 #define SHX_DICT_HOST_SPECTRA 1
 int shx_prepare_shearing_filters_ex(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s,
                                     int shift, int flags, shx_dict** out);
+/* The whole dictionary from its parameters, everything on the device: the
+ * 2^dcomp[j] spatial Meyer filters of side dsize[j] of every scale (FP64,
+ * shx_meyer_filters), then their spectra.  Replaces
+ * shearing_filters_Myer(dsize, dcomp, L, 'GPUsingle')
+ * (shearcuda/shearing_filters_Myer.m:29-81, GPU branch, which builds the
+ * filters on the host and calls prepare_shearing_filters_cuda).  flags as in
+ * shx_prepare_shearing_filters_ex. */
+int shx_shearing_filters_myer(int L, int nscales, const int* dcomp, const int* dsize, int shift, int flags,
+                              shx_dict** out);
+/* Spatial Meyer shearing filters of one scale on the device (FP64): the
+ * pseudo-polar grid of an n1 x n1 block, 2^level Meyer windows over its 2n1
+ * lines, each re-assembled on the block and brought to space by a centred
+ * inverse DFT (shearlet_toolbox_1/shearing_filters_Myer.m:18-27 with
+ * gen_x_y_cordinates.m:22-111, windowing.m:11-27, meyer_wind.m:11-19,
+ * rec_from_pol.m:16-35).  w_dev: device array of 2^level * n1 * n1 doubles
+ * (element [k][r][c] = MATLAB w_s(r+1, c+1, k+1)); 2*n1 must be a multiple
+ * of 2^level.  Allocates scratch and synchronises the stream. */
+int shx_meyer_filters(int n1, int level, double* w_dev, void* stream);
 /* Frees a dictionary (clear_shearing_filters_cuda.cpp:51-96). NULL is a no-op. */
 int shx_clear_shearing_filters(shx_dict* dict);
 /* Dictionary shape: L, S and ndir[0..S-1] (ndir may be NULL). */

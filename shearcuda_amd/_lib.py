@@ -30,6 +30,8 @@ _SIGNATURES = [
     ("shx_prepare_shearing_filters", c_int, [c_int, c_int, P(c_int), P(c_int), P(P(c_double)), c_int, P(c_void_p)]),
     ("shx_prepare_shearing_filters_ex", c_int, [c_int, c_int, P(c_int), P(c_int), P(P(c_double)), c_int, c_int,
                                                 P(c_void_p)]),
+    ("shx_shearing_filters_myer", c_int, [c_int, c_int, P(c_int), P(c_int), c_int, c_int, P(c_void_p)]),
+    ("shx_meyer_filters", c_int, [c_int, c_int, c_void_p, c_void_p]),
     ("shx_clear_shearing_filters", c_int, [c_void_p]),
     ("shx_dict_info", c_int, [c_void_p, P(c_int), P(c_int), P(c_int)]),
     ("shx_dict_bytes", c_size_t, [c_void_p]),

@@ -158,12 +158,40 @@ def prepare_shearing_filters_cuda(w_s, L, shift="none", spectra="device"):
     return ShearDict(handle, int(L), list(ndir), list(dsize), shift, w_s, torch.device("cuda", torch.cuda.current_device()))
 
 
-def shearing_filters_Myer(m, num, L, dataType="GPUsingle", shift="none"):
+def shearing_filters_Myer(m, num, L, dataType="GPUsingle", shift="none", filters="device"):
     """shear = shearing_filters_Myer(dsize, dcomp, L, 'GPUsingle')
-    (shearcuda/shearing_filters_Myer.m:29-81, GPU branch)."""
+    (shearcuda/shearing_filters_Myer.m:29-81, GPU branch).
+
+    filters: 'device' builds the spatial Meyer filters and their spectra on
+    the GPU (shx_shearing_filters_myer); 'host' builds the filters in numpy
+    (filters.py) and the spectra on the GPU."""
     if dataType not in ("GPUsingle",):
         raise ValueError("only 'GPUsingle' dictionaries exist on this path")
-    return prepare_shearing_filters_cuda(dictionary_filters(num, m), L, shift)
+    if shift not in _SHIFTS:
+        raise ValueError("shift must be 'none' or 'center'")
+    if filters == "host":
+        return prepare_shearing_filters_cuda(dictionary_filters(num, m), L, shift)
+    if filters != "device":
+        raise ValueError("filters must be 'device' or 'host'")
+    if len(m) != len(num):
+        raise ValueError("dcomp and dsize must have the same length")
+    S = len(m)
+    dcomp = (ctypes.c_int * S)(*[int(v) for v in num])
+    dsize = (ctypes.c_int * S)(*[int(v) for v in m])
+    handle = ctypes.c_void_p()
+    torch.cuda.init()
+    check(lib.shx_shearing_filters_myer(int(L), S, dcomp, dsize, _SHIFTS[shift], 0, ctypes.byref(handle)))
+    return ShearDict(handle, int(L), [1 << int(v) for v in num], [int(v) for v in m], shift, None,
+                     torch.device("cuda", torch.cuda.current_device()))
+
+
+def meyer_filters_cuda(n1, level):
+    """The 2^level spatial Meyer shearing filters of side n1, built on the GPU
+    in FP64 (shx_meyer_filters): a (2^level, n1, n1) float64 device tensor,
+    element [k, r, c] = MATLAB w_s(r+1, c+1, k+1)."""
+    w = torch.empty((1 << int(level), int(n1), int(n1)), device="cuda", dtype=torch.float64)
+    check(lib.shx_meyer_filters(int(n1), int(level), ctypes.c_void_p(w.data_ptr()), _stream()))
+    return w
 
 
 def clear_shearing_filters_cuda(shear):

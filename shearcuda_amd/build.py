@@ -17,7 +17,7 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libshx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["api.hip", "api3d.hip", "pyramid.hip", "spectral.hip", "spectral3d.hip", "rdwt.hip", "select.hip", "dict_dev.hip", "maxflat.cpp",
+SOURCES = ["api.hip", "api3d.hip", "pyramid.hip", "spectral.hip", "spectral3d.hip", "rdwt.hip", "select.hip", "dict_dev.hip", "meyer_dev.hip", "maxflat.cpp",
            "dict_host.cpp"]
 GEN = os.path.join(BUILD, "gen")
 

@@ -16,6 +16,7 @@
 #include "api_err.h"
 #include "dict_dev.h"
 #include "dict_host.h"
+#include "meyer_dev.h"
 #include "pyramid.h"
 #include "rdwt.h"
 #include "spectral.h"
@@ -1191,8 +1192,14 @@ int shx_prepare_shearing_filters(int L, int nscales, const int* ndir, const int*
   return shx_prepare_shearing_filters_ex(L, nscales, ndir, dsize, w_s, shift, 0, out);
 }
 
-int shx_prepare_shearing_filters_ex(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s,
-                                    int shift, int flags, shx_dict** out) {
+}  // extern "C"
+
+namespace {
+// Dictionary from spatial filters in host memory, or in device memory
+// (w_on_device: the spectra then come from them without a host round trip,
+// except that the host-spectra flag copies them down first).
+int prepare_dict(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s, bool w_on_device,
+                 int shift, int flags, shx_dict** out) {
   if (!out) return fail(SHX_ERR_ARG, "null output pointer");
   if (flags & ~SHX_DICT_HOST_SPECTRA) return fail(SHX_ERR_ARG, "unknown dictionary flags 0x%x", flags);
   const bool host_spectra = (flags & SHX_DICT_HOST_SPECTRA) != 0;
@@ -1239,7 +1246,17 @@ int shx_prepare_shearing_filters_ex(int L, int nscales, const int* ndir, const i
   std::vector<float> tw((size_t)L * 2);
   for (int j = 0; host_spectra && j < nscales; ++j) {
     std::vector<std::complex<double>> H;
-    shx::scale_spectra_fp64(L, ndir[j], dsize[j], w_s[j], shift, H);
+    std::vector<double> wh;
+    const double* wj = w_s[j];
+    if (w_on_device) {
+      wh.resize((size_t)ndir[j] * dsize[j] * dsize[j]);
+      if (hipMemcpy(wh.data(), w_s[j], wh.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        delete d;
+        return fail(SHX_ERR_HIP, "scale %d: filter download failed", j);
+      }
+      wj = wh.data();
+    }
+    shx::scale_spectra_fp64(L, ndir[j], dsize[j], wj, shift, H);
     for (size_t i = 0; i < H.size(); ++i)
       if (!std::isfinite(H[i].real()) || !std::isfinite(H[i].imag())) {
         delete d;
@@ -1275,12 +1292,16 @@ int shx_prepare_shearing_filters_ex(int L, int nscales, const int* ndir, const i
     return r;
   }
   d->bytes = bytes_hm + bytes_hb + tw.size() * sizeof(float) + 2 * bytes_dm + bytes_dt + d->w.bytes;
-  for (int j = 0; !host_spectra && j < nscales; ++j)
-    if (int r = shx::spectra_device(L, ndir[j], dsize[j], w_s[j], shift, d->Hmain + (size_t)d->map.first[j] * L * N,
-                                    d->Hbeta + (size_t)d->map.first[j] * L, j)) {
+  for (int j = 0; !host_spectra && j < nscales; ++j) {
+    float2* hm_j = d->Hmain + (size_t)d->map.first[j] * L * N;
+    float2* hb_j = d->Hbeta + (size_t)d->map.first[j] * L;
+    const int r = w_on_device ? shx::spectra_device_from_dev(L, ndir[j], dsize[j], w_s[j], shift, hm_j, hb_j, j)
+                              : shx::spectra_device(L, ndir[j], dsize[j], w_s[j], shift, hm_j, hb_j, j);
+    if (r) {
       shx_clear_shearing_filters(d);
       return r;
     }
+  }
   if ((host_spectra && ((e = hipMemcpy(d->Hmain, hm.data(), bytes_hm, hipMemcpyHostToDevice)) != hipSuccess ||
                         (e = hipMemcpy(d->Hbeta, hb.data(), bytes_hb, hipMemcpyHostToDevice)) != hipSuccess)) ||
       (e = hipMemcpy(d->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
@@ -1292,6 +1313,40 @@ int shx_prepare_shearing_filters_ex(int L, int nscales, const int* ndir, const i
   }
   *out = d;
   return SHX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int shx_prepare_shearing_filters_ex(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s,
+                                    int shift, int flags, shx_dict** out) {
+  return prepare_dict(L, nscales, ndir, dsize, w_s, false, shift, flags, out);
+}
+
+int shx_meyer_filters(int n1, int level, double* w_dev, void* stream) {
+  return shx::meyer_filters_device(n1, level, w_dev, (hipStream_t)stream);
+}
+
+int shx_shearing_filters_myer(int L, int nscales, const int* dcomp, const int* dsize, int shift, int flags,
+                              shx_dict** out) {
+  if (!out) return fail(SHX_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (!dcomp || !dsize) return fail(SHX_ERR_ARG, "null filter description");
+  if (nscales < 1 || nscales > SHX_MAX_SCALES)
+    return fail(SHX_ERR_ARG, "number of scales %d outside [1, %d]", nscales, SHX_MAX_SCALES);
+  std::vector<DevMem> w(nscales);
+  std::vector<const double*> wp(nscales);
+  std::vector<int> ndir(nscales);
+  for (int j = 0; j < nscales; ++j) {
+    if (dcomp[j] < 0 || (1 << dcomp[j]) > SHX_MAX_DIRS)
+      return fail(SHX_ERR_ARG, "scale %d: 2^%d directions (at most %d)", j, dcomp[j], SHX_MAX_DIRS);
+    if (dsize[j] < 2 || dsize[j] > L) return fail(SHX_ERR_ARG, "scale %d: filter size %d invalid for L=%d", j, dsize[j], L);
+    ndir[j] = 1 << dcomp[j];
+    SHX_HIP(hipMalloc(&w[j].p, sizeof(double) * (size_t)ndir[j] * dsize[j] * dsize[j]));
+    if (int r = shx::meyer_filters_device(dsize[j], dcomp[j], w[j].as<double>(), nullptr)) return r;
+    wp[j] = w[j].as<double>();
+  }
+  return prepare_dict(L, nscales, ndir.data(), dsize, wp.data(), true, shift, flags, out);
 }
 
 int shx_clear_shearing_filters(shx_dict* d) {

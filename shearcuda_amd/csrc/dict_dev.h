@@ -12,5 +12,8 @@ namespace shx {
 // Returns SHX_OK, or an error code with the message set.
 int spectra_device(int L, int D, int n1, const double* w_host, int shift, float2* main_out, float2* beta_out,
                    int scale);
+// The same from filters already in device memory (w_dev: D x n1 x n1 FP64).
+int spectra_device_from_dev(int L, int D, int n1, const double* w_dev, int shift, float2* main_out,
+                            float2* beta_out, int scale);
 
 }  // namespace shx

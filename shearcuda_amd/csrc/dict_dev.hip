@@ -105,8 +105,24 @@ __global__ void __launch_bounds__(ST) k_spec_cols(const double2* __restrict__ G,
 
 }  // namespace
 
+namespace {
+int spectra_impl(int L, int D, int n1, const double* w_src, bool src_on_device, int shift, float2* main_out,
+                 float2* beta_out, int scale);
+}  // namespace
+
 int spectra_device(int L, int D, int n1, const double* w_host, int shift, float2* main_out, float2* beta_out,
                    int scale) {
+  return spectra_impl(L, D, n1, w_host, false, shift, main_out, beta_out, scale);
+}
+
+int spectra_device_from_dev(int L, int D, int n1, const double* w_dev, int shift, float2* main_out,
+                            float2* beta_out, int scale) {
+  return spectra_impl(L, D, n1, w_dev, true, shift, main_out, beta_out, scale);
+}
+
+namespace {
+int spectra_impl(int L, int D, int n1, const double* w_src, bool src_on_device, int shift, float2* main_out,
+                 float2* beta_out, int scale) {
   const int N = L / 2, NC = N + 1;
   const int s = shift ? n1 / 2 : 0;
   std::vector<double2> tw(L);
@@ -125,7 +141,8 @@ int spectra_device(int L, int D, int n1, const double* w_host, int shift, float2
   int r = SHX_OK;
   if (e != hipSuccess) r = fail(SHX_ERR_ALLOC, "spectra workspace allocation failed: %s", hipGetErrorString(e));
   int hbad = 0;
-  if (r == SHX_OK && ((e = hipMemcpy(w, w_host, bw, hipMemcpyHostToDevice)) != hipSuccess ||
+  if (r == SHX_OK && ((e = hipMemcpy(w, w_src, bw, src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice)) !=
+                          hipSuccess ||
                       (e = hipMemcpy(twd, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice)) != hipSuccess ||
                       (e = hipMemcpy(bad, &hbad, sizeof(int), hipMemcpyHostToDevice)) != hipSuccess))
     r = fail(SHX_ERR_HIP, "spectra upload failed: %s", hipGetErrorString(e));
@@ -145,5 +162,6 @@ int spectra_device(int L, int D, int n1, const double* w_host, int shift, float2
   if (bad) (void)hipFree(bad);
   return r;
 }
+}  // namespace
 
 }  // namespace shx
