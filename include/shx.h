@@ -161,6 +161,32 @@ int shx_inverse_shear_adjoint(shx_dict* dict, const float* x_bar, float* const* 
 int shx_batch_shear_trans_adjoint(shx_batch* batch, const float* coeff_bar, float* x_bar, void* stream);
 int shx_batch_inverse_shear_adjoint(shx_batch* batch, const float* x_bar, float* coeff_bar, void* stream);
 
+/* Soft-threshold shrinkage, one step of the unrolled separation
+ * (shearcuda/separate.m:162-257 with the threshold of
+ * shearcuda/SoftThresh.m:11-13, per band):
+ *   out = S(soft(A x, th)),  soft(c, t) = sign(c) max(|c| - t, 0).
+ * The soft threshold is applied as the reconstruction reads each band and the
+ * lowpass (no thresholded copy of the coefficients is made); the result is
+ * shx_soft_bands followed by shx_batch_inverse_shear up to FP32 rounding
+ * (the fused row pass splits its FFT differently; thresholded values are
+ * identical).
+ * th_dev: device FP32, th_dev[i * th_stride + p] is the threshold of band p
+ * (lowpass p = 0) of image i; th_stride 0 shares one row among the images,
+ * otherwise th_stride >= 1 + nbands.  coeff (may be NULL) receives A x, the
+ * unthresholded stack, which the backward takes.
+ * Backward, given those coefficients c and the output gradient out_bar:
+ *   x_bar = A^T(1[|c| > t] . S^T out_bar),
+ *   gth_dev[i * th_stride + p] += sum over band p of image i of
+ *                                 -sign(c) 1[|c| > t] (S^T out_bar)
+ * (th_stride 0: every image adds into the one row; gth_dev is not cleared).
+ * The mask and the threshold gradient are applied between S^T's last row
+ * pass and A^T's first, in one kernel per band row.  No counterpart in the
+ * reference (learned thresholds). */
+int shx_batch_shrink(shx_batch* batch, const float* x, float* coeff, const float* th_dev, int th_stride, float* out,
+                     void* stream);
+int shx_batch_shrink_backward(shx_batch* batch, const float* coeff, const float* th_dev, int th_stride,
+                              const float* out_bar, float* x_bar, float* gth_dev, void* stream);
+
 /* Autotuner: times the pyramid tile shapes, the reconstruction's
  * directions-per-block grouping, the column tile width of the
  * decomposition and reconstruction column passes and the plan options
@@ -173,7 +199,8 @@ int shx_batch_inverse_shear_adjoint(shx_batch* batch, const float* x_bar, float*
  * reference. */
 int shx_autotune(shx_dict* dict, shx_batch* batch, int reps, int* chosen, void* stream);
 /* Sets a schedule explicitly (pyramid variant 0..2, group 2/4/8, column tiles
- * 4 (L >= 128), 8, or 16 (L <= 512); 0 = default). */
+ * 4 (L >= 128), 8, or 16 (L <= 512); 0 = default; negative: the three-pass
+ * column FFT with |ct| columns, -4 or -8 at L = 512, -2 or -4 at L = 1024). */
 int shx_set_schedule(shx_dict* dict, shx_batch* batch, int pyr_variant, int group, int ct_dec, int ct_rec);
 /* Plan options (both picked by shx_autotune; defaults 0, 0; L >= 64 for 1).
  * With the pyramid's half-sample symmetric extension every a trous level is

@@ -53,6 +53,9 @@ _SIGNATURES = [
     ("shx_inverse_shear_adjoint", c_int, [c_void_p, c_void_p, P(c_void_p), c_void_p]),
     ("shx_batch_shear_trans_adjoint", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("shx_batch_inverse_shear_adjoint", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("shx_batch_shrink", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    ("shx_batch_shrink_backward", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                          c_void_p]),
     ("shx_autotune", c_int, [c_void_p, c_void_p, c_int, P(c_int), c_void_p]),
     ("shx_set_schedule", c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int]),
     ("shx_set_plan_options", c_int, [c_void_p, c_void_p, c_int, c_int]),

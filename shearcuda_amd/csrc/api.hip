@@ -88,6 +88,7 @@ struct Work {
 
 struct shx_dict {
   int L = 0, S = 0, nbands = 0, shift = 0, device = -1;
+  int nfilt = 0;  // stored filter spectrum planes (scales with identical filters share theirs)
   int ndir[SHX_MAX_SCALES] = {};
   int dsize[SHX_MAX_SCALES] = {};
   BandMap map{};
@@ -309,6 +310,27 @@ __global__ void __launch_bounds__(256) k_soft_bands_bwd(const float* __restrict_
     if (acc[b] != 0.f) atomicAdd(&gth[b], acc[b]);
 }
 
+// Lowpass part of the shrink backward, in place on g: g = g * 1[|c| > t] and
+// gth[img * stride] += sum -sign(c) 1[|c| > t] g over image img's plane of n
+// elements (c at img * cs, g at img * gs).  grid (blocks, nimg).
+__global__ void __launch_bounds__(256) k_low_mask_bwd(const float* __restrict__ c, long long cs, float* __restrict__ g,
+                                                      long long gs, long long n, const float* __restrict__ th,
+                                                      int stride, float* __restrict__ gth) {
+  const int img = blockIdx.y;
+  c += (size_t)img * cs;
+  g += (size_t)img * gs;
+  const float t = th[(size_t)img * stride];
+  float acc = 0.f;
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+    const float v = c[e], gv = g[e];
+    const bool keep = fabsf(v) > t;
+    g[e] = keep ? gv : 0.f;
+    acc += keep ? (v > 0.f ? -gv : gv) : 0.f;
+  }
+  acc = block_reduce256(acc, [](float a, float b) { return a + b; });
+  if (threadIdx.x == 0 && acc != 0.f) atomicAdd(&gth[(size_t)img * stride], acc);
+}
+
 // Per-band hard threshold y = x * 1[|x| > t] and its input gradient
 // gx = gy * 1[|x| > t] (the threshold has no gradient almost everywhere).
 __global__ void k_hard_bands(const float* __restrict__ x, const float* __restrict__ gy, float* __restrict__ y,
@@ -341,7 +363,7 @@ hipError_t pyr_rec(const float* const* y, int L, int nlev, float* out, float* tA
     const float sx = (first && w) ? (float)w[0] : 1.0f;
     const float sy = w ? (float)w[j] : 1.0f;
     hipError_t e = shx::launch_atrous_rec(xl, y[j], o, shx::ImgStrides{0, 0, 0}, 1, L, 1 << lev,
-                                          shx::LowThresh{0, 0.f, nullptr, 0}, sx, sy, s);
+                                          shx::LowThresh{0, 0.f, nullptr, 0, 0}, sx, sy, s);
     if (e != hipSuccess) return e;
     xl = o;
   }
@@ -471,11 +493,18 @@ struct Stage {
 
 // Thresholds of a reconstruction: none, host values (one image, kernel
 // arguments) or a device table [nimg][1 + nbands] (lowpass first).
+// The device table's row stride is 1 + nbands unless `stride` says otherwise
+// (0: one row for every image); soft selects sign(c) max(|c| - t, 0) over the
+// hard keep rule.
 struct Thr {
   int mode = 0;
   const float* host = nullptr;
   const float* dev = nullptr;
+  int stride = -1;
+  bool soft = false;
 };
+
+int thr_stride(const shx_dict* d, const Thr& t) { return t.stride >= 0 ? t.stride : 1 + d->nbands; }
 
 void stages_pyr_dec(shx_dict* d, const Work& w, int nimg, const float* x, long long xs, const Coeffs& c,
                     hipStream_t s, std::vector<Stage>& st) {
@@ -564,11 +593,12 @@ void stages_front_dec(shx_dict* d, const Work& w, int nimg, const float* x, long
 shx::ThreshSpec band_thresh(const shx_dict* d, const Thr& t) {
   shx::ThreshSpec th{};
   th.mode = t.mode;
+  th.soft = t.soft ? 1 : 0;
   if (t.mode == 1)
     for (int b = 0; b < d->nbands; ++b) th.arg[b] = t.host[1 + b];
   if (t.mode == 2) {
     th.dev = t.dev;
-    th.dev_stride = 1 + d->nbands;
+    th.dev_stride = thr_stride(d, t);
     th.dev_off = 1;
   }
   return th;
@@ -593,11 +623,12 @@ void stages_spec_rec_tail(shx_dict* d, const Work& w, int nimg, hipStream_t s, s
                 [=] { return shx::launch_rows_c2r(L, qset, yset, S, nimg, sf.data(), d->tw, s); }});
 }
 
-// The lowpass hard threshold of a reconstruction (lowpass first in the tables).
+// The lowpass threshold of a reconstruction (lowpass first in the tables).
 shx::LowThresh low_thresh(const shx_dict* d, const Thr& t) {
-  shx::LowThresh lt{0, 0.f, nullptr, 0};
-  if (t.mode == 1) lt = shx::LowThresh{1, t.host[0], nullptr, 0};
-  if (t.mode == 2) lt = shx::LowThresh{2, 0.f, t.dev, 1 + d->nbands};
+  const int soft = t.soft ? 1 : 0;
+  shx::LowThresh lt{0, 0.f, nullptr, 0, 0};
+  if (t.mode == 1) lt = shx::LowThresh{1, t.host[0], nullptr, 0, soft};
+  if (t.mode == 2) lt = shx::LowThresh{2, 0.f, t.dev, thr_stride(d, t), soft};
   return lt;
 }
 
@@ -615,7 +646,7 @@ void stages_pyr_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const
     float* o = last ? x : w.planes + (size_t)(S + (lev & 1)) * pl;
     const long long os = last ? xs : w.s_planes;
     const float* yh = w.planes + (size_t)(j - 1) * pl;
-    const shx::LowThresh tl = (lev == S - 1) ? lt : shx::LowThresh{0, 0.f, nullptr, 0};
+    const shx::LowThresh tl = (lev == S - 1) ? lt : shx::LowThresh{0, 0.f, nullptr, 0, 0};
     shx::ImgStrides is{xls, w.s_planes, os};
     st.push_back({"atrous_rec_m" + std::to_string(m), 3 * P, [=] {
                     return shx::launch_atrous_rec(xl, yh, o, is, nimg, L, m, tl, 1.0f, 1.0f, s, w.pyr_variant);
@@ -760,6 +791,12 @@ void stages_denoise_dag(shx_dict* d, Work& w, int nimg, const float* x, long lon
     m.group_first[0] = 0;
     m.group_first[1] = ng;
     for (int g = 0; g <= ng; ++g) m.group_band0[g] = (unsigned short)(w.map.group_band0[g0 + g] - f);
+    for (int p = 0; p < nb; ++p) {
+      m.filt_of[p] = d->map.filt_of[f + p];
+      m.order[p] = (unsigned char)p;
+    }
+    for (int g = 0; g < ng; ++g) m.gorder[g] = (unsigned short)g;
+    m.share = m.gshare = 1;
     shx::ThreshSpec th = thall;
     if (th.mode == 1)
       for (int p = 0; p < nb; ++p) th.arg[p] = thall.arg[f + p];
@@ -772,8 +809,8 @@ void stages_denoise_dag(shx_dict* d, Work& w, int nimg, const float* x, long lon
     const float2* Y1 = w.spec + (size_t)j * L * NH;
     float2* T = w.band + (size_t)f * L * NH;
     float2* Q = w.q + (size_t)g0 * L * NH;
-    const float2* Hm = d->Hmain + (size_t)f * L * NH;
-    const float2* Hb = d->Hbeta + (size_t)f * L;
+    const float2* Hm = d->Hmain;  // the sub-map's filt_of indexes the whole table
+    const float2* Hb = d->Hbeta;
     shx::ThreshSpec none{};
     const std::string sfx = "_s" + std::to_string(j);
     if (!w.dct_front) {
@@ -908,6 +945,45 @@ void stages_rec_adjoint(shx_dict* d, const Work& w, int nimg, const float* gx, l
   PlaneSet out = c.bands;
   st.push_back({"rows_c2r_bands", 2.0 * nb * P,
                 [=] { return shx::launch_rows_c2r(L, bandset, out, nb, nimg, nullptr, d->tw, s); }});
+}
+
+// Backward of the soft-threshold shrink out = S(soft(A x, th)) given the
+// forward coefficients c: S^T out_bar to the band spectra, the mask
+// 1[|c| > t] and the threshold gradient between S^T's last row pass and A^T's
+// first (one fused kernel per band row), the lowpass's mask in place, then
+// A^T.  The lowpass gradient lives in workspace plane S + 2.
+void stages_shrink_backward(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const Thr& t, const float* gout,
+                            long long gos, float* gth, float* gx, long long gxs, hipStream_t s,
+                            std::vector<Stage>& st) {
+  const int L = d->L, S = d->S, nb = d->nbands;
+  const long long pl = (long long)L * L;
+  const double P = 4.0 * pl * nimg;
+  Coeffs g{};
+  g.low = w.planes + (size_t)(S + 2) * pl;
+  g.low_stride = w.s_planes;
+  stages_pyr_rec_adjoint(d, w, nimg, gout, gos, g, s, st);
+  stages_spec_dec(d, w, nimg, s, st);
+  PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
+  PlaneSet cb = c.bands;
+  shx::ThreshSpec th = band_thresh(d, t);
+  // reads T and the coefficients, writes U in place of T
+  st.push_back({"rows_c2r_mask_r2c_bands", 3.0 * nb * P, [=] {
+                  return shx::launch_rows_c2r_mask_r2c(L, bandset, cb, bandset, nb, nimg, th, gth, d->tw, s);
+                }});
+  const float* clow = c.low;
+  const long long cls = c.low_stride;
+  float* glow = g.low;
+  const long long gls = g.low_stride;
+  const float* thd = t.dev;
+  const int ts = thr_stride(d, t);
+  st.push_back({"low_mask", 3.0 * P, [=] {
+                  const int blocks = (int)std::min<long long>(256, (pl + 255) / 256);
+                  hipLaunchKernelGGL(k_low_mask_bwd, dim3(blocks, nimg), dim3(256), 0, s, clow, cls, glow, gls, pl,
+                                     thd, ts, gth);
+                  return hipGetLastError();
+                }});
+  stages_spec_rec_tail(d, w, nimg, s, st);
+  stages_pyr_dec_adjoint(d, w, nimg, g, gx, gxs, s, st);
 }
 
 int run_stages(const std::vector<Stage>& st) {
@@ -1083,12 +1159,14 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t caller, 
     if (tp[v] < tp[bestv]) bestv = v;
   w.pyr_variant = bestv;
   // reconstruction: directions per block x column tile; decomposition: tile
+  // column tiles: two-pass engine 4/8/16 columns, three-pass engines (-8/-4/-2)
   const int gcand[3] = {2, 4, 8};
-  int ccand[3], nct = 0;
-  for (int ct : {8, 16, 4})
+  int ccand[6], nct = 0;
+  for (int ct : {8, 16, 4, -8, -4, -2})
     if (shx::col_tile_ok(d->L, ct)) ccand[nct++] = ct;
-  double tg[3][3];
-  for (auto& r : tg) r[0] = r[1] = r[2] = 1e30;
+  double tg[3][6];
+  for (auto& r : tg)
+    for (double& t : r) t = 1e30;
   for (int round = 0; round < 3; ++round)
     for (int i = 0; i < 3; ++i)
       for (int c = 0; c < nct; ++c) {
@@ -1105,7 +1183,7 @@ int autotune_work(shx_dict* d, Work& w, int nimg, int reps, hipStream_t caller, 
       if (tg[i][c] < tg[bi][bc]) bi = i, bc = c;
   set_group(d, w, gcand[bi]);
   w.ct_rec = ccand[bc];
-  double td[3] = {1e30, 1e30, 1e30};
+  double td[6] = {1e30, 1e30, 1e30, 1e30, 1e30, 1e30};
   for (int round = 0; round < 3; ++round)
     for (int c = 0; c < nct; ++c) {
       w.ct_dec = ccand[c];
@@ -1199,7 +1277,7 @@ namespace {
 // (w_on_device: the spectra then come from them without a host round trip,
 // except that the host-spectra flag copies them down first).
 int prepare_dict(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s, bool w_on_device,
-                 int shift, int flags, shx_dict** out) {
+                 const int* same_as, int shift, int flags, shx_dict** out) {
   if (!out) return fail(SHX_ERR_ARG, "null output pointer");
   if (flags & ~SHX_DICT_HOST_SPECTRA) return fail(SHX_ERR_ARG, "unknown dictionary flags 0x%x", flags);
   const bool host_spectra = (flags & SHX_DICT_HOST_SPECTRA) != 0;
@@ -1235,16 +1313,44 @@ int prepare_dict(int L, int nscales, const int* ndir, const int* dsize, const do
   d->map.first[nscales] = nb;
   d->map.nscales = nscales;
   d->map.nbands = nb;
+  // scales with identical filters (same direction count and size, equal
+  // values) read one stored set of spectra
+  std::vector<int> base(nscales, -1);
+  for (int j = 0; j < nscales; ++j) {
+    int src = -1;
+    for (int i = 0; i < j && src < 0; ++i) {
+      if (base[i] < 0 || ndir[i] != ndir[j] || dsize[i] != dsize[j]) continue;
+      if (same_as) {
+        if (same_as[j] == i) src = i;
+      } else if (!w_on_device &&
+                 memcmp(w_s[i], w_s[j], sizeof(double) * (size_t)ndir[j] * dsize[j] * dsize[j]) == 0) {
+        src = i;
+      }
+    }
+    base[j] = src >= 0 ? base[src] : d->nfilt;
+    if (src < 0) d->nfilt += ndir[j];
+    for (int k = 0; k < ndir[j]; ++k) d->map.filt_of[d->map.first[j] + k] = (unsigned char)(base[j] + k);
+  }
+  if (d->nfilt > 255) {
+    delete d;
+    return fail(SHX_ERR_ARG, "%d distinct filter planes (at most 255)", d->nfilt);
+  }
   shx::band_map_groups(d->map, 4);
+  // the first scale with a given set of planes computes their spectra
+  std::vector<bool> owner(nscales, true);
+  for (int j = 0; j < nscales; ++j)
+    for (int i = 0; i < j; ++i)
+      if (base[i] == base[j]) owner[j] = false;
   d->nbands = nb;
   if (hipGetDevice(&d->device) != hipSuccess) {
     delete d;
     return fail(SHX_ERR_HIP, "no HIP device");
   }
-  const int N = L / 2;
-  std::vector<float> hm(host_spectra ? (size_t)nb * L * N * 2 : 0), hb(host_spectra ? (size_t)nb * L * 2 : 0);
+  const int N = L / 2, nf = d->nfilt;
+  std::vector<float> hm(host_spectra ? (size_t)nf * L * N * 2 : 0), hb(host_spectra ? (size_t)nf * L * 2 : 0);
   std::vector<float> tw((size_t)L * 2);
   for (int j = 0; host_spectra && j < nscales; ++j) {
+    if (!owner[j]) continue;
     std::vector<std::complex<double>> H;
     std::vector<double> wh;
     const double* wj = w_s[j];
@@ -1262,8 +1368,7 @@ int prepare_dict(int L, int nscales, const int* ndir, const int* dsize, const do
         delete d;
         return fail(SHX_ERR_ARG, "scale %d: filter spectra vanish at some frequency (division by zero)", j);
       }
-    shx::pack_spectra_fp32(L, ndir[j], H, hm.data() + (size_t)d->map.first[j] * L * N * 2,
-                           hb.data() + (size_t)d->map.first[j] * L * 2);
+    shx::pack_spectra_fp32(L, ndir[j], H, hm.data() + (size_t)base[j] * L * N * 2, hb.data() + (size_t)base[j] * L * 2);
   }
   shx::twiddles_fp32(L, tw.data());
   // DCT-domain front end (L >= 64): masks and twiddles
@@ -1276,7 +1381,7 @@ int prepare_dict(int L, int nscales, const int* ndir, const int* dsize, const do
     shx::dct_pyramid_masks(L, nscales, 1, rmask.data());
     shx::dct_twiddles_fp32(L, dtw.data());
   }
-  const size_t bytes_hm = sizeof(float2) * (size_t)nb * L * N, bytes_hb = sizeof(float2) * (size_t)nb * L;
+  const size_t bytes_hm = sizeof(float2) * (size_t)nf * L * N, bytes_hb = sizeof(float2) * (size_t)nf * L;
   const size_t bytes_dm = dmask.size() * sizeof(float), bytes_dt = dtw.size() * sizeof(float);
   hipError_t e = hipSuccess;
   if ((e = hipMalloc(&d->Hmain, bytes_hm)) != hipSuccess || (e = hipMalloc(&d->Hbeta, bytes_hb)) != hipSuccess ||
@@ -1293,8 +1398,9 @@ int prepare_dict(int L, int nscales, const int* ndir, const int* dsize, const do
   }
   d->bytes = bytes_hm + bytes_hb + tw.size() * sizeof(float) + 2 * bytes_dm + bytes_dt + d->w.bytes;
   for (int j = 0; !host_spectra && j < nscales; ++j) {
-    float2* hm_j = d->Hmain + (size_t)d->map.first[j] * L * N;
-    float2* hb_j = d->Hbeta + (size_t)d->map.first[j] * L;
+    if (!owner[j]) continue;
+    float2* hm_j = d->Hmain + (size_t)base[j] * L * N;
+    float2* hb_j = d->Hbeta + (size_t)base[j] * L;
     const int r = w_on_device ? shx::spectra_device_from_dev(L, ndir[j], dsize[j], w_s[j], shift, hm_j, hb_j, j)
                               : shx::spectra_device(L, ndir[j], dsize[j], w_s[j], shift, hm_j, hb_j, j);
     if (r) {
@@ -1320,7 +1426,7 @@ extern "C" {
 
 int shx_prepare_shearing_filters_ex(int L, int nscales, const int* ndir, const int* dsize, const double* const* w_s,
                                     int shift, int flags, shx_dict** out) {
-  return prepare_dict(L, nscales, ndir, dsize, w_s, false, shift, flags, out);
+  return prepare_dict(L, nscales, ndir, dsize, w_s, false, nullptr, shift, flags, out);
 }
 
 int shx_meyer_filters(int n1, int level, double* w_dev, void* stream) {
@@ -1336,17 +1442,24 @@ int shx_shearing_filters_myer(int L, int nscales, const int* dcomp, const int* d
     return fail(SHX_ERR_ARG, "number of scales %d outside [1, %d]", nscales, SHX_MAX_SCALES);
   std::vector<DevMem> w(nscales);
   std::vector<const double*> wp(nscales);
-  std::vector<int> ndir(nscales);
+  std::vector<int> ndir(nscales), same_as(nscales, -1);
   for (int j = 0; j < nscales; ++j) {
+    // equal parameters give equal filters: compute them once
+    for (int i = 0; i < j && same_as[j] < 0; ++i)
+      if (dcomp[i] == dcomp[j] && dsize[i] == dsize[j]) same_as[j] = same_as[i] >= 0 ? same_as[i] : i;
     if (dcomp[j] < 0 || (1 << dcomp[j]) > SHX_MAX_DIRS)
       return fail(SHX_ERR_ARG, "scale %d: 2^%d directions (at most %d)", j, dcomp[j], SHX_MAX_DIRS);
     if (dsize[j] < 2 || dsize[j] > L) return fail(SHX_ERR_ARG, "scale %d: filter size %d invalid for L=%d", j, dsize[j], L);
     ndir[j] = 1 << dcomp[j];
+    if (same_as[j] >= 0) {
+      wp[j] = wp[same_as[j]];
+      continue;
+    }
     SHX_HIP(hipMalloc(&w[j].p, sizeof(double) * (size_t)ndir[j] * dsize[j] * dsize[j]));
     if (int r = shx::meyer_filters_device(dsize[j], dcomp[j], w[j].as<double>(), nullptr)) return r;
     wp[j] = w[j].as<double>();
   }
-  return prepare_dict(L, nscales, ndir.data(), dsize, wp.data(), true, shift, flags, out);
+  return prepare_dict(L, nscales, ndir.data(), dsize, wp.data(), true, same_as.data(), shift, flags, out);
 }
 
 int shx_clear_shearing_filters(shx_dict* d) {
@@ -1536,6 +1649,45 @@ int shx_batch_denoise(shx_batch* b, const float* x, float* coeff, const float* t
   stages_denoise_dag(d, b->w, b->nimg, x, pl, coeff ? stack_coeffs(d, coeff) : work_coeffs(d, b->w), t, out, pl,
                      (hipStream_t)stream, st);
   return run_stage_dag(st, b->w);
+}
+
+static int shrink_thr(shx_batch* b, const float* th_dev, int th_stride, Thr& t) {
+  if (!th_dev) return fail(SHX_ERR_ARG, "null threshold table");
+  if (th_stride != 0 && th_stride < 1 + b->d->nbands)
+    return fail(SHX_ERR_ARG, "threshold row stride %d (0, or at least 1 + nbands = %d)", th_stride,
+                1 + b->d->nbands);
+  t.mode = 2;
+  t.dev = th_dev;
+  t.stride = th_stride;
+  t.soft = true;
+  return SHX_OK;
+}
+
+int shx_batch_shrink(shx_batch* b, const float* x, float* coeff, const float* th_dev, int th_stride, float* out,
+                     void* stream) {
+  if (int r = batch_args(b, x, out, out)) return r;
+  Thr t;
+  if (int r = shrink_thr(b, th_dev, th_stride, t)) return r;
+  shx_dict* d = b->d;
+  std::vector<Stage> st;
+  const long long pl = (long long)d->L * d->L;
+  stages_denoise_dag(d, b->w, b->nimg, x, pl, coeff ? stack_coeffs(d, coeff) : work_coeffs(d, b->w), t, out, pl,
+                     (hipStream_t)stream, st);
+  return run_stage_dag(st, b->w);
+}
+
+int shx_batch_shrink_backward(shx_batch* b, const float* coeff, const float* th_dev, int th_stride,
+                              const float* out_bar, float* x_bar, float* gth_dev, void* stream) {
+  if (int r = batch_args(b, coeff, out_bar, x_bar)) return r;
+  if (!gth_dev) return fail(SHX_ERR_ARG, "null threshold gradient");
+  Thr t;
+  if (int r = shrink_thr(b, th_dev, th_stride, t)) return r;
+  shx_dict* d = b->d;
+  std::vector<Stage> st;
+  const long long pl = (long long)d->L * d->L;
+  stages_shrink_backward(d, b->w, b->nimg, stack_coeffs(d, const_cast<float*>(coeff)), t, out_bar, pl, gth_dev,
+                         x_bar, pl, (hipStream_t)stream, st);
+  return run_stages(st);
 }
 
 int shx_autotune(shx_dict* d, shx_batch* b, int reps, int* chosen, void* stream) {

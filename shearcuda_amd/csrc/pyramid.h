@@ -29,14 +29,22 @@ struct ImgStrides {
   long long a, b, c;
 };
 
-// Hard threshold applied to the rec level's xlow input: none, one value, or
-// one per image in device memory (dev[img * stride]).
+// Threshold applied to the rec level's xlow input: none, one value, or one
+// per image in device memory (dev[img * stride]; stride 0: one for all);
+// hard or soft (soft != 0).
 struct LowThresh {
   int mode;  // 0 none, 1 v, 2 dev
   float v;
   const float* dev;
   int stride;
+  int soft;
 };
+
+// ShrinkRule of a lowpass read.
+__host__ __device__ inline int low_rule(const LowThresh& lt) { return lt.mode == 0 ? 0 : (lt.soft ? 2 : 1); }
+__device__ inline float low_value(const LowThresh& lt, int img) {
+  return lt.mode == 1 ? lt.v : (lt.mode == 2 ? lt.dev[(size_t)img * lt.stride] : 0.0f);
+}
 
 constexpr int PYR_VARIANTS = 3;  // tile shapes the autotuner chooses from
 

@@ -129,11 +129,12 @@ __device__ __forceinline__ void tile_fetch(float (&v)[NIT], const float* __restr
 }
 
 template <int TH, int TW, int T, int NIT>
-__device__ __forceinline__ void tile_store(float* __restrict__ tile, const float (&v)[NIT], float th, float scale) {
+__device__ __forceinline__ void tile_store(float* __restrict__ tile, const float (&v)[NIT], float th, int rule,
+                                           float scale) {
   using TC = TileCfg<TH, TW, T>;
   static_assert(NIT == TC::NIT, "register count of the tile");
 #pragma unroll
-  for (int i = 0; i < TC::NIT; ++i) tile[threadIdx.x + i * T] = (th >= 0.f ? hard_keep(v[i], th) : v[i]) * scale;
+  for (int i = 0; i < TC::NIT; ++i) tile[threadIdx.x + i * T] = shrink(v[i], th, rule) * scale;
 }
 
 // Output grid of a level: the image (forward) or the extended grid of the
@@ -186,7 +187,7 @@ __global__ void __launch_bounds__(T) k_pyr_one2two(const float* __restrict__ x, 
   float v[TileCfg<TH, TW, T>::NIT];
   tile_fetch<TH, TW, T, ADJ>(v, x, L, m, pr, pc, R0 - shift, C0 - shift);
   __builtin_amdgcn_sched_barrier(0);  // the tile's loads in flight together
-  tile_store<TH, TW, T>(tile, v, -1.0f, 1.0f);
+  tile_store<TH, TW, T>(tile, v, 0.0f, SHRINK_NONE, 1.0f);
   __syncthreads();
   // shared vertical passes: tv[i] = u_i (x) tile, every tile column
   for (int it = threadIdx.x; it < W * (TH / VS); it += T) {
@@ -258,7 +259,7 @@ __global__ void __launch_bounds__(T) k_pyr_two2one(const float* __restrict__ xa,
   xa += (size_t)img * st3.a;
   xb += (size_t)img * st3.b;
   out += (size_t)img * st3.c;
-  const float th = lt.mode == 0 ? -1.0f : (lt.mode == 1 ? lt.v : lt.dev[(size_t)img * lt.stride]);
+  const float th = low_value(lt, img);
   const int E = out_side(L, m, ADJ);
   const int nout = E / m;
   const int R0 = by * TH, C0 = bx * TW;
@@ -267,8 +268,8 @@ __global__ void __launch_bounds__(T) k_pyr_two2one(const float* __restrict__ xa,
   tile_fetch<TH, TW, T, ADJ>(va, xa, L, m, pr, pc, R0 - shift, C0 - shift);
   tile_fetch<TH, TW, T, ADJ>(vb, xb, L, m, pr, pc, R0 - shift, C0 - shift);
   __builtin_amdgcn_sched_barrier(0);  // both tiles' loads in flight together
-  tile_store<TH, TW, T>(tile[0], va, th, sa);
-  tile_store<TH, TW, T>(tile[1], vb, -1.0f, sb);
+  tile_store<TH, TW, T>(tile[0], va, th, low_rule(lt), sa);
+  tile_store<TH, TW, T>(tile[1], vb, 0.0f, SHRINK_NONE, sb);
   __syncthreads();
   // vertical passes of both inputs into the three shared intermediates:
   // tv[i] = a_i (x) tile0 + b_i (x) tile1
@@ -411,7 +412,7 @@ hipError_t launch_atrous_dec_adjoint(const float* glow, const float* ghigh, floa
   if (!level_ok(L, m, nimg)) return hipErrorInvalidValue;
   // Z = C_h0^T glow + C_h1^T ghigh on the extended grid, then fold-add
   ImgStrides sz{st.a, st.b, zstride};
-  hipError_t e = two2one_v<0, true, 0>(glow, ghigh, zext, sz, nimg, L, m, LowThresh{0, 0.f, nullptr, 0}, 1.0f,
+  hipError_t e = two2one_v<0, true, 0>(glow, ghigh, zext, sz, nimg, L, m, LowThresh{0, 0.f, nullptr, 0, 0}, 1.0f,
                                            1.0f, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_fold_add, dim3((L + 255) / 256, L, nimg), dim3(256), 0, s, zext, gx, L, HALO * m, zstride,

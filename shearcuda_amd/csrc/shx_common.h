@@ -33,6 +33,22 @@ __host__ __device__ __forceinline__ float2 cmul_i(float2 a) {
 // Hard threshold used everywhere on the path: keep iff |v| > th (strict).
 __device__ __forceinline__ float hard_keep(float v, float th) { return fabsf(v) > th ? v : 0.0f; }
 
+// Soft threshold sign(v) max(|v| - th, 0), in the operation order of the
+// stand-alone band kernel (so fused and unfused results agree bit for bit).
+__device__ __forceinline__ float soft_keep(float v, float th) {
+  const float a = fabsf(v) - th;
+  return a > 0.f ? copysignf(a, v) : 0.f;
+}
+
+// Threshold rule of a fused read: 0 none, 1 hard, 2 soft (wave-uniform).
+enum ShrinkRule { SHRINK_NONE = 0, SHRINK_HARD = 1, SHRINK_SOFT = 2 };
+__device__ __forceinline__ float shrink(float v, float th, int rule) {
+  return rule == SHRINK_SOFT ? soft_keep(v, th) : (rule == SHRINK_HARD ? hard_keep(v, th) : v);
+}
+__device__ __forceinline__ float2 shrink2(float2 v, float th, int rule) {
+  return make_float2(shrink(v.x, th, rule), shrink(v.y, th, rule));
+}
+
 __host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
 
 // Raw buffer view of `bytes` bytes from a wave-uniform base address.  An

@@ -15,12 +15,15 @@ struct PlaneSet {
   long long img_stride;  // floats between consecutive images (0: one image)
 };
 
-// Hard thresholds for a row pass: none, per band in the kernel arguments
-// (one image), or per image and band in device memory.
+// Thresholds for a row pass: none, per band in the kernel arguments (one
+// image), or per image and band in device memory (dev_stride 0: one row of
+// band thresholds for every image); hard (keep iff |c| > t) or soft
+// (sign(c) max(|c| - t, 0)).
 struct ThreshSpec {
   int mode;              // 0 none, 1 arg[p], 2 dev[b * dev_stride + dev_off + p]
   int dev_stride;
   int dev_off;
+  int soft;              // 0 hard, 1 soft
   const float* dev;
   float arg[SHX_MAX_BANDS];
 };
@@ -30,20 +33,31 @@ struct SumTable {
   int first[SHX_MAX_SCALES + 1];
 };
 
-// Band p -> scale, first band of each scale, and the direction groups the
-// reconstruction column pass splits each scale into.
+// Band p -> scale, first band of each scale, the stored filter plane of each
+// band, and the direction groups the reconstruction column pass splits each
+// scale into.  Scales whose spatial filters are identical share their
+// spectra: filt_of[p] is the plane of the filter spectrum table band p reads.
+// The column passes dispatch their planes in the orders below, so that the
+// `share` bands (and the `gshare` groups) reading the same filter planes are
+// neighbours in the grid and run on one XCD, whose L2 then serves the second
+// read of each filter tile.
 struct BandMap {
   int nscales;
   int nbands;
   int first[SHX_MAX_SCALES + 1];
   unsigned char scale_of[SHX_MAX_BANDS];
+  unsigned char filt_of[SHX_MAX_BANDS];
   int ngroups;
   int group_first[SHX_MAX_SCALES + 1];        // first group of each scale
   unsigned short group_band0[SHX_MAX_BANDS + 1];  // band range of group g: [band0[g], band0[g+1])
+  int share, gshare;                           // bands / groups per shared filter plane set (1: none)
+  unsigned char order[SHX_MAX_BANDS];          // dispatch slot -> band
+  unsigned short gorder[SHX_MAX_BANDS];        // dispatch slot -> group
 };
 
 // Fill the group fields of a BandMap: each scale's directions are split into
-// groups of at most `gsize` consecutive directions.
+// groups of at most `gsize` consecutive directions; then the dispatch orders
+// of bands and groups from filt_of.
 void band_map_groups(BandMap& m, int gsize);
 
 inline PlaneSet single_cell(float* base, int nplanes, long long img_stride = 0) {
@@ -66,6 +80,12 @@ hipError_t launch_rows_c2r(int L, const PlaneSet& in, const PlaneSet& out, int n
 // Denoise fusion: c = C2R(T) written to `band`, then U = R2C(keep(c, th)).
 hipError_t launch_rows_c2r_r2c(int L, const PlaneSet& T, const PlaneSet& band, const PlaneSet& U, int nplanes,
                                int nimg, const ThreshSpec& th, const float2* tw, hipStream_t s);
+// Backward of a soft-thresholded fused read (th.mode 2): U = R2C(C2R(T) *
+// 1[|c| > t]) with c the forward coefficients (cref), and the threshold
+// gradient sum -sign(c) 1[|c| > t] C2R(T) added into gth at the threshold's
+// index.
+hipError_t launch_rows_c2r_mask_r2c(int L, const PlaneSet& Tin, const PlaneSet& cref, const PlaneSet& U, int nplanes,
+                                    int nimg, const ThreshSpec& th, float* gth, const float2* tw, hipStream_t s);
 // ct: column tile width of the block (4, 8 or 16 where col_tile_ok; 0 = the
 // default for L).  pre: Y1 is already column-transformed (launch_cols_fwd).
 bool col_tile_ok(int L, int ct);

@@ -28,6 +28,10 @@
 // H_k(u,N) act through alpha = (H(u,0)+H(u,N))/2 (stored in slot 0) and
 // beta = (H(u,0)-H(u,N))/2 (stored separately):
 //   Q(u) = alpha(u) P(u) + beta(u) conj(P(-u)).
+#include <algorithm>
+#include <vector>
+
+#include "fft3.h"
 #include "fft_reg.h"
 #include "pyramid.h"
 #include "spectral.h"
@@ -43,6 +47,54 @@ void band_map_groups(BandMap& m, int gsize) {
   m.group_first[m.nscales] = g;
   m.group_band0[g] = (unsigned short)m.nbands;
   m.ngroups = g;
+  // bands: grouped by filter plane when every plane is read by the same number of bands
+  int nfilt = 0;
+  for (int p = 0; p < m.nbands; ++p) nfilt = std::max(nfilt, (int)m.filt_of[p] + 1);
+  std::vector<int> cnt(nfilt, 0);
+  for (int p = 0; p < m.nbands; ++p) ++cnt[m.filt_of[p]];
+  bool uniform = nfilt > 0 && m.nbands % nfilt == 0;
+  for (int f = 0; uniform && f < nfilt; ++f) uniform = cnt[f] == m.nbands / nfilt;
+  m.share = uniform ? m.nbands / nfilt : 1;
+  int k = 0;
+  if (m.share > 1) {
+    for (int f = 0; f < nfilt; ++f)
+      for (int p = 0; p < m.nbands; ++p)
+        if (m.filt_of[p] == f) m.order[k++] = (unsigned char)p;
+  } else {
+    for (int p = 0; p < m.nbands; ++p) m.order[p] = (unsigned char)p;
+  }
+  // groups: two groups share when they read the same sequence of filter planes
+  auto same = [&](int a, int b) {
+    const int na = m.group_band0[a + 1] - m.group_band0[a], nb = m.group_band0[b + 1] - m.group_band0[b];
+    if (na != nb) return false;
+    for (int i = 0; i < na; ++i)
+      if (m.filt_of[m.group_band0[a] + i] != m.filt_of[m.group_band0[b] + i]) return false;
+    return true;
+  };
+  std::vector<int> cls(g, -1);
+  std::vector<int> rep;
+  for (int a = 0; a < g; ++a) {
+    for (int c = 0; c < (int)rep.size() && cls[a] < 0; ++c)
+      if (same(rep[c], a)) cls[a] = c;
+    if (cls[a] < 0) {
+      cls[a] = (int)rep.size();
+      rep.push_back(a);
+    }
+  }
+  const int ncls = (int)rep.size();
+  std::vector<int> gcnt(ncls, 0);
+  for (int a = 0; a < g; ++a) ++gcnt[cls[a]];
+  bool guni = ncls > 0 && g % ncls == 0;
+  for (int c = 0; guni && c < ncls; ++c) guni = gcnt[c] == g / ncls;
+  m.gshare = guni ? g / ncls : 1;
+  k = 0;
+  if (m.gshare > 1) {
+    for (int c = 0; c < ncls; ++c)
+      for (int a = 0; a < g; ++a)
+        if (cls[a] == c) m.gorder[k++] = (unsigned short)a;
+  } else {
+    for (int a = 0; a < g; ++a) m.gorder[a] = (unsigned short)a;
+  }
 }
 
 __device__ __forceinline__ float* plane_ptr(const PlaneSet& s, int p, size_t plane_elems, int b) {
@@ -292,8 +344,8 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_r2c(PlaneSet in, PlaneSet
   const size_t pe = (size_t)L * L;
   const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe, b)) + (size_t)row0 * N;
   float2* dst = reinterpret_cast<float2*>(plane_ptr(out, p, pe, b)) + (size_t)row0 * N;
-  const bool use_th = th.mode != 0;
-  const float t = use_th ? thresh_of(th, p, b) : 0.0f;
+  const int rule = th.mode == 0 ? SHRINK_NONE : (th.soft ? SHRINK_SOFT : SHRINK_HARD);
+  const float t = rule ? thresh_of(th, p, b) : 0.0f;
   const int r = tid / R1, n1 = tid % R1;
   const bool validA = r < C::ROWS;
   float2 v[R2];
@@ -303,9 +355,9 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_r2c(PlaneSet in, PlaneSet
   }
   const auto ftw = row_fwd_tw<L, R1, R2>(n1, tid % R2, tw);
   __builtin_amdgcn_sched_barrier(0);  // data and twiddle loads in flight together
-  if (validA && use_th) {
+  if (validA && rule) {
 #pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) v[n2] = make_float2(hard_keep(v[n2].x, t), hard_keep(v[n2].y, t));
+    for (int n2 = 0; n2 < R2; ++n2) v[n2] = shrink2(v[n2], t, rule);
   }
   row_fwd_from_A<L, R1, R2, C::PADSH>(v, validA, r, n1, buf, dst, true, tid / R2, tid % R2, ftw);
 }
@@ -381,8 +433,8 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
   const bool store_c = band.ncells > 0;
   float2* cdst = store_c ? reinterpret_cast<float2*>(plane_ptr(band, p, pe, b)) + (size_t)row0 * N : nullptr;
   float2* udst = reinterpret_cast<float2*>(plane_ptr(Uout, p, pe, b)) + (size_t)row0 * N;
-  const bool use_th = th.mode != 0;
-  const float t = use_th ? thresh_of(th, p, b) : 0.0f;
+  const int rule = th.mode == 0 ? SHRINK_NONE : (th.soft ? SHRINK_SOFT : SHRINK_HARD);
+  const float t = rule ? thresh_of(th, p, b) : 0.0f;
   const int r = tid / R1, n1 = tid % R1, rB = tid / R2, k2 = tid % R2;
   // twiddles of the forward half (split (R2, R1), pass-A index k2), loaded
   // before any store of this block
@@ -396,11 +448,68 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
   for (int k1 = 0; k1 < R1; ++k1) {
     const float2 c = u[k1];
     if (store_c) cdst[(size_t)rB * N + k2 + R2 * k1] = c;
-    u[k1] = use_th ? make_float2(hard_keep(c.x, t), hard_keep(c.y, t)) : c;
+    u[k1] = shrink2(c, t, rule);
   }
   __syncthreads();  // pass-B reads of buf are complete
   // forward transform split (R2, R1): thread k2 is pass-A index n1' = k2
   row_fwd_from_A<L, R2, R1, C::PADSH>(u, true, rB, k2, buf, udst, tid < C::ROWS * R1, tid / R1, tid % R1, ftw);
+}
+
+// Backward of a soft-thresholded band read, fused the same way: g = C2R(T)
+// is the gradient of the thresholded coefficients; with c the coefficients
+// of the forward pass (cref) and t the band's threshold,
+//   U = R2C(g * 1[|c| > t]),   gth += sum over the block's rows of -sign(c) 1[|c| > t] g
+// (one atomic per block into gth[b * dev_stride + dev_off + p]).  th: mode 2.
+template <int L>
+__global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_mask_r2c(PlaneSet Tin, PlaneSet cref, PlaneSet Uout,
+                                                                     ThreshSpec th, float* __restrict__ gth,
+                                                                     const float2* __restrict__ tw) {
+  using C = RowCfg<L>;
+  constexpr int N = C::N, R1 = C::R1, R2 = C::R2, NW = C::T / 64;
+  static_assert(C::T % 64 == 0, "whole waves");
+  __shared__ float2 buf[C::LDS];
+  __shared__ float red[NW];
+  const int tid = threadIdx.x;
+  const int p = blockIdx.y, b = blockIdx.z;
+  const int row0 = blockIdx.x * C::ROWS;
+  const size_t pe = (size_t)L * L;
+  const float2* src = reinterpret_cast<const float2*>(plane_ptr(Tin, p, pe, b)) + (size_t)row0 * N;
+  const float2* csrc = reinterpret_cast<const float2*>(plane_ptr(cref, p, pe, b)) + (size_t)row0 * N;
+  float2* udst = reinterpret_cast<float2*>(plane_ptr(Uout, p, pe, b)) + (size_t)row0 * N;
+  const size_t ti = (size_t)b * th.dev_stride + th.dev_off + p;
+  const float t = th.dev[ti];
+  const int r = tid / R1, n1 = tid % R1, rB = tid / R2, k2 = tid % R2;
+  const auto ftw = row_fwd_tw<L, R2, R1>(k2, tid % R1, tw);
+  // the forward coefficients at this thread's pass-B positions, in flight
+  // during the inverse transform
+  float2 cv[R1];
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) cv[k1] = csrc[(size_t)rB * N + k2 + R2 * k1];
+  float2 u[R1];
+  auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
+    gather_row<N, R1, R2>(src + (size_t)rr * N, nn, x, xm);
+  };
+  row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+  float acc = 0.f;
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    const float2 g = u[k1], c = cv[k1];
+    const bool kx = fabsf(c.x) > t, ky = fabsf(c.y) > t;
+    u[k1] = make_float2(kx ? g.x : 0.f, ky ? g.y : 0.f);
+    acc += kx ? (c.x > 0.f ? -g.x : g.x) : 0.f;
+    acc += ky ? (c.y > 0.f ? -g.y : g.y) : 0.f;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((tid & 63) == 0) red[tid >> 6] = acc;
+  __syncthreads();  // pass-B reads of buf are complete; red is written
+  row_fwd_from_A<L, R2, R1, C::PADSH>(u, true, rB, k2, buf, udst, tid < C::ROWS * R1, tid / R1, tid % R1, ftw);
+  if (tid == 0) {
+    float sum = 0.f;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sum += red[w];
+    if (sum != 0.f) atomicAdd(&gth[ti], sum);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -753,7 +862,9 @@ __device__ __forceinline__ int map_u16(const unsigned short* a, int i) {
   const unsigned w = reinterpret_cast<const unsigned*>(a)[i >> 1];
   return (i & 1) ? (w >> 16) : (w & 0xffffu);
 }
-static_assert(offsetof(BandMap, scale_of) % 4 == 0 && offsetof(BandMap, group_band0) % 4 == 0,
+static_assert(offsetof(BandMap, scale_of) % 4 == 0 && offsetof(BandMap, group_band0) % 4 == 0 &&
+                  offsetof(BandMap, filt_of) % 4 == 0 && offsetof(BandMap, order) % 4 == 0 &&
+                  offsetof(BandMap, gorder) % 4 == 0,
               "BandMap tables must be dword aligned");
 
 // XCD-aware block order of the column passes: workgroups are dispatched to
@@ -762,17 +873,22 @@ static_assert(offsetof(BandMap, scale_of) % 4 == 0 && offsetof(BandMap, group_ba
 // (tile fastest, then image, then plane): neighbouring column tiles, which
 // share the 128-byte lines of narrow tiles, and the images sharing a filter
 // plane then run in the same L2.
-__device__ __forceinline__ void col_block_order(int& tile, int& plane, int& img) {
+// share > 1: the planes come in runs of `share` that read the same filter
+// planes (BandMap order / gorder); the run member is then the fastest index,
+// so the blocks reading one filter tile are dispatched together.
+__device__ __forceinline__ void col_block_order(int& tile, int& plane, int& img, int share = 1) {
   const int nt = gridDim.x, np = gridDim.y, ni = gridDim.z;
   const int total = nt * np * ni;
   int lid = blockIdx.x + nt * (blockIdx.y + np * blockIdx.z);
   if (total % 8 == 0) lid = (lid % 8) * (total / 8) + lid / 8;
   // the block index arithmetic runs on the vector unit; the results are
   // wave-uniform, so move them to scalar registers (scalar table lookups)
+  const int m = lid % share;
+  lid /= share;
   tile = __builtin_amdgcn_readfirstlane(lid % nt);
   const int r = lid / nt;
   img = __builtin_amdgcn_readfirstlane(r % ni);
-  plane = __builtin_amdgcn_readfirstlane(r / ni);
+  plane = __builtin_amdgcn_readfirstlane((r / ni) * share + m);
 }
 
 // Forward column FFT of every scale spectrum, in place: the column pass of the
@@ -808,10 +924,12 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_dec(const float2* _
   using C = ColCfg<L, CT_>;
   constexpr int NH = L / 2;
   __shared__ float2 buf[C::LDS];
-  int tile, p, b;
-  col_block_order(tile, p, b);
+  int tile, slot, b;
+  col_block_order(tile, slot, b, map.share);
+  const int p = map_u8(map.order, slot);
   const int c0 = tile * C::CT;
   const int j = map_u8(map.scale_of, p);
+  const int f = map_u8(map.filt_of, p);
   float2 u[C::R1], q[C::R1];
   const float2* src = Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0;
   const int k2 = threadIdx.x / C::CT;
@@ -819,15 +937,15 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_dec(const float2* _
   PassTw<C::R1> ivt;
   if constexpr (PRE) {
     col_load_B<L, CT_>(u, src);
-    col_load_h<L, CT_>(h, Hmain + (size_t)p * L * NH + c0);
+    col_load_h<L, CT_>(h, Hmain + (size_t)f * L * NH + c0);
     ivt = col_inv_tw<L>(k2, tw);
     __builtin_amdgcn_sched_barrier(0);  // all loads in flight before the arithmetic
   } else {
-    col_load_h<L, CT_>(h, Hmain + (size_t)p * L * NH + c0);
+    col_load_h<L, CT_>(h, Hmain + (size_t)f * L * NH + c0);
     ivt = col_inv_tw<L>(k2, tw);
     col_fwd_to_B<L, CT_>(u, src, buf, tw);
   }
-  col_filter<L, CT_, false, false>(u, q, c0, h, Hbeta + (size_t)p * L, buf);
+  col_filter<L, CT_, false, false>(u, q, c0, h, Hbeta + (size_t)f * L, buf);
   col_inv_from_B<L, CT_>(q, Tout + (size_t)b * t_stride + (size_t)p * L * NH + c0, buf, ivt);
 }
 
@@ -852,8 +970,9 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* _
   constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
   __shared__ float2 buf[C::LDS];
   __shared__ float2 accl[ColRecAcc<L, CT_>::SIZE];
-  int tile, g, b;
-  col_block_order(tile, g, b);
+  int tile, slot, b;
+  col_block_order(tile, slot, b, map.gshare);
+  const int g = map_u16(map.gorder, slot);
   const int c0 = tile * CT;
   const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
   const int p0 = map_u16(map.group_band0, g), p1 = map_u16(map.group_band0, g + 1);
@@ -864,13 +983,14 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* _
   for (int p = p0; p < p1; ++p) {
     float2 u[R1], h[R1];
     __syncthreads();  // previous direction's reads of buf are complete
+    const int f = map_u8(map.filt_of, p);
     col_fwd_A<L, CT_>(ub + (size_t)p * L * NH, buf, tw);
-    col_load_h<L, CT_>(h, Hmain + (size_t)p * L * NH + c0);  // in flight across pass B
+    col_load_h<L, CT_>(h, Hmain + (size_t)f * L * NH + c0);  // in flight across pass B
     __syncthreads();
     col_fwd_B<L, CT_>(u, buf);
     if constexpr (ColRecAcc<L, CT_>::LDS) {
       float2 q[R1];
-      col_filter<L, CT_, true, false>(u, q, c0, h, Hbeta + (size_t)p * L, buf);
+      col_filter<L, CT_, true, false>(u, q, c0, h, Hbeta + (size_t)f * L, buf);
       float2* row = accl + C::Lds::ix(k2, c);  // row k2 + R2 k1
       if (p == p0) {
 #pragma unroll
@@ -883,7 +1003,7 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* _
         }
       }
     } else {
-      col_filter<L, CT_, true, true>(u, acc, c0, h, Hbeta + (size_t)p * L, buf);
+      col_filter<L, CT_, true, true>(u, acc, c0, h, Hbeta + (size_t)f * L, buf);
     }
   }
   if constexpr (ColRecAcc<L, CT_>::LDS) {
@@ -892,6 +1012,241 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* _
     for (int k1 = 0; k1 < R1; ++k1) acc[k1] = row[k1 * (R2 / R1) * C::Lds::BLK];
   }
   col_inv_from_B<L, CT_>(acc, Q + (size_t)b * q_stride + (size_t)g * L * NH + c0, buf, col_inv_tw<L>(k2, tw));
+}
+
+// ---------------------------------------------------------------------------
+// Column passes on the three-pass 512-point engine (fft3.h): eight values per
+// thread instead of sixteen, so the blocks hold far fewer registers and the
+// reconstruction keeps its running sum in registers (no second LDS tile).
+// Same grids, band maps and results (to FP32 rounding) as k_cols_dec<PRE> and
+// k_cols_rec.  Thread (c, tau) of a block of CT * 64 threads owns rows
+// tau + 64 s of tile column c.
+template <int L, int CT>
+__global__ void __launch_bounds__(CT * 64) k_cols_dec3(const float2* __restrict__ Y1, long long y1_stride,
+                                                   float2* __restrict__ Tout, long long t_stride,
+                                                   const float2* __restrict__ Hmain, const float2* __restrict__ Hbeta,
+                                                   BandMap map, const float2* __restrict__ tw) {
+  static_assert(L == 512, "three-pass engine: 512 points");
+  using E = Col3<CT>;
+  constexpr int NH = L / 2;
+  __shared__ float2 buf[E::LDS];
+  int tile, slot, b;
+  col_block_order(tile, slot, b, map.share);
+  const int p = map_u8(map.order, slot);
+  const int j = map_u8(map.scale_of, p), f = map_u8(map.filt_of, p);
+  const int c0 = tile * CT;
+  const int c = threadIdx.x % CT, tau = threadIdx.x / CT;
+  const float2* src = Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0;
+  const BufView xv = col_view<L, CT>(src);
+  const BufView hv = col_view<L, CT>(Hmain + (size_t)f * L * NH + c0);
+  const unsigned lane = col_off<L>(tau, c);
+  float2 x[8], h[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    x[k] = xv.ld2(lane, col_off<L>(64 * k, 0));
+    h[k] = hv.ld2(lane, col_off<L>(64 * k, 0));
+  }
+  const Fft3Tw t = fft3_tw(tau, tw);
+  __builtin_amdgcn_sched_barrier(0);  // data, filter and twiddle loads in flight together
+  float2 q[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) q[k] = cmul(h[k], x[k]);
+  if (c0 == 0 && c == 0) {
+    // packed (DC, Nyquist) column: + beta(u) conj(X(-u))
+    const float2* hb = Hbeta + (size_t)f * L;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int u = tau + 64 * k;
+      const float2 pm = cconj(src[(size_t)((L - u) & (L - 1)) * NH]);
+      q[k] = cadd(q[k], cmul(hb[u], pm));
+    }
+  }
+  fft512_3<CT, +1>(q, buf, t);
+  const BufView tv = col_view<L, CT>(Tout + (size_t)b * t_stride + (size_t)p * L * NH + c0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) tv.st2(q[k], lane, col_off<L>(64 * k, 0));
+}
+
+template <int L, int CT>
+__global__ void __launch_bounds__(CT * 64) k_cols_rec3(const float2* __restrict__ U, long long u_stride,
+                                                   float2* __restrict__ Q, long long q_stride,
+                                                   const float2* __restrict__ Hmain, const float2* __restrict__ Hbeta,
+                                                   BandMap map, const float2* __restrict__ tw) {
+  static_assert(L == 512, "three-pass engine: 512 points");
+  using E = Col3<CT>;
+  constexpr int NH = L / 2;
+  __shared__ float2 buf[E::LDS];
+  int tile, slot, b;
+  col_block_order(tile, slot, b, map.gshare);
+  const int g = map_u16(map.gorder, slot);
+  const int c0 = tile * CT;
+  const int c = threadIdx.x % CT, tau = threadIdx.x / CT;
+  const int p0 = map_u16(map.group_band0, g), p1 = map_u16(map.group_band0, g + 1);
+  const float2* ub = U + (size_t)b * u_stride + c0;
+  const unsigned lane = col_off<L>(tau, c);
+  const Fft3Tw t = fft3_tw(tau, tw);
+  float2 acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = make_float2(0.f, 0.f);
+  // software pipeline: the band tile of the next direction is loaded while
+  // this direction is transformed (the filter tile is already in flight
+  // across the transform: it is used last)
+  float2 vn[8];
+  {
+    const BufView uv = col_view<L, CT>(ub + (size_t)p0 * L * NH);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) vn[k] = uv.ld2(lane, col_off<L>(64 * k, 0));
+  }
+  for (int p = p0; p < p1; ++p) {
+    const int f = map_u8(map.filt_of, p);
+    const BufView hv = col_view<L, CT>(Hmain + (size_t)f * L * NH + c0);
+    float2 v[8], h[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      v[k] = vn[k];
+      h[k] = hv.ld2(lane, col_off<L>(64 * k, 0));
+    }
+    if (p + 1 < p1) {
+      const BufView uv = col_view<L, CT>(ub + (size_t)(p + 1) * L * NH);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) vn[k] = uv.ld2(lane, col_off<L>(64 * k, 0));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (p > p0) __syncthreads();  // the previous direction's reads of buf are complete
+    fft512_3<CT, -1>(v, buf, t);
+    if (c0 == 0) {
+      // packed (DC, Nyquist) column: + conj(beta(u)) X(-u)* (the partner row
+      // of column 0 sits in another thread: exchanged through LDS)
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 8; ++k) buf[E::ix(tau + 64 * k, c)] = v[k];
+      __syncthreads();
+      if (c == 0) {
+        const float2* hb = Hbeta + (size_t)f * L;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int u = tau + 64 * k;
+          const float2 pm = cconj(buf[E::ix((L - u) & (L - 1), 0)]);
+          acc[k] = cadd(acc[k], cmulc(pm, hb[u]));
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = cadd(acc[k], cmulc(v[k], h[k]));
+  }
+  __syncthreads();
+  fft512_3<CT, +1>(acc, buf, t);
+  const BufView qv = col_view<L, CT>(Q + (size_t)b * q_stride + (size_t)g * L * NH + c0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) qv.st2(acc[k], lane, col_off<L>(64 * k, 0));
+}
+
+// The same two passes at L = 1024 on the 16 x 8 x 8 engine with 128 threads
+// per column (Col3m): the filter product is formed in the engine's spectral
+// layout (rows q + 128 s), between the forward transform (natural in,
+// spectral out) and the inverse (spectral in, natural out).
+template <int CT>
+__global__ void __launch_bounds__(CT * 128) k_cols_dec3k(const float2* __restrict__ Y1, long long y1_stride,
+                                                     float2* __restrict__ Tout, long long t_stride,
+                                                     const float2* __restrict__ Hmain, const float2* __restrict__ Hbeta,
+                                                     BandMap map, const float2* __restrict__ tw) {
+  constexpr int L = 1024, NH = L / 2;
+  using E = Col3m<CT>;
+  __shared__ float2 buf[E::LDS];
+  int tile, slot, b;
+  col_block_order(tile, slot, b, map.share);
+  const int p = map_u8(map.order, slot);
+  const int j = map_u8(map.scale_of, p), f = map_u8(map.filt_of, p);
+  const int c0 = tile * CT;
+  const int c = E::col(), q = E::q();
+  const float2* src = Y1 + (size_t)b * y1_stride + (size_t)j * L * NH + c0;
+  const BufView xv = col_view<L, CT>(src);
+  const BufView hv = col_view<L, CT>(Hmain + (size_t)f * L * NH + c0);
+  const unsigned slane = col_off<L>(q, c);
+  float2 x[8], h[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    x[k] = xv.ld2(slane, col_off<L>(128 * k, 0));
+    h[k] = hv.ld2(slane, col_off<L>(128 * k, 0));
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = cmul(h[k], x[k]);
+  if (c0 == 0 && c == 0) {
+    // packed (DC, Nyquist) column: + beta(u) conj(X(-u))
+    const float2* hb = Hbeta + (size_t)f * L;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int u = q + 128 * k;
+      const float2 pm = cconj(src[(size_t)((L - u) & (L - 1)) * NH]);
+      x[k] = cadd(x[k], cmul(hb[u], pm));
+    }
+  }
+  fft1024_inv<CT>(x, buf, tw);
+  const BufView tv = col_view<L, CT>(Tout + (size_t)b * t_stride + (size_t)p * L * NH + c0);
+  const unsigned nlane = col_off<L>(E::in_row(q), c);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) tv.st2(x[k], nlane, col_off<L>(128 * k, 0));
+}
+
+template <int CT>
+__global__ void __launch_bounds__(CT * 128) k_cols_rec3k(const float2* __restrict__ U, long long u_stride,
+                                                     float2* __restrict__ Q, long long q_stride,
+                                                     const float2* __restrict__ Hmain, const float2* __restrict__ Hbeta,
+                                                     BandMap map, const float2* __restrict__ tw) {
+  constexpr int L = 1024, NH = L / 2;
+  using E = Col3m<CT>;
+  __shared__ float2 buf[E::LDS];
+  int tile, slot, b;
+  col_block_order(tile, slot, b, map.gshare);
+  const int g = map_u16(map.gorder, slot);
+  const int c0 = tile * CT;
+  const int c = E::col(), q = E::q();
+  const int p0 = map_u16(map.group_band0, g), p1 = map_u16(map.group_band0, g + 1);
+  const float2* ub = U + (size_t)b * u_stride + c0;
+  const unsigned nlane = col_off<L>(E::in_row(q), c);
+  const unsigned slane = col_off<L>(q, c);
+  float2 acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = make_float2(0.f, 0.f);
+  for (int p = p0; p < p1; ++p) {
+    const int f = map_u8(map.filt_of, p);
+    const BufView uv = col_view<L, CT>(ub + (size_t)p * L * NH);
+    const BufView hv = col_view<L, CT>(Hmain + (size_t)f * L * NH + c0);
+    float2 v[8], h[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = uv.ld2(nlane, col_off<L>(128 * k, 0));
+    __builtin_amdgcn_sched_barrier(0);
+    if (p > p0) __syncthreads();  // the previous direction's reads of buf are complete
+    fft1024_fwd<CT>(v, buf, tw);
+    // the filter tile is read after the forward transform (not held across it)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) h[k] = hv.ld2(slane, col_off<L>(128 * k, 0));
+    if (c0 == 0) {
+      // packed (DC, Nyquist) column: + conj(beta(u)) X(-u)*, the partner row
+      // of column 0 exchanged through LDS
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 8; ++k) buf[E::ix(q + 128 * k, c)] = v[k];
+      __syncthreads();
+      if (c == 0) {
+        const float2* hb = Hbeta + (size_t)f * L;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int u = q + 128 * k;
+          const float2 pm = cconj(buf[E::ix((L - u) & (L - 1), 0)]);
+          acc[k] = cadd(acc[k], cmulc(pm, hb[u]));
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = cadd(acc[k], cmulc(v[k], h[k]));
+  }
+  __syncthreads();
+  fft1024_inv<CT>(acc, buf, tw);
+  const BufView qv = col_view<L, CT>(Q + (size_t)b * q_stride + (size_t)g * L * NH + c0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) qv.st2(acc[k], nlane, col_off<L>(128 * k, 0));
 }
 
 // ---------------------------------------------------------------------------
@@ -921,6 +1276,12 @@ struct Launch {
     hipLaunchKernelGGL((k_rows_c2r_r2c<L>), dim3(L / R::ROWS, np, nimg), dim3(R::T), 0, s, Tin, band, U, th, tw);
     return hipGetLastError();
   }
+  static hipError_t rows_c2r_mask_r2c(const PlaneSet& Tin, const PlaneSet& cref, const PlaneSet& U, int np, int nimg,
+                                      const ThreshSpec& th, float* gth, const float2* tw, hipStream_t s) {
+    hipLaunchKernelGGL((k_rows_c2r_mask_r2c<L>), dim3(L / R::ROWS, np, nimg), dim3(R::T), 0, s, Tin, cref, U, th, gth,
+                       tw);
+    return hipGetLastError();
+  }
   template <int CT>
   static hipError_t cols_dec_ct(const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
                                 const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s,
@@ -946,10 +1307,40 @@ struct Launch {
                        Q, qs, Hm, Hb, map, tw);
     return hipGetLastError();
   }
-  // ct: column tile width (8 or 16; 0 = default for L)
+  template <int CT>
+  static hipError_t cols_dec3_ct(const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
+                                 const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
+    if constexpr (L == 512 && CT >= 4)
+      hipLaunchKernelGGL((k_cols_dec3<L, CT>), dim3(L / 2 / CT, map.nbands, nimg), dim3(CT * 64), 0, s, Y1, ys, To,
+                         ts, Hm, Hb, map, tw);
+    if constexpr (L == 1024)
+      hipLaunchKernelGGL((k_cols_dec3k<CT>), dim3(L / 2 / CT, map.nbands, nimg), dim3(CT * 128), 0, s, Y1, ys, To, ts,
+                         Hm, Hb, map, tw);
+    return hipGetLastError();
+  }
+  template <int CT>
+  static hipError_t cols_rec3_ct(const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
+                                 const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s) {
+    if constexpr (L == 512 && CT >= 4)
+      hipLaunchKernelGGL((k_cols_rec3<L, CT>), dim3(L / 2 / CT, map.ngroups, nimg), dim3(CT * 64), 0, s, U, us, Q, qs,
+                         Hm, Hb, map, tw);
+    if constexpr (L == 1024)
+      hipLaunchKernelGGL((k_cols_rec3k<CT>), dim3(L / 2 / CT, map.ngroups, nimg), dim3(CT * 128), 0, s, U, us, Q, qs,
+                         Hm, Hb, map, tw);
+    return hipGetLastError();
+  }
+  // ct: column tile width (8 or 16; 0 = default for L); -4 / -8: the
+  // three-pass engine with 4 / 8 columns (L = 512 and 1024, pre only)
   static hipError_t cols_dec(const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
                              const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct,
                              bool pre) {
+    if (ct < 0) {
+      if ((L != 512 && L != 1024) || !pre) return hipErrorInvalidValue;
+      if (ct == -2 && L == 1024) return cols_dec3_ct<2>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s);
+      if (ct == -4) return cols_dec3_ct<4>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s);
+      if (ct == -8) return cols_dec3_ct<8>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s);
+      return hipErrorInvalidValue;
+    }
     if (ct == 0) ct = ColTile<L>::dflt;
     if (ct == 4 && ColTile<L>::narrow)
       return cols_dec_ct<(ColTile<L>::narrow ? 4 : 8)>(Y1, ys, To, ts, Hm, Hb, map, nimg, tw, s, pre);
@@ -959,7 +1350,7 @@ struct Launch {
     return hipErrorInvalidValue;
   }
   static hipError_t cols_fwd(float2* Y1, long long ys, int nplanes, int nimg, const float2* tw, hipStream_t s, int ct) {
-    if (ct == 0) ct = ColTile<L>::dflt;
+    if (ct <= 0) ct = ColTile<L>::dflt;
     if (ct == 4 && ColTile<L>::narrow) return cols_fwd_ct<(ColTile<L>::narrow ? 4 : 8)>(Y1, ys, nplanes, nimg, tw, s);
     if (ct == 8) return cols_fwd_ct<8>(Y1, ys, nplanes, nimg, tw, s);
     if (ct == 16 && ColTile<L>::wide) return cols_fwd_ct<(ColTile<L>::wide ? 16 : 8)>(Y1, ys, nplanes, nimg, tw, s);
@@ -967,6 +1358,13 @@ struct Launch {
   }
   static hipError_t cols_rec(const float2* U, long long us, float2* Q, long long qs, const float2* Hm,
                              const float2* Hb, const BandMap& map, int nimg, const float2* tw, hipStream_t s, int ct) {
+    if (ct < 0) {
+      if (L != 512 && L != 1024) return hipErrorInvalidValue;
+      if (ct == -2 && L == 1024) return cols_rec3_ct<2>(U, us, Q, qs, Hm, Hb, map, nimg, tw, s);
+      if (ct == -4) return cols_rec3_ct<4>(U, us, Q, qs, Hm, Hb, map, nimg, tw, s);
+      if (ct == -8) return cols_rec3_ct<8>(U, us, Q, qs, Hm, Hb, map, nimg, tw, s);
+      return hipErrorInvalidValue;
+    }
     if (ct == 0) ct = ColTile<L>::dflt;
     if (ct == 4 && ColTile<L>::narrow) return cols_rec_ct<(ColTile<L>::narrow ? 4 : 8)>(U, us, Q, qs, Hm, Hb, map, nimg, tw, s);
     if (ct == 8) return cols_rec_ct<8>(U, us, Q, qs, Hm, Hb, map, nimg, tw, s);
@@ -1000,6 +1398,12 @@ hipError_t launch_rows_c2r(int L, const PlaneSet& in, const PlaneSet& out, int n
 hipError_t launch_rows_c2r_r2c(int L, const PlaneSet& Tin, const PlaneSet& band, const PlaneSet& U, int np, int nimg,
                                const ThreshSpec& th, const float2* tw, hipStream_t s) {
   SHX_BY_L(L, LT::rows_c2r_r2c(Tin, band, U, np, nimg, th, tw, s))
+}
+
+hipError_t launch_rows_c2r_mask_r2c(int L, const PlaneSet& Tin, const PlaneSet& cref, const PlaneSet& U, int np,
+                                    int nimg, const ThreshSpec& th, float* gth, const float2* tw, hipStream_t s) {
+  if (th.mode != 2 || !th.dev || !gth) return hipErrorInvalidValue;
+  SHX_BY_L(L, LT::rows_c2r_mask_r2c(Tin, cref, U, np, nimg, th, gth, tw, s))
 }
 
 hipError_t launch_cols_dec(int L, const float2* Y1, long long ys, float2* To, long long ts, const float2* Hm,
@@ -1612,7 +2016,8 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_rec(const float2* 
       }
     }
   } else {
-    const float th = lt.mode == 0 ? -1.0f : (lt.mode == 1 ? lt.v : lt.dev[(size_t)b * lt.stride]);
+    const int rule = low_rule(lt);
+    const float th = low_value(lt, b);
     if (validA) {
       const float4* xr = reinterpret_cast<const float4*>(low + (size_t)b * ls + (size_t)(row0 + r) * L);
 #pragma unroll
@@ -1620,10 +2025,10 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_rec(const float2* 
         const int n = n1 + R1 * n2;
         if (n2 < R2 / 2) {
           const float4 f = xr[n];
-          v[n2] = make_float2(hard_keep(f.x, th), hard_keep(f.z, th));
+          v[n2] = shrink2(make_float2(f.x, f.z), th, rule);
         } else {
           const float4 f = xr[N - 1 - n];
-          v[n2] = make_float2(hard_keep(f.w, th), hard_keep(f.y, th));
+          v[n2] = shrink2(make_float2(f.w, f.y), th, rule);
         }
       }
     }
@@ -1841,17 +2246,18 @@ __global__ void __launch_bounds__(PRow<L>::T) k_dct_rows_rec_p(const float2* __r
       z[j] = j < H / 2 ? make_float2(a.x, c.x) : make_float2(c.y, a.y);
     }
   } else {
-    const float th = lt.mode == 0 ? -1.0f : (lt.mode == 1 ? lt.v : lt.dev[(size_t)b * lt.stride]);
+    const int rule = low_rule(lt);
+    const float th = low_value(lt, b);
     const float4* x4 = reinterpret_cast<const float4*>(low + (size_t)b * ls + (size_t)row * L);
 #pragma unroll
     for (int j = 0; j < H; ++j) {
       const int n = p + R * (2 * j + h);
       if (j < H / 2) {
         const float4 f = x4[n];
-        z[j] = make_float2(hard_keep(f.x, th), hard_keep(f.z, th));
+        z[j] = shrink2(make_float2(f.x, f.z), th, rule);
       } else {
         const float4 f = x4[N - 1 - n];
-        z[j] = make_float2(hard_keep(f.w, th), hard_keep(f.y, th));
+        z[j] = shrink2(make_float2(f.w, f.y), th, rule);
       }
     }
   }
@@ -2000,6 +2406,9 @@ hipError_t launch_dct_rec(int L, int step, const float2* Qg, long long qs, const
 }
 
 bool col_tile_ok(int L, int ct) {
+  // three-pass engines: 512 (8 x 8 x 8) with 4 or 8 columns, 1024
+  // (16 x 8 x 8 on 128 threads per column) with 2 or 4
+  if (ct < 0) return (L == 512 && (ct == -4 || ct == -8)) || (L == 1024 && (ct == -2 || ct == -4));
   switch (L) {
 #define SHX_CT_OK(LL) \
   case LL: return ct == 8 || (ct == 16 && ColTile<LL>::wide) || (ct == 4 && ColTile<LL>::narrow);

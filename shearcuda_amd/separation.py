@@ -78,38 +78,43 @@ def hard_bands_backward(x, gy, th, nimg, img_elems, rowlen, seglen, band_off=0):
 
 
 class ShearShrink(torch.autograd.Function):
-    """a (B, L, L) -> S_sh(soft(A_sh(a), th)) with per-band thresholds th."""
+    """a (B, L, L) -> S_sh(soft(A_sh(a), th)) with per-band thresholds th,
+    (nbands,) shared by the images or (B, nbands).
+
+    Forward: one shx_batch_shrink call, the soft threshold fused into the
+    reconstruction's reads; when a gradient is wanted it also keeps the
+    coefficients A_sh(a).  Backward: one shx_batch_shrink_backward call, the
+    threshold's mask and gradient fused between S^T and A^T."""
 
     @staticmethod
-    def forward(ctx, a, th, plan):
+    def forward(ctx, a, th, plan, keep=True):
         a = a.contiguous()
-        B, L = a.shape[0], a.shape[-1]
-        C = plan.alloc_coeffs()
-        plan.shear_trans(a, C)
-        n = C[0].numel()
-        Cs = soft_bands(C, th, B, n, n, L * L)
+        th = th.contiguous()
         out = torch.empty_like(a)
-        plan.inverse_shear(Cs, out)
-        ctx.save_for_backward(a, th)
+        C = plan.alloc_coeffs() if keep else None
+        plan.shrink(a, th, out, C)
         ctx.plan = plan
+        if keep:
+            ctx.save_for_backward(C, th)
         return out
 
     @staticmethod
     def backward(ctx, g):
-        a, th = ctx.saved_tensors
-        plan = ctx.plan
+        if not ctx.saved_tensors:
+            raise RuntimeError("ShearShrink ran without keeping its coefficients (keep=False)")
+        C, th = ctx.saved_tensors
         g = g.contiguous()
-        B, L = a.shape[0], a.shape[-1]
-        C = plan.alloc_coeffs()
-        plan.shear_trans(a, C)                      # recomputed bands
-        Gs = plan.alloc_coeffs()
-        plan.inverse_shear_adjoint(g, Gs)           # S^T g
-        n = C[0].numel()
+        ga = torch.empty_like(g)
         gth = torch.zeros_like(th)
-        gC = soft_bands_backward(C, Gs, th, gth, B, n, n, L * L)
-        ga = torch.empty_like(a)
-        plan.shear_trans_adjoint(gC, ga)            # A^T
-        return ga, gth, None
+        ctx.plan.shrink_backward(C, th, g, ga, gth)
+        return ga, gth, None, None
+
+
+def shear_shrink(a, th, plan):
+    """ShearShrink.apply, keeping the coefficients only when autograd will
+    call the backward."""
+    keep = torch.is_grad_enabled() and (a.requires_grad or th.requires_grad)
+    return ShearShrink.apply(a, th, plan, keep)
 
 
 class WaveShrink(torch.autograd.Function):
@@ -147,7 +152,7 @@ def unrolled_separation(x, th_sh, th_w, plan, h, levels=3, iters=10):
     c = torch.zeros_like(x)
     p = torch.zeros_like(x)
     for _ in range(iters):
-        c = ShearShrink.apply(x - p, th_sh, plan)
+        c = shear_shrink(x - p, th_sh, plan)
         p = WaveShrink.apply(x - c, th_w, h, levels)
     return c, p
 

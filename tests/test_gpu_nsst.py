@@ -256,8 +256,11 @@ def test_batch_matches_plain(api, L, nimg):
     shear.close()
 
 
-def test_mixed_size_batches_in_one_graph(api):
-    print("case mixed graph", flush=True)
+@pytest.mark.parametrize("parallel", [False, pytest.param(True, marks=pytest.mark.skip(
+    reason="capture_parallel is disabled: ending the forked capture crashed the process inside "
+           "hipStreamEndCapture twice (also with every fork/join event kept alive); cause not established"))])
+def test_mixed_size_batches_in_one_graph(api, parallel):
+    print("case mixed graph parallel=%s" % parallel, flush=True)
     plans = []
     for L, n in [(256, 2), (512, 1)]:
         shear = api.make_dictionary(L)
@@ -266,8 +269,11 @@ def test_mixed_size_batches_in_one_graph(api):
         b = api.Batch(shear, n)
         plans.append((shear, b, to_gpu(xs), b.alloc_coeffs(), api.batch_thresholds(shear, [20.0] * n, sc),
                       torch.empty((n, L, L), device="cuda"), sc))
-    g = api.capture(lambda: [b.denoise(x, c, t, o) for (_, b, x, c, t, o, _) in plans])
-    for _ in range(2):
+    if parallel:   # one graph branch per plan (no order between them)
+        g = api.capture_parallel([lambda p=p: p[1].denoise(p[2], p[3], p[4], p[5]) for p in plans])
+    else:
+        g = api.capture(lambda: [b.denoise(x, c, t, o) for (_, b, x, c, t, o, _) in plans])
+    for _ in range(3):
         for p in plans:
             p[5].zero_()
         g.launch()

@@ -3,7 +3,7 @@
 The autotuner picks, per plan, the pyramid tile shape (3 variants), the
 directions summed per reconstruction block (2, 4, 8) and the column tile width
 of the decomposition and reconstruction column passes (4, 8, 16 where the size
-allows).  The default schedule is pinned to the FP64 oracle in
+allows; -4 and -8 at 512: the three-pass column FFT with 4 or 8 columns).  The default schedule is pinned to the FP64 oracle in
 test_gpu_nsst.py; here every other schedule is compared with it: the
 coefficients must agree to FP32 rounding (they do not depend on the direction
 grouping) and so must the denoised image (the grouping only reorders the sum
@@ -31,7 +31,7 @@ def _rel(a, b):
 def _tiles(api, shear):
     from shearcuda_amd._lib import ShxError
     ok = []
-    for ct in (4, 8, 16):
+    for ct in (4, 8, 16, -4, -8, -2):   # negative: the three-pass engines
         try:
             api.set_schedule(shear, 0, 4, col_tile_dec=ct, col_tile_rec=ct)
             ok.append(ct)
@@ -62,27 +62,30 @@ def test_every_schedule_matches_default(gpu, L):
         ref = ref.cpu().numpy()
         cref = _bands(cref, L)
         worst_x = worst_c = 0.0
-        for pv in range(3):
-            for g in (2, 4, 8):
-                for ctd in tiles:
-                    for ctr in tiles:
-                        api.set_schedule(shear, pv, g, col_tile_dec=ctd, col_tile_rec=ctr)
-                        out, c = api.denoise_cuda(xg, shear, 20.0, sc)
-                        ex, ec = _rel(out.cpu().numpy(), ref), _rel(_bands(c, L), cref)
-                        worst_x, worst_c = max(worst_x, ex), max(worst_c, ec)
-                        assert ex <= TOL and ec <= TOL, (pv, g, ctd, ctr, ex, ec)
+        # the three-pass engine (negative tiles) is independent of the
+        # pyramid tile shape: it is crossed with the first pyramid variant
+        combos = [(pv, g, ctd, ctr) for pv in range(3) for g in (2, 4, 8) for ctd in tiles for ctr in tiles
+                  if pv == 0 or (ctd > 0 and ctr > 0)]
+        for pv, g, ctd, ctr in combos:
+            api.set_schedule(shear, pv, g, col_tile_dec=ctd, col_tile_rec=ctr)
+            out, c = api.denoise_cuda(xg, shear, 20.0, sc)
+            ex, ec = _rel(out.cpu().numpy(), ref), _rel(_bands(c, L), cref)
+            worst_x, worst_c = max(worst_x, ex), max(worst_c, ec)
+            assert ex <= TOL and ec <= TOL, (pv, g, ctd, ctr, ex, ec)
         print("L=%d tiles %s: worst rel diff image %.2e coefficients %.2e" % (L, tiles, worst_x, worst_c), flush=True)
     finally:
         shear.close()
 
 
-def test_narrow_tiles_in_batched_plan(gpu):
+@pytest.mark.parametrize("L,cts", [(256, (8, 4)), (512, (8, -8, -4)), (1024, (8, -2, -4))])
+def test_narrow_tiles_in_batched_plan(gpu, L, cts):
     """The batched plan (image index as a grid dimension) with the 4-column
-    tiles against the same images through the default schedule."""
+    tiles (and at 512 the three-pass engine) against the same images through
+    the 8-column schedule."""
     from shearcuda_amd import api
     from shearcuda_amd.synth import noisy_ellipses, white_noise
-    L, n = 256, 3
-    print("case batched narrow tiles L=%d n=%d" % (L, n), flush=True)
+    n = 3
+    print("case batched tiles L=%d n=%d %s" % (L, n, cts), flush=True)
     shear = api.make_dictionary(L)
     b = api.Batch(shear, n)
     try:
@@ -91,14 +94,15 @@ def test_narrow_tiles_in_batched_plan(gpu):
         xs = _gpu(np.stack([noisy_ellipses(L, sigma=float(s), seed=40 + i)[1] for i, s in enumerate(sig)]))
         th = api.batch_thresholds(shear, sig, sc)
         outs = []
-        for ct in (8, 4):
+        for ct in cts:
             api.set_schedule(shear, 0, 4, batch=b, col_tile_dec=ct, col_tile_rec=ct)
             out = torch.empty_like(xs)
             b.denoise(xs, b.alloc_coeffs(), th, out)
             outs.append(out.cpu().numpy())
-        e = _rel(outs[1], outs[0])
-        print("batched 4-column vs 8-column rel diff %.2e" % e, flush=True)
-        assert e <= TOL
+        for ct, o in zip(cts[1:], outs[1:]):
+            e = _rel(o, outs[0])
+            print("batched tile %d vs 8-column rel diff %.2e" % (ct, e), flush=True)
+            assert e <= TOL
     finally:
         b.close()
         shear.close()
