@@ -38,6 +38,17 @@ def _require_gpu(x, name="x"):
         raise ValueError("%s must be contiguous" % name)
 
 
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _workspace(nfloats, device):
+    """Scratch of the C calls that take an optional workspace, from torch's
+    caching allocator: passing one spares the library a hipMalloc, a stream
+    synchronisation and a hipFree per call."""
+    return torch.empty(int(nfloats), device=device, dtype=torch.float32)
+
+
 def _require_shape(x, shape, name="x"):
     """_require_gpu plus an exact shape; the C ABI trusts buffer sizes, so
     every tensor is checked before anything is launched."""
@@ -349,8 +360,9 @@ def atrousdec_cuda(x, pfilt, nlevels):
         raise ValueError("Input data should be a square")
     L = x.shape[0]
     y = [torch.empty_like(x) for _ in range(nlevels + 1)]
+    work = _workspace(2 * L * L, x.device)
     check(lib.shx_atrousdec(ctypes.c_void_p(x.data_ptr()), L, int(nlevels), ptr_array([t.data_ptr() for t in y]),
-                            None, _stream()))
+                            _vp(work), _stream()))
     return y
 
 
@@ -365,8 +377,9 @@ def atrousrec_cuda(y, pfilt):
     for i, t in enumerate(y):
         _require_shape(t, y[0].shape, "y{%d}" % (i + 1))
     out = torch.empty_like(y[0])
+    work = _workspace(2 * y[0].numel(), out.device)
     check(lib.shx_atrousrec(ptr_array([t.data_ptr() for t in y]), y[0].shape[0], len(y) - 1,
-                            ctypes.c_void_p(out.data_ptr()), None, _stream()))
+                            ctypes.c_void_p(out.data_ptr()), _vp(work), _stream()))
     return out
 
 
@@ -448,9 +461,10 @@ def re_weight_band_cuda(x, coeff, shear=None):
     if coeff.size <= 1:
         raise ValueError("There should be at least one band")
     out = torch.empty_like(x)
+    work = _workspace((coeff.size + 2) * x.numel(), x.device)
     check(lib.shx_re_weight_band(ctypes.c_void_p(x.data_ptr()), x.shape[0],
                                  coeff.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), coeff.size,
-                                 ctypes.c_void_p(out.data_ptr()), None, _stream()))
+                                 ctypes.c_void_p(out.data_ptr()), _vp(work), _stream()))
     return out
 
 
@@ -842,8 +856,9 @@ def mrdwt_cuda(x, h, L):
     h, hp = _h_arg(h)
     yl = torch.empty_like(x)
     yh = torch.empty(x.shape[:-1] + (3 * n * L,), device=x.device, dtype=torch.float32)
+    work = _workspace(2 * m * n * nimg, x.device)
     check(lib.shx_mrdwt(ctypes.c_void_p(x.data_ptr()), m, n, hp, h.size, int(L), ctypes.c_void_p(yl.data_ptr()),
-                        ctypes.c_void_p(yh.data_ptr()), None, nimg, _stream()))
+                        ctypes.c_void_p(yh.data_ptr()), _vp(work), nimg, _stream()))
     return yl, yh
 
 
@@ -854,8 +869,9 @@ def mirdwt_cuda(yl, yh, h, L):
     _require_shape(yh, yl.shape[:-1] + (3 * n * L,), "yh")
     h, hp = _h_arg(h)
     x = torch.empty_like(yl)
+    work = _workspace(2 * m * n * nimg, yl.device)
     check(lib.shx_mirdwt(ctypes.c_void_p(yl.data_ptr()), ctypes.c_void_p(yh.data_ptr()), m, n, hp, h.size, int(L),
-                         ctypes.c_void_p(x.data_ptr()), None, nimg, _stream()))
+                         ctypes.c_void_p(x.data_ptr()), _vp(work), nimg, _stream()))
     return x
 
 
@@ -866,8 +882,9 @@ def mrdwt_adjoint_cuda(yl_bar, yh_bar, h, L):
     _require_shape(yh_bar, yl_bar.shape[:-1] + (3 * n * L,), "yh_bar")
     h, hp = _h_arg(h)
     xb = torch.empty_like(yl_bar)
+    work = _workspace(2 * m * n * nimg, yl_bar.device)
     check(lib.shx_mrdwt_adjoint(ctypes.c_void_p(yl_bar.data_ptr()), ctypes.c_void_p(yh_bar.data_ptr()), m, n, hp,
-                                h.size, int(L), ctypes.c_void_p(xb.data_ptr()), None, nimg, _stream()))
+                                h.size, int(L), ctypes.c_void_p(xb.data_ptr()), _vp(work), nimg, _stream()))
     return xb
 
 
@@ -878,7 +895,8 @@ def mirdwt_adjoint_cuda(x_bar, h, L):
     h, hp = _h_arg(h)
     yl = torch.empty_like(x_bar)
     yh = torch.empty(x_bar.shape[:-1] + (3 * n * L,), device=x_bar.device, dtype=torch.float32)
+    work = _workspace(2 * m * n * nimg, x_bar.device)
     check(lib.shx_mirdwt_adjoint(ctypes.c_void_p(x_bar.data_ptr()), m, n, hp, h.size, int(L),
-                                 ctypes.c_void_p(yl.data_ptr()), ctypes.c_void_p(yh.data_ptr()), None, nimg,
+                                 ctypes.c_void_p(yl.data_ptr()), ctypes.c_void_p(yh.data_ptr()), _vp(work), nimg,
                                  _stream()))
     return yl, yh

@@ -46,6 +46,41 @@ using shx::fail;
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
 // Device allocation owned by a scope: freed on every return path.
+// Accumulator of the reductions whose result the host reads back before the
+// call returns (norms, the maximum of shx_thresh): a file-scope device array,
+// so those calls allocate nothing.  A call holds ReduceSlot's lock from the
+// clear to the read-back, so host threads take turns.  Host code never passes
+// the array's name to a kernel: each device's copy is found with
+// hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) double reduce_slot[1];
+
+struct ReduceSlot {
+  std::unique_lock<std::mutex> lock;
+  void* p = nullptr;
+};
+
+int reduce_slot_acquire(ReduceSlot& rs) {
+  static std::mutex addr_mu, use_mu;
+  static void* addr[64] = {};
+  int dev = -1;
+  SHX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(SHX_ERR_UNSUPPORTED, "device %d out of range", dev);
+  {
+    std::lock_guard<std::mutex> g(addr_mu);
+    if (!addr[dev]) {
+      void* q = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(reduce_slot));
+      if (e != hipSuccess || !q)
+        return fail(SHX_ERR_HIP, "hipGetSymbolAddress(reduce_slot) on device %d failed: %s", dev,
+                    hipGetErrorString(e));
+      addr[dev] = q;
+    }
+    rs.p = addr[dev];
+  }
+  rs.lock = std::unique_lock<std::mutex>(use_mu);
+  return SHX_OK;
+}
+
 struct DevMem {
   void* p = nullptr;
   DevMem() = default;
@@ -557,23 +592,26 @@ void stages_spec_dec(shx_dict* d, const Work& w, int nimg, hipStream_t s, std::v
 // DCT-domain front end: the lowpass (c.low) and the scale spectra (w.spec)
 // of the whole pyramid in three passes; the image's column DCT goes through
 // the first plane of w.planes.
+// mask: the decomposition masks (NULL), or the reconstruction masks for the
+// transpose of the DCT-domain reconstruction (S^T).
 void stages_dct_front(shx_dict* d, const Work& w, int nimg, const float* x, long long xs, const Coeffs& c,
-                      hipStream_t s, std::vector<Stage>& st) {
+                      hipStream_t s, std::vector<Stage>& st, const float* mask = nullptr) {
   const int L = d->L, S = d->S;
   const double P = 4.0 * L * L * nimg;
   const long long ss = w.s_spec / 2;
+  if (!mask) mask = d->dmask;
   st.push_back({"dct_cols", 2 * P, [=] {
                   return shx::launch_dct_dec(L, 0, x, xs, w.planes, w.s_planes, w.spec, ss, S, c.low, c.low_stride,
-                                             d->dmask, nimg, d->tw, d->dtw, s);
+                                             mask, nimg, d->tw, d->dtw, s);
                 }});
   // reads the column DCT and the S + 1 masks, writes S spectra and the lowpass
   st.push_back({"dct_rows_dec", (2.0 * S + 3) * P, [=] {
                   return shx::launch_dct_dec(L, 1, x, xs, w.planes, w.s_planes, w.spec, ss, S, c.low, c.low_stride,
-                                             d->dmask, nimg, d->tw, d->dtw, s);
+                                             mask, nimg, d->tw, d->dtw, s);
                 }});
   st.push_back({"dct_cols_dec", 2.0 * (S + 1) * P, [=] {
                   return shx::launch_dct_dec(L, 2, x, xs, w.planes, w.s_planes, w.spec, ss, S, c.low, c.low_stride,
-                                             d->dmask, nimg, d->tw, d->dtw, s);
+                                             mask, nimg, d->tw, d->dtw, s);
                 }});
 }
 
@@ -659,9 +697,12 @@ void stages_pyr_rec(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const
 // DCT-domain back end: from the direction-group partials (w.q) and the
 // lowpass to the image in three passes (the scale planes' row DCTs go through
 // w.planes' first S + 1 planes).
+// mask: the reconstruction masks (NULL), or the decomposition masks for the
+// transpose of the DCT-domain decomposition (A^T).
 void stages_dct_back(shx_dict* d, const Work& w, int nimg, const Coeffs& c, const Thr& t, float* x, long long xs,
-                     hipStream_t s, std::vector<Stage>& st) {
+                     hipStream_t s, std::vector<Stage>& st, const float* mask = nullptr) {
   const int L = d->L, S = d->S, ng = w.map.ngroups;
+  if (!mask) mask = d->rmask;
   const double P = 4.0 * L * L * nimg;
   const shx::LowThresh lt = low_thresh(d, t);
   std::vector<int> gf(w.map.group_first, w.map.group_first + S + 1);
@@ -669,16 +710,16 @@ void stages_dct_back(shx_dict* d, const Work& w, int nimg, const Coeffs& c, cons
   // reads the partials and the lowpass, writes S + 1 row-transformed planes
   st.push_back({"dct_rows_rec", (ng + 1.0 + (S + 1)) * P, [=] {
                   return shx::launch_dct_rec(L, 0, w.q, qs, gf.data(), S, c.low, c.low_stride, lt, w.planes,
-                                             w.s_planes, d->rmask, x, xs, nimg, d->tw, d->dtw, s);
+                                             w.s_planes, mask, x, xs, nimg, d->tw, d->dtw, s);
                 }});
   // reads and writes every plane, reads its mask
   st.push_back({"dct_cols_rec", 3.0 * (S + 1) * P, [=] {
                   return shx::launch_dct_rec(L, 1, w.q, qs, gf.data(), S, c.low, c.low_stride, lt, w.planes,
-                                             w.s_planes, d->rmask, x, xs, nimg, d->tw, d->dtw, s);
+                                             w.s_planes, mask, x, xs, nimg, d->tw, d->dtw, s);
                 }});
   st.push_back({"dct_rows_out", (S + 2.0) * P, [=] {
                   return shx::launch_dct_rec(L, 2, w.q, qs, gf.data(), S, c.low, c.low_stride, lt, w.planes,
-                                             w.s_planes, d->rmask, x, xs, nimg, d->tw, d->dtw, s);
+                                             w.s_planes, mask, x, xs, nimg, d->tw, d->dtw, s);
                 }});
 }
 
@@ -919,6 +960,38 @@ void stages_pyr_rec_adjoint(shx_dict* d, const Work& w, int nimg, const float* g
   }
 }
 
+// The pyramid part of A^T from the band spectra U (w.band): the directional
+// reconstruction's column pass, then either the transposed pyramid levels or,
+// when the plan's decomposition runs in the DCT domain, its transpose there.
+// A DCT-domain level is x -> D2^-1(M . D2 x) with D2 the raw 2D DCT-II.  With
+// C the 1D DCT-II matrix, C C^T = diag(L, L/2, ..., L/2) = Lam, so C^T =
+// C^-1 Lam and the diagonal Lam commutes with the diagonal mask: the
+// transpose is D2^-1(M . D2 y), the same operator.  So A^T is the DCT-domain
+// reconstruction run with the decomposition masks, and S^T the DCT-domain
+// decomposition run with the reconstruction masks.
+void stages_dec_adjoint_tail(shx_dict* d, const Work& w, int nimg, const Coeffs& c, float* gx, long long gxs,
+                             hipStream_t s, std::vector<Stage>& st) {
+  if (w.dct_front && d->dmask) {
+    stages_spec_rec_tail(d, w, nimg, s, st, false);
+    stages_dct_back(d, w, nimg, c, Thr{}, gx, gxs, s, st, d->dmask);
+  } else {
+    stages_spec_rec_tail(d, w, nimg, s, st);
+    stages_pyr_dec_adjoint(d, w, nimg, c, gx, gxs, s, st);
+  }
+}
+
+// The pyramid part of S^T, up to the band spectra T (w.band).
+void stages_rec_adjoint_front(shx_dict* d, const Work& w, int nimg, const float* gx, long long gxs, const Coeffs& c,
+                              hipStream_t s, std::vector<Stage>& st) {
+  if (w.dct_back && d->rmask) {
+    stages_dct_front(d, w, nimg, gx, gxs, c, s, st, d->rmask);
+    stage_cols_dec(d, w, nimg, s, st);
+  } else {
+    stages_pyr_rec_adjoint(d, w, nimg, gx, gxs, c, s, st);
+    stages_spec_dec(d, w, nimg, s, st);
+  }
+}
+
 // A^T: coefficient gradients -> image gradient (directional stage: its
 // adjoint is the directional reconstruction, the filters form a tight frame).
 void stages_dec_adjoint(shx_dict* d, const Work& w, int nimg, const Coeffs& c, float* gx, long long gxs,
@@ -930,15 +1003,13 @@ void stages_dec_adjoint(shx_dict* d, const Work& w, int nimg, const Coeffs& c, f
   shx::ThreshSpec none{};
   st.push_back({"rows_r2c_bands", 2.0 * nb * P,
                 [=] { return shx::launch_rows_r2c(L, in, bandset, nb, nimg, none, d->tw, s); }});
-  stages_spec_rec_tail(d, w, nimg, s, st);
-  stages_pyr_dec_adjoint(d, w, nimg, c, gx, gxs, s, st);
+  stages_dec_adjoint_tail(d, w, nimg, c, gx, gxs, s, st);
 }
 
 // S^T: image gradient -> coefficient gradients.
 void stages_rec_adjoint(shx_dict* d, const Work& w, int nimg, const float* gx, long long gxs, const Coeffs& c,
                         hipStream_t s, std::vector<Stage>& st) {
-  stages_pyr_rec_adjoint(d, w, nimg, gx, gxs, c, s, st);
-  stages_spec_dec(d, w, nimg, s, st);
+  stages_rec_adjoint_front(d, w, nimg, gx, gxs, c, s, st);
   const int L = d->L, nb = d->nbands;
   const double P = 4.0 * L * L * nimg;
   PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
@@ -961,8 +1032,7 @@ void stages_shrink_backward(shx_dict* d, const Work& w, int nimg, const Coeffs& 
   Coeffs g{};
   g.low = w.planes + (size_t)(S + 2) * pl;
   g.low_stride = w.s_planes;
-  stages_pyr_rec_adjoint(d, w, nimg, gout, gos, g, s, st);
-  stages_spec_dec(d, w, nimg, s, st);
+  stages_rec_adjoint_front(d, w, nimg, gout, gos, g, s, st);
   PlaneSet bandset = shx::single_cell(reinterpret_cast<float*>(w.band), nb, w.s_band);
   PlaneSet cb = c.bands;
   shx::ThreshSpec th = band_thresh(d, t);
@@ -982,8 +1052,7 @@ void stages_shrink_backward(shx_dict* d, const Work& w, int nimg, const Coeffs& 
                                      thd, ts, gth);
                   return hipGetLastError();
                 }});
-  stages_spec_rec_tail(d, w, nimg, s, st);
-  stages_pyr_dec_adjoint(d, w, nimg, g, gx, gxs, s, st);
+  stages_dec_adjoint_tail(d, w, nimg, g, gx, gxs, s, st);
 }
 
 int run_stages(const std::vector<Stage>& st) {
@@ -1871,11 +1940,11 @@ int shx_thresh(float* const* cells, int ncells, const int* cell_planes, const in
       }
     }
   }
-  DevMem dmax_mem;
+  ReduceSlot slot;
   unsigned int* dmax = nullptr;
   if (opt == 1) {
-    SHX_HIP(hipMalloc(&dmax_mem.p, sizeof(unsigned int)));
-    dmax = dmax_mem.as<unsigned int>();
+    if (int r = reduce_slot_acquire(slot)) return r;
+    dmax = static_cast<unsigned int*>(slot.p);
     SHX_HIP(hipMemsetAsync(dmax, 0, sizeof(unsigned int), s));
   }
   for (size_t b = 0; b < ptr.size(); b += 64) {
@@ -2168,9 +2237,9 @@ int shx_mirdwt_adjoint(const float* x_bar, int m, int n, const double* h, int lh
 int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream) {
   if (!x || !out) return fail(SHX_ERR_ARG, "null input or output");
   hipStream_t s = (hipStream_t)stream;
-  DevMem acc_mem;
-  SHX_HIP(hipMalloc(&acc_mem.p, sizeof(double)));
-  double* acc = acc_mem.as<double>();
+  ReduceSlot slot;
+  if (int r = reduce_slot_acquire(slot)) return r;
+  double* acc = static_cast<double*>(slot.p);
   SHX_HIP(hipMemsetAsync(acc, 0, sizeof(double), s));
   const int blocks = (int)std::max<long long>(1, std::min<long long>(1024, (n + 4095) / 4096));
   if (n > 0) hipLaunchKernelGGL(k_norm_pow, dim3(blocks), dim3(256), 0, s, x, (long long)n, (float)p, acc);
@@ -2183,9 +2252,9 @@ int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream)
 int shx_norm_pow_diff(const float* a, const float* b, int64_t n, double p, double* out, void* stream) {
   if (!a || !b || !out) return fail(SHX_ERR_ARG, "null input or output");
   hipStream_t s = (hipStream_t)stream;
-  DevMem acc_mem;
-  SHX_HIP(hipMalloc(&acc_mem.p, sizeof(double)));
-  double* acc = acc_mem.as<double>();
+  ReduceSlot slot;
+  if (int r = reduce_slot_acquire(slot)) return r;
+  double* acc = static_cast<double*>(slot.p);
   SHX_HIP(hipMemsetAsync(acc, 0, sizeof(double), s));
   const int blocks = (int)std::max<long long>(1, std::min<long long>(1024, (n + 4095) / 4096));
   if (n > 0) hipLaunchKernelGGL(k_norm_pow_diff, dim3(blocks), dim3(256), 0, s, a, b, (long long)n, (float)p, acc);

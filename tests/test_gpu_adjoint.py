@@ -128,12 +128,15 @@ def test_synthesis_adjoint_dense(api, dense32):
     shear.close()
 
 
-@pytest.mark.parametrize("L", [64, 512])
-def test_adjoint_identity_default_config(api, L):
+@pytest.mark.parametrize("L,dct", [(64, False), (512, False), (64, True), (512, True)])
+def test_adjoint_identity_default_config(api, L, dct):
     """<A x, c> = <x, A^T c> and <S c, y> = <c, S^T y> with the 4-scale
-    [8 8 16 16] dictionary (size-independent check)."""
-    print("case adjoint identity L=%d" % L, flush=True)
+    [8 8 16 16] dictionary (size-independent check), with the pyramid levels
+    and with the DCT-domain forms (whose transposes run the other side's
+    masks)."""
+    print("case adjoint identity L=%d dct=%s" % (L, dct), flush=True)
     shear = api.make_dictionary(L)
+    api.set_plan_options(shear, dct_front=dct, dct_back=dct)
     rng = np.random.default_rng(L)
     x = rng.standard_normal((L, L))
     c = [rng.standard_normal((L, L))] + [rng.standard_normal((d, L, L)) for d in shear.ndir]
@@ -167,4 +170,31 @@ def test_batched_adjoints_equal_single(api):
         ys = api.inverse_shear_adjoint_cuda(xs, shear)
         assert torch.equal(torch.cat([ys[0][None]] + list(ys[1:])), yb[i])
     b.close()
+    shear.close()
+
+
+@pytest.mark.parametrize("L,nimg", [(64, 1), (256, 3), (512, 2)])
+def test_dct_domain_adjoints_equal_level_adjoints(api, L, nimg):
+    """A^T and S^T through the DCT domain against the transposed pyramid
+    levels (pinned by the dense 32x32 tests and the identities above)."""
+    print("case dct adjoints L=%d nimg=%d" % (L, nimg), flush=True)
+    shear = api.make_dictionary(L)
+    rng = np.random.default_rng(L + 7)
+    cb = g(rng.standard_normal((nimg, shear.nbands, L, L)))
+    xb = g(rng.standard_normal((nimg, L, L)))
+    out = {}
+    for dct in (False, True):
+        b = api.Batch(shear, nimg)
+        api.set_plan_options(shear, b, dct_front=dct, dct_back=dct)
+        xa = torch.empty((nimg, L, L), device="cuda")
+        b.shear_trans_adjoint(cb, xa)
+        ya = b.alloc_coeffs()
+        b.inverse_shear_adjoint(xb, ya)
+        torch.cuda.synchronize()
+        out[dct] = (xa, ya)
+        b.close()
+    ex = float((out[True][0] - out[False][0]).abs().max() / out[False][0].abs().max())
+    ey = float((out[True][1] - out[False][1]).abs().max() / out[False][1].abs().max())
+    print("A^T rel %.2e S^T rel %.2e" % (ex, ey), flush=True)
+    assert ex <= 1e-5 and ey <= 1e-5
     shear.close()
