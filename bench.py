@@ -17,10 +17,15 @@ restatement of shearlet_toolbox_1) on the same workload for a bounded time,
 on all host cores (one process per core) with the one-core figure beside it.
 """
 import argparse
+import csv
 import ctypes
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -202,13 +207,24 @@ def profile_plain(shear, xg, sc, sigma, reps, coeffs=False):
     return stages
 
 
-def bench_plain(L, sigma, steps, warmup):
+def apply_schedule(shear, sched):
+    from shearcuda_amd import api
+    api.set_plan_options(shear, dct_front=sched["dct_front"], dct_back=sched["dct_back"])
+    api.set_schedule(shear, sched["pyramid_variant"], sched["group"], col_tile_dec=sched["col_tile_dec"],
+                     col_tile_rec=sched["col_tile_rec"])
+
+
+def bench_plain(L, sigma, steps, warmup, sched=None):
     """ms per step of the plain denoise graph returning the image only (the
     timed step), ms per step of the same step also writing the unthresholded
-    coefficients, the per-kernel profile of the timed step and the schedule."""
+    coefficients, the per-kernel profile of the timed step and the schedule
+    (autotuned unless given)."""
     from shearcuda_amd import api
     shear, xg, sc = plain_setup(L, sigma)
-    sched = api.autotune(shear)
+    if sched is None:
+        sched = api.autotune(shear)
+    else:
+        apply_schedule(shear, sched)
     g, _, out = api.denoise_graph(xg, shear, sigma, sc, coeffs=False)
     el = _timed(g.launch, steps, warmup)
     gc, Ct, outc = api.denoise_graph(xg, shear, sigma, sc)
@@ -462,6 +478,147 @@ def bench_3d(steps=5, warmup=1, nvol=4, n=128, sigma=25.0, group=8, cpu=True):
     return info
 
 
+# ---- HBM traffic ledger of the plain step (rocprofv3 counters, live) -------------
+LEDGER_LAUNCHES = 3
+LEDGER_COPY_MIB = 256
+
+
+def ledger_child(spec):
+    """Child process of traffic_ledger.  Without a schedule: autotune the plain
+    step and write the schedule.  With one: run the step's graph
+    LEDGER_LAUNCHES times, then one device copy of LEDGER_COPY_MIB MiB (the
+    counters' calibration), and write the step's stage names in launch
+    order.  Run under rocprofv3 --pmc for the counted passes."""
+    import torch
+
+    from shearcuda_amd import api
+    torch.cuda.set_device(0)
+    L, sigma = spec["L"], spec["sigma"]
+    shear, xg, sc = plain_setup(L, sigma)
+    out = {}
+    if spec.get("schedule") is None:
+        out["schedule"] = api.autotune(shear)
+    else:
+        apply_schedule(shear, spec["schedule"])
+        out["stages"] = [st["name"] for st in profile_plain(shear, xg, sc, sigma, 1)]
+        g, _, _ = api.denoise_graph(xg, shear, sigma, sc, coeffs=False)
+        # the copy's buffers are filled before the launches, so that the copy
+        # is the only dispatch after the last launch
+        src = torch.ones(LEDGER_COPY_MIB * (1 << 20) // 4, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(LEDGER_LAUNCHES):
+            g.launch()
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        g.close()
+    shear.close()
+    with open(spec["out"], "w") as f:
+        json.dump(out, f)
+
+
+def _run_child(spec, counter=None, timeout=240):
+    """bench.py --ledger-child in a fresh process (optionally under
+    rocprofv3 --pmc counter); returns (result dict, counter CSV rows)."""
+    d = tempfile.mkdtemp(prefix="shx_ledger_", dir="/tmp")
+    spec = dict(spec, out=os.path.join(d, "child.json"))
+    cmd = [sys.executable, os.path.abspath(__file__), "--ledger-child", json.dumps(spec)]
+    if counter:
+        cmd = ["timeout", "-s", "KILL", str(timeout), shutil.which("rocprofv3"), "--pmc", counter,
+               "--output-format", "csv", "-d", d, "-o", "pmc", "--"] + cmd
+    else:
+        cmd = ["timeout", "-s", "KILL", str(timeout)] + cmd
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    try:
+        if r.returncode != 0:
+            tail = r.stdout.decode(errors="replace")[-600:]
+            raise RuntimeError("ledger child (%s) exited %d: %s" % (counter or "tuning", r.returncode, tail))
+        with open(spec["out"]) as f:
+            res = json.load(f)
+        rows = []
+        if counter:
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            if not files:
+                raise RuntimeError("rocprofv3 wrote no counter_collection.csv")
+            with open(files[0]) as f:
+                rows = [row for row in csv.DictReader(f) if row.get("Counter_Name") == counter]
+        return res, rows
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def _dispatches(rows):
+    """(dispatch id, kernel name, counter value in bytes) in dispatch order;
+    FETCH_SIZE and WRITE_SIZE count KiB."""
+    out = {}
+    for row in rows:
+        k = int(row["Dispatch_Id"])
+        name = row["Kernel_Name"]
+        v = float(row["Counter_Value"]) * 1024.0
+        out[k] = (name, out.get(k, (name, 0.0))[1] + v)
+    return [(k, n, v) for k, (n, v) in sorted(out.items())]
+
+
+def _split_launch(disp, nk):
+    """The last graph launch's nk dispatches and the calibration copy's
+    dispatches after them."""
+    own = [i for i, (_, n, _) in enumerate(disp) if "shx::" in n or n.startswith("k_")]
+    if not own:
+        raise RuntimeError("no kernels of the library in the counter run")
+    last = own[-1]
+    step = disp[last - nk + 1:last + 1]
+    copy = [d for d in disp[last + 1:] if "copy" in d[1].lower()]
+    prev = disp[last - 2 * nk + 1:last - nk + 1]
+    if len(step) != nk or not copy or [n for _, n, _ in prev] != [n for _, n, _ in step]:
+        raise RuntimeError("could not separate one graph launch and the calibration copy")
+    return step, copy
+
+
+def traffic_ledger(L, sigma):
+    """Per-kernel HBM bytes of one plain step from two rocprofv3 --pmc passes
+    (FETCH_SIZE, WRITE_SIZE) of a child process running the step's graph,
+    each scaled by what the same counter reads for a device copy of
+    LEDGER_COPY_MIB MiB.  A tuning child first fixes the schedule, which the
+    counted passes and the timed run then share.  Runs before this process
+    touches the GPU (the children are separate processes).  Returns the
+    ledger, or {"error": ...} when the counters are unavailable."""
+    if not shutil.which("rocprofv3"):
+        return {"error": "rocprofv3 not found"}
+    try:
+        base = {"L": L, "sigma": sigma}
+        tuned, _ = _run_child(dict(base, schedule=None))
+        sched = tuned["schedule"]
+        spec = dict(base, schedule=sched)
+        res_f, rows_f = _run_child(spec, "FETCH_SIZE")
+        res_w, rows_w = _run_child(spec, "WRITE_SIZE")
+        names = res_f["stages"]
+        nk = len(names)
+        step_f, copy_f = _split_launch(_dispatches(rows_f), nk)
+        step_w, copy_w = _split_launch(_dispatches(rows_w), nk)
+        cbytes = float(LEDGER_COPY_MIB << 20)
+        fscale = cbytes / sum(v for _, _, v in copy_f)
+        wscale = cbytes / sum(v for _, _, v in copy_w)
+        kernels = []
+        for stage, (_, kn, fv), (_, kw, wv) in zip(names, step_f, step_w):
+            if kn != kw:
+                raise RuntimeError("the two counter passes launched different kernels")
+            kernels.append({"stage": stage, "kernel": kn.split("(")[0].replace("void ", ""),
+                            "fetch_bytes": round(fv * fscale), "write_bytes": round(wv * wscale),
+                            "hbm_bytes": round(fv * fscale + wv * wscale)})
+        total = sum(k["hbm_bytes"] for k in kernels)
+        model = 400.0 * L * L
+        return {"schedule": sched, "kernels": kernels, "step_hbm_bytes": total, "model_bytes": model,
+                "ratio_to_model": round(total / model, 3),
+                "calibration": {"copy_mib": LEDGER_COPY_MIB, "fetch_scale": round(fscale, 4),
+                                "write_scale": round(wscale, 4)},
+                "method": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over a child running the step's graph %d "
+                          "times; the last launch's dispatches" % LEDGER_LAUNCHES}
+    except Exception as e:  # counters are a report, not the measurement
+        return {"error": str(e)[-800:]}
+
+
 def measured_traffic(stage, L, sched, algo_bytes):
     """HBM bytes per launch of a stage's kernel from the committed rocprofv3
     counter runs (profiles/hbm_traffic.json, FETCH_SIZE and WRITE_SIZE scaled
@@ -496,7 +653,12 @@ def main():
     ap.add_argument("--vol-steps", type=int, default=5)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (HBM counter calibration)")
+    ap.add_argument("--no-ledger", action="store_true", help="skip the rocprofv3 traffic ledger of the plain step")
+    ap.add_argument("--ledger-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.ledger_child:
+        ledger_child(json.loads(args.ledger_child))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the NSST path has no multi-GPU form here); "
               "run with --gpus 1", file=sys.stderr)
@@ -504,6 +666,9 @@ def main():
     cpu = None
     if args.cpu_budget > 0 and not args.only:
         cpu = cpu_baseline(args.size, 20.0, args.cpu_budget)   # before anything touches the GPU
+    ledger = None
+    if not args.no_ledger and not args.only:
+        ledger = traffic_ledger(args.size, 20.0)   # child processes, also before this one touches the GPU
     import torch
     torch.cuda.set_device(0)
     if args.copy_calibration_mib > 0:
@@ -521,13 +686,20 @@ def main():
         print(json.dumps(r), flush=True)
         return
     L, sigma = args.size, 20.0
-    ms, ms_coeffs, stages, sched = bench_plain(L, sigma, args.steps, args.warmup)
+    ms, ms_coeffs, stages, sched = bench_plain(L, sigma, args.steps, args.warmup,
+                                               ledger.get("schedule") if ledger else None)
     value = L * L / (ms * 1e-3) / 1e6
     dom = max(stages, key=lambda s: s["ms"])
+    traffic, traffic_src = None, None
+    if ledger and "kernels" in ledger and len(ledger["kernels"]) == len(stages):
+        traffic = ledger["kernels"][stages.index(dom)]["hbm_bytes"]
+        traffic_src = "ledger"
+    else:
+        traffic = measured_traffic(dom["name"], L, sched, dom["bytes"])
+        traffic_src = "profiles/hbm_traffic.json" if traffic is not None else None
     roof = {"kernel": dom["name"], "bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(dom["name"], L, sched, dom["bytes"]),
-            "bytes_per_launch": dom["bytes"], "ms_per_launch": round(dom["ms"], 5)}
+            "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": traffic_src, "bytes_per_launch": dom["bytes"], "ms_per_launch": round(dom["ms"], 5)}
     path_bytes = 400.0 * L * L
     path_gbs = path_bytes / (ms * 1e-3) / 1e9
     res = {
@@ -549,6 +721,8 @@ def main():
                                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": valu_frac(L, L * L, ms * 1e-3)},
         "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()} for s in stages],
     }
+    if ledger is not None:
+        res["traffic_ledger"] = ledger
     if L == 512 and not args.no_plain_1024:
         ms1k, ms1kc, _, _ = bench_plain(1024, sigma, max(20, args.steps // 4), 5)
         res["plain_1024"] = {"ms_per_image": round(ms1k, 5), "Mpixel_s": round(1024 * 1024 / (ms1k * 1e-3) / 1e6, 2),
