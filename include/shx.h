@@ -236,9 +236,10 @@ int shx_denoise_graph(shx_dict* dict, const float* x, float* const* coeff, doubl
  *   option 1 (wavelets): only cells[1], th = lambda.
  *   option 2 (shearlets): th(s,w) = sc[s] * lambda * E[s + w*E_rows]
  *       (E column-major E_rows x E_cols doubles, E_rows = ncells).
- *   opt 1: *maxi = max over thresholded bands of max|c|/E (else 0); this
- *       allocates a device word and synchronises the stream, so an opt 1
- *       call cannot be recorded into a graph (opt 0 can). */
+ *   opt 1: *maxi = max over thresholded bands of max|c|/E (else 0); the
+ *       maximum accumulates in a file-scope device word (no allocation; host
+ *       threads take turns on it) and the call synchronises the stream, so
+ *       an opt 1 call cannot be recorded into a graph (opt 0 can). */
 int shx_thresh(float* const* cells, int ncells, const int* cell_planes, const int64_t* plane_elems, double lambda,
                int option, const double* E, int E_rows, int E_cols, const double* sc, int n_sc, int opt,
                double* maxi, void* stream);
@@ -310,7 +311,8 @@ int shx_hard_bands_backward(const float* x, const float* gy, float* gx, int nimg
 
 /* Sum |x|^p over n elements, result (p-th root not taken) in *out (host).
  * norm_cuda's reduction (shearcuda/src_cuda/norm_cuda.cpp:172-215).
- * Allocates a device word and synchronises the stream (not capturable). */
+ * Accumulates in a file-scope device word (no allocation) and synchronises
+ * the stream (not capturable). */
 int shx_norm_pow(const float* x, int64_t n, double p, double* out, void* stream);
 
 /* Median absolute deviation / 0.6745 of each of nplanes planes (plane i at
@@ -365,7 +367,8 @@ int shx_softth(float* x, int64_t n, double t, void* stream);
 
 /* Sum |a - b|^p over n elements into *out (host): the reduction of
  * Shearcuda::getNormErr (demo_denoise/shearcuda/shearcuda.cpp:672-717).
- * Allocates a device word and synchronises the stream (not capturable). */
+ * Accumulates in a file-scope device word (no allocation) and synchronises
+ * the stream (not capturable). */
 int shx_norm_pow_diff(const float* a, const float* b, int64_t n, double p, double* out, void* stream);
 
 /* x += N(0, sigma^2) noise from a counter-based generator (splitmix64 of

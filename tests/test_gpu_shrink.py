@@ -70,8 +70,10 @@ def test_shrink_matches_unfused_composition(gpu, L, nimg, per_image, dct):
     plan.inverse_shear_adjoint(gout, Gs)
     gC = torch.empty_like(Gs)
     gth2 = torch.zeros((nimg, nb), dtype=torch.float32, device="cuda")
+    # the mask of the coefficients the backward is given (C2 agrees with C to
+    # rounding, and a coefficient at its threshold can flip between them)
     for i in range(nimg):
-        gC[i] = sep.soft_bands_backward(C2[i:i + 1], Gs[i:i + 1], thr[i].contiguous(), gth2[i], 1, n, n, pl)[0]
+        gC[i] = sep.soft_bands_backward(C[i:i + 1], Gs[i:i + 1], thr[i].contiguous(), gth2[i], 1, n, n, pl)[0]
     gx2 = torch.empty_like(x)
     plan.shear_trans_adjoint(gC, gx2)
     gth_ref = gth2 if per_image else gth2.sum(0)
