@@ -225,6 +225,12 @@ int shx_capture_begin(void* stream);
 int shx_capture_end(void* stream, shx_graph** out);
 int shx_graph_launch(shx_graph* graph, void* stream);
 int shx_graph_destroy(shx_graph* graph);
+/* One graph holding copies of n recorded graphs as child nodes with no order
+ * between them, so one launch runs them all and the runtime may run them
+ * concurrently (e.g. the batch plans of a mixed-size batch).  The parts stay
+ * owned by the caller and may be destroyed afterwards.  No counterpart in the
+ * reference. */
+int shx_graph_parallel(shx_graph* const* parts, int n, shx_graph** out);
 /* Records shx_denoise on an internal stream into a graph. */
 int shx_denoise_graph(shx_dict* dict, const float* x, float* const* coeff, double sigma, const double* Tscalars,
                       const double* scalars, float* out, shx_graph** out_graph);

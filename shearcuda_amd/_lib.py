@@ -61,6 +61,7 @@ _SIGNATURES = [
     ("shx_set_plan_options", c_int, [c_void_p, c_void_p, c_int, c_int]),
     ("shx_get_plan_options", c_int, [c_void_p, c_void_p, P(c_int), P(c_int)]),
     ("shx_capture_begin", c_int, [c_void_p]),
+    ("shx_graph_parallel", c_int, [P(c_void_p), c_int, P(c_void_p)]),
     ("shx_capture_end", c_int, [c_void_p, P(c_void_p)]),
     ("shx_graph_launch", c_int, [c_void_p, c_void_p]),
     ("shx_graph_destroy", c_int, [c_void_p]),

@@ -600,52 +600,23 @@ def capture(fn, keep=()):
 
 
 def capture_parallel(fns, keep=()):
-    """Record independent groups of shx calls into one Graph whose groups can
-    run concurrently: each fn in fns is recorded on its own stream forked from
-    the capture stream and joined back to it, so the replay has no ordering
-    between the groups (e.g. batch plans of different image sizes).
-
-    The fork and join events are created here and kept with the graph: none
-    of them is destroyed while the capture that recorded it is open.
-
-    Disabled: ending such a capture (a fork into streams whose shx calls fork
-    again into their plans' lane streams) crashed the process inside
-    hipStreamEndCapture on the MI355X runtime, also with every event kept
-    alive, and the cause is not established."""
-    raise NotImplementedError("capture_parallel is disabled: ending the capture crashed the process (cause not "
-                              "established); use capture() to record the groups one after another")
-    s = torch.cuda.Stream()
-    side = [torch.cuda.Stream() for _ in fns]
-    evs = []
-
-    def link(waiter, src):
-        e = torch.cuda.Event()
-        e.record(src)
-        waiter.wait_event(e)
-        evs.append(e)
-
-    link(s, torch.cuda.current_stream())
-    with torch.cuda.stream(s):
-        check(lib.shx_capture_begin(ctypes.c_void_p(s.cuda_stream)))
-        err = None
-        try:
-            for fn, si in zip(fns, side):
-                link(si, s)
-                with torch.cuda.stream(si):
-                    fn()
-            for si in side:
-                link(s, si)
-        except Exception as e:  # finish the capture before re-raising
-            err = e
+    """Record independent groups of shx calls into one Graph whose groups have
+    no order between them (e.g. the batch plans of a mixed-size batch): each
+    fn is recorded into a graph of its own, and shx_graph_parallel puts
+    copies of those graphs side by side as child nodes of one graph, so the
+    runtime may run them concurrently.  No stream is forked while a capture
+    is open."""
+    parts = []
+    try:
+        for fn in fns:
+            parts.append(capture(fn))
+        arr = (ctypes.c_void_p * len(parts))(*[p._h.value for p in parts])
         h = ctypes.c_void_p()
-        st = lib.shx_capture_end(ctypes.c_void_p(s.cuda_stream), ctypes.byref(h))
-    link(torch.cuda.current_stream(), s)
-    if err is not None:
-        if st == 0:
-            lib.shx_graph_destroy(h)
-        raise err
-    check(st)
-    return Graph(h, tuple(keep) + tuple(evs) + tuple(side) + (s,))
+        check(lib.shx_graph_parallel(arr, len(parts), ctypes.byref(h)))
+    finally:
+        for p in parts:
+            p.close()
+    return Graph(h, keep)
 
 
 def denoise_graph(x, shear, sigma, scalars, Tscalars=DEFAULT_TSCALARS, Ct=None, out=None, coeffs=True):

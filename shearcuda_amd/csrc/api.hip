@@ -1875,6 +1875,35 @@ int shx_graph_launch(shx_graph* g, void* stream) {
   return SHX_OK;
 }
 
+int shx_graph_parallel(shx_graph* const* parts, int n, shx_graph** out) {
+  if (!out) return fail(SHX_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  if (!parts || n < 1) return fail(SHX_ERR_ARG, "no graphs to combine");
+  for (int i = 0; i < n; ++i)
+    if (!parts[i] || !parts[i]->graph) return fail(SHX_ERR_ARG, "graph %d is null", i);
+  hipGraph_t g = nullptr;
+  SHX_HIP(hipGraphCreate(&g, 0));
+  for (int i = 0; i < n; ++i) {
+    hipGraphNode_t node = nullptr;
+    const hipError_t e = hipGraphAddChildGraphNode(&node, g, nullptr, 0, parts[i]->graph);
+    if (e != hipSuccess) {
+      hipGraphDestroy(g);
+      return fail(SHX_ERR_HIP, "adding graph %d failed: %s", i, hipGetErrorString(e));
+    }
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(g);
+    return fail(SHX_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+  }
+  auto gr = new shx_graph();
+  gr->graph = g;
+  gr->exec = ex;
+  *out = gr;
+  return SHX_OK;
+}
+
 int shx_graph_destroy(shx_graph* g) {
   if (!g) return SHX_OK;
   if (g->exec) hipGraphExecDestroy(g->exec);

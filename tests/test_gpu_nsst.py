@@ -256,9 +256,7 @@ def test_batch_matches_plain(api, L, nimg):
     shear.close()
 
 
-@pytest.mark.parametrize("parallel", [False, pytest.param(True, marks=pytest.mark.skip(
-    reason="capture_parallel is disabled: ending the forked capture crashed the process inside "
-           "hipStreamEndCapture twice (also with every fork/join event kept alive); cause not established"))])
+@pytest.mark.parametrize("parallel", [False, True])
 def test_mixed_size_batches_in_one_graph(api, parallel):
     print("case mixed graph parallel=%s" % parallel, flush=True)
     plans = []
