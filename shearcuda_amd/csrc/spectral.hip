@@ -330,6 +330,56 @@ __device__ __forceinline__ void row_inv_to_B_tw(float2 (&u)[R1], bool validA, in
   }
 }
 
+// row_inv_to_B for long rows (L >= 1024), where holding P[N-k] beside P[k]
+// would double pass A's registers (and halve the waves per SIMD): the
+// gathered row goes through LDS once and each P[N-k] is read back from there.
+// gather(r, n1, x) fills x[n2] = P(n1 + R1 n2) only.
+template <int L, int R1, int R2, int PADSH, class GATHER>
+__device__ __forceinline__ void row_inv_to_B_lds(float2 (&u)[R1], bool validA, int r, int n1, float2* buf,
+                                                 bool validB, int rB, int k2, const float2* __restrict__ tw,
+                                                 GATHER gather) {
+  constexpr int N = L / 2;
+  float2 v[R2];
+  float2 w0 = make_float2(1.f, 0.f);
+  PassTw<R2> pt;
+  if (validA) {
+    gather(r, n1, v);
+    w0 = tw[n1];
+    passA_tw<R2, +1, 2>(pt, n1, tw);
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) buf[lds_ix<PADSH>(r * N + n1 + R1 * n2)] = v[n2];
+  }
+  __syncthreads();
+  if (validA) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int k = n1 + R1 * n2;
+      const float2 xm = buf[lds_ix<PADSH>(r * N + ((N - k) & (N - 1)))];
+      const float2 x = v[n2];
+      v[n2] = (k == 0) ? make_float2(x.x + x.y, x.x - x.y) : c2r_pre(x, xm, ctw_mul<L / R1, -1>(w0, n2));
+    }
+    passA_apply<R2, +1>(v, pt);
+  }
+  __syncthreads();  // every partner read is done before buf takes pass A's output
+  if (validA) {
+#pragma unroll
+    for (int k = 0; k < R2; ++k) buf[lds_ix<PADSH>(r * N + n1 + R1 * k)] = v[k];
+  }
+  __syncthreads();
+  if (validB) {
+#pragma unroll
+    for (int a = 0; a < R1; ++a) u[a] = buf[lds_ix<PADSH>(rB * N + a + R1 * k2)];
+    dft<R1, +1>(u);
+  }
+}
+
+// Loads P(n1 + R1 n2) of one packed row (the gather of row_inv_to_B_lds).
+template <int N, int R1, int R2>
+__device__ __forceinline__ void load_row_x(const float2* __restrict__ row, int n1, float2 (&x)[R2]) {
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) x[n2] = row[n1 + R1 * n2];
+}
+
 // Row real-to-complex FFT (packed output), optional hard threshold on load.
 // grid: (L / ROWS, nplanes, nimg).
 template <int L>
@@ -384,7 +434,22 @@ __global__ void __launch_bounds__(ROWS * Split<L / 2>::R2) k_rows_c2r(PlaneSet i
   float2* dst = reinterpret_cast<float2*>(plane_ptr(out, p, pe, b)) + (size_t)row0 * N;
   const int r = tid / R1, n1 = tid % R1, rB = tid / R2, k2 = tid % R2;
   float2 u[R1];
-  if constexpr (SUM) {
+  if constexpr (SUM && L >= 1024) {
+    const int q0 = sum.first[p], q1 = sum.first[p + 1];
+    auto gather = [&](int rr, int nn, float2 (&x)[R2]) {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) x[n2] = make_float2(0.f, 0.f);
+      for (int q = q0; q < q1; ++q) {
+        float2 a[R2];
+        load_row_x<N, R1, R2>(reinterpret_cast<const float2*>(plane_ptr(in, q, pe, b)) + (size_t)(row0 + rr) * N, nn,
+                              a);
+        __builtin_amdgcn_sched_barrier(0);  // the group's loads in flight together
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) x[n2] = cadd(x[n2], a[n2]);
+      }
+    };
+    row_inv_to_B_lds<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+  } else if constexpr (SUM) {
     const int q0 = sum.first[p], q1 = sum.first[p + 1];
     // groups outermost: each group's R2 pairs of loads are in flight together
     auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
@@ -403,6 +468,10 @@ __global__ void __launch_bounds__(ROWS * Split<L / 2>::R2) k_rows_c2r(PlaneSet i
       }
     };
     row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+  } else if constexpr (L >= 1024) {
+    const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe, b)) + (size_t)row0 * N;
+    auto gather = [&](int rr, int nn, float2 (&x)[R2]) { load_row_x<N, R1, R2>(src + (size_t)rr * N, nn, x); };
+    row_inv_to_B_lds<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
   } else {
     const float2* src = reinterpret_cast<const float2*>(plane_ptr(in, p, pe, b)) + (size_t)row0 * N;
     auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
@@ -440,10 +509,15 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
   // before any store of this block
   const auto ftw = row_fwd_tw<L, R2, R1>(k2, tid % R1, tw);
   float2 u[R1];
-  auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
-    gather_row<N, R1, R2>(src + (size_t)rr * N, nn, x, xm);
-  };
-  row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+  if constexpr (L >= 1024) {
+    auto gather = [&](int rr, int nn, float2 (&x)[R2]) { load_row_x<N, R1, R2>(src + (size_t)rr * N, nn, x); };
+    row_inv_to_B_lds<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+  } else {
+    auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
+      gather_row<N, R1, R2>(src + (size_t)rr * N, nn, x, xm);
+    };
+    row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+  }
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) {
     const float2 c = u[k1];
@@ -486,10 +560,15 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_mask_r2c(PlaneSet Tin
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) cv[k1] = csrc[(size_t)rB * N + k2 + R2 * k1];
   float2 u[R1];
-  auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
-    gather_row<N, R1, R2>(src + (size_t)rr * N, nn, x, xm);
-  };
-  row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+  if constexpr (L >= 1024) {
+    auto gather = [&](int rr, int nn, float2 (&x)[R2]) { load_row_x<N, R1, R2>(src + (size_t)rr * N, nn, x); };
+    row_inv_to_B_lds<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+  } else {
+    auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
+      gather_row<N, R1, R2>(src + (size_t)rr * N, nn, x, xm);
+    };
+    row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+  }
   float acc = 0.f;
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) {
