@@ -342,9 +342,9 @@ def bench_separate(L=512, reps=10):
             "native_s_per_call": round(sn, 4), "native_ms_per_iteration": round(sn * 1e3 / 10, 3)}
 
 
-def bench_adjoint(iters=10, nimg=8, L=512, reps=3):
-    """Unrolled separation, 8 x 512^2 images: ms per unrolled iteration of the
-    forward pass and of forward + backward (adjoint kernels)."""
+def _adjoint_setup(iters=10, nimg=8, L=512):
+    """The unrolled separation of bench_adjoint: (forward, forward + backward,
+    objects to keep alive)."""
     import numpy as np
     import torch
 
@@ -374,6 +374,13 @@ def bench_adjoint(iters=10, nimg=8, L=512, reps=3):
         c, _ = separation.unrolled_separation(xr, th_sh, th_w, plan, h, levels, iters)
         torch.mean((c - gt) ** 2).backward()
 
+    return fwd, fwd_bwd, (shear, plan)
+
+
+def bench_adjoint(iters=10, nimg=8, L=512, reps=3):
+    """Unrolled separation, 8 x 512^2 images: ms per unrolled iteration of the
+    forward pass and of forward + backward (adjoint kernels)."""
+    fwd, fwd_bwd, (shear, plan) = _adjoint_setup(iters, nimg, L)
     # two untimed calls each: the caching allocator's pool for the coefficient
     # stacks kept for the backward (one per iteration) fills over the first two
     t_f = _timed(fwd, reps, 2)
@@ -619,6 +626,111 @@ def traffic_ledger(L, sigma):
         return {"error": str(e)[-800:]}
 
 
+# ---- kernel-trace windows (profiling aid) -------------------------------------------
+WINDOW_GAP_S = 0.05
+
+
+def trace_windows(which):
+    """Set a workload up as the bench does (autotuned), then launch its timed
+    unit three times between idle gaps of WINDOW_GAP_S, so that a kernel trace
+    of this process separates single launches by time (summarize_windows).
+    which: plain (the 512^2 image-only step), batched (the mix in one graph),
+    adjoint (2 unrolled iterations forward, then forward + backward)."""
+    import torch
+
+    from shearcuda_amd import api
+    torch.cuda.set_device(0)
+    units = []
+    keep = []
+    if which == "plain":
+        shear, xg, sc = plain_setup(512, 20.0)
+        api.autotune(shear)
+        g, _, _ = api.denoise_graph(xg, shear, 20.0, sc, coeffs=False)
+        units.append(("plain512", g.launch))
+        keep += [shear, g]
+    elif which == "batched":
+        import numpy as np
+
+        from shearcuda_amd.synth import noisy_ellipses, white_noise
+        rng = np.random.default_rng(2024)
+        plans, seed = [], 100
+        for L, n in BATCH_MIX:
+            shear = api.make_dictionary(L)
+            sc = api.nsst_scalars_cuda(shear, torch.from_numpy(white_noise(L).astype(np.float32)).cuda())
+            sig = rng.uniform(5.0, 40.0, size=n)
+            xs = np.stack([noisy_ellipses(L, sigma=float(sig[i]), seed=seed + i)[1] for i in range(n)])
+            seed += n
+            b = api.Batch(shear, n)
+            api.autotune(shear, b)
+            plans.append(dict(b=b, x=torch.from_numpy(xs.astype(np.float32)).cuda(),
+                              th=api.batch_thresholds(shear, sig, sc), out=torch.empty((n, L, L), device="cuda")))
+            keep += [shear, b]
+        g = api.capture(lambda: [p["b"].denoise(p["x"], None, p["th"], p["out"]) for p in plans])
+        units.append(("batched_mix", g.launch))
+        keep += [g, plans]
+    elif which == "adjoint":
+        fwd, fwd_bwd, hold = _adjoint_setup(iters=2)
+        units += [("adjoint_forward_2iter", fwd), ("adjoint_fwd_bwd_2iter", fwd_bwd)]
+        keep.append(hold)
+    else:
+        raise ValueError(which)
+    for _, fn in units:   # warm: caching allocator pools, first launches
+        fn()
+        fn()
+    torch.cuda.synchronize()
+    for name, fn in units:
+        for _ in range(3):
+            time.sleep(WINDOW_GAP_S)
+            fn()
+            torch.cuda.synchronize()
+            print("window", name, flush=True)
+    time.sleep(WINDOW_GAP_S)
+
+
+def summarize_windows(path, names):
+    """Markdown tables of the launch windows of a kernel trace written under
+    path (rocprofv3 --kernel-trace --output-format csv) by --trace-windows:
+    windows are separated by idle gaps of at least half WINDOW_GAP_S; the last
+    3 * len(names) windows are the measured launches, the middle one of each
+    three is tabulated."""
+    files = glob.glob(os.path.join(path, "**", "*kernel_trace.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no kernel_trace.csv under %s" % path)
+    with open(files[0]) as f:
+        ks = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in csv.DictReader(f))
+    wins, cur, last = [], [], None
+    for k in ks:
+        if last is not None and k[0] - last > WINDOW_GAP_S * 0.5e9:
+            wins.append(cur)
+            cur = []
+        cur.append(k)
+        last = k[1] if last is None else max(last, k[1])
+    wins.append(cur)
+    need = 3 * len(names)
+    if len(wins) < need:
+        raise SystemExit("found %d windows, need %d" % (len(wins), need))
+    out = []
+    for i, name in enumerate(names):
+        w = wins[len(wins) - need + 3 * i + 1]
+        t0 = min(k[0] for k in w)
+        wall = (max(k[1] for k in w) - t0) / 1e3
+        agg = {}
+        for a, b, n in w:
+            short = n.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+            c, t = agg.get(short, (0, 0.0))
+            agg[short] = (c + 1, t + (b - a) / 1e3)
+        tot = sum(t for _, t in agg.values())
+        out.append("### %s: one launch, %d kernels, %.1f us from first start to last end, %.1f us of kernels"
+                   % (name, len(w), wall, tot))
+        out.append("")
+        out.append("| kernel | launches | total (us) | mean (us) | share of kernel time |")
+        out.append("|---|---:|---:|---:|---:|")
+        for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+            out.append("| `%s` | %d | %.1f | %.1f | %.1f%% |" % (n, c, t, t / c, 100.0 * t / tot))
+        out.append("")
+    print("\n".join(out))
+
+
 def measured_traffic(stage, L, sched, algo_bytes):
     """HBM bytes per launch of a stage's kernel from the committed rocprofv3
     counter runs (profiles/hbm_traffic.json, FETCH_SIZE and WRITE_SIZE scaled
@@ -655,9 +767,21 @@ def main():
                     help="run one plain device copy of this many MiB first (HBM counter calibration)")
     ap.add_argument("--no-ledger", action="store_true", help="skip the rocprofv3 traffic ledger of the plain step")
     ap.add_argument("--ledger-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--trace-windows", choices=["plain", "batched", "adjoint"], default=None,
+                    help="profiling aid: launch one workload's timed unit between idle gaps (run under rocprofv3)")
+    ap.add_argument("--summarize-windows", default=None, metavar="DIR",
+                    help="print per-launch kernel tables from a --trace-windows kernel trace under DIR")
     args = ap.parse_args()
     if args.ledger_child:
         ledger_child(json.loads(args.ledger_child))
+        return
+    if args.summarize_windows:
+        names = {"plain": ["plain512"], "batched": ["batched_mix"],
+                 "adjoint": ["adjoint_forward_2iter", "adjoint_fwd_bwd_2iter"]}[args.trace_windows or "plain"]
+        summarize_windows(args.summarize_windows, names)
+        return
+    if args.trace_windows:
+        trace_windows(args.trace_windows)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the NSST path has no multi-GPU form here); "

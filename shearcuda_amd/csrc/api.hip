@@ -1047,7 +1047,9 @@ void stages_shrink_backward(shx_dict* d, const Work& w, int nimg, const Coeffs& 
   const float* thd = t.dev;
   const int ts = thr_stride(d, t);
   st.push_back({"low_mask", 3.0 * P, [=] {
-                  const int blocks = (int)std::min<long long>(256, (pl + 255) / 256);
+                  // few blocks per image: each adds its sum to one threshold
+                  // gradient with an atomic, and all images may share it
+                  const int blocks = (int)std::min<long long>(32, (pl + 255) / 256);
                   hipLaunchKernelGGL(k_low_mask_bwd, dim3(blocks, nimg), dim3(256), 0, s, clow, cls, glow, gls, pl,
                                      thd, ts, gth);
                   return hipGetLastError();
