@@ -1708,6 +1708,24 @@ int shx_batch_inverse_shear(shx_batch* b, const float* coeff, const float* th_de
   return run_stages(st);
 }
 
+// The batched denoise: with the DCT-domain front end every scale's spectrum
+// comes out of one pass, so the per-scale side-stream chains of
+// stages_denoise_dag only add cross-stream edges (measured 2% slower at 32 x
+// 512^2 and 16 x 1024^2) and the single-stream chain runs; with the level
+// pyramid each scale's chain starts as soon as its level is done.
+static int run_batch_denoise(shx_dict* d, shx_batch* b, const float* x, float* coeff, const Thr& t, float* out,
+                             hipStream_t s) {
+  std::vector<Stage> st;
+  const long long pl = (long long)d->L * d->L;
+  const Coeffs c = coeff ? stack_coeffs(d, coeff) : work_coeffs(d, b->w);
+  if (b->w.dct_front) {
+    stages_denoise(d, b->w, b->nimg, x, pl, c, t, out, pl, s, st);
+    return run_stages(st);
+  }
+  stages_denoise_dag(d, b->w, b->nimg, x, pl, c, t, out, pl, s, st);
+  return run_stage_dag(st, b->w);
+}
+
 int shx_batch_denoise(shx_batch* b, const float* x, float* coeff, const float* th_dev, float* out, void* stream) {
   if (int r = batch_args(b, x, out, out)) return r;
   if (!th_dev) return fail(SHX_ERR_ARG, "null threshold table");
@@ -1715,11 +1733,7 @@ int shx_batch_denoise(shx_batch* b, const float* x, float* coeff, const float* t
   Thr t;
   t.mode = 2;
   t.dev = th_dev;
-  std::vector<Stage> st;
-  const long long pl = (long long)d->L * d->L;
-  stages_denoise_dag(d, b->w, b->nimg, x, pl, coeff ? stack_coeffs(d, coeff) : work_coeffs(d, b->w), t, out, pl,
-                     (hipStream_t)stream, st);
-  return run_stage_dag(st, b->w);
+  return run_batch_denoise(d, b, x, coeff, t, out, (hipStream_t)stream);
 }
 
 static int shrink_thr(shx_batch* b, const float* th_dev, int th_stride, Thr& t) {
@@ -1739,12 +1753,7 @@ int shx_batch_shrink(shx_batch* b, const float* x, float* coeff, const float* th
   if (int r = batch_args(b, x, out, out)) return r;
   Thr t;
   if (int r = shrink_thr(b, th_dev, th_stride, t)) return r;
-  shx_dict* d = b->d;
-  std::vector<Stage> st;
-  const long long pl = (long long)d->L * d->L;
-  stages_denoise_dag(d, b->w, b->nimg, x, pl, coeff ? stack_coeffs(d, coeff) : work_coeffs(d, b->w), t, out, pl,
-                     (hipStream_t)stream, st);
-  return run_stage_dag(st, b->w);
+  return run_batch_denoise(b->d, b, x, coeff, t, out, (hipStream_t)stream);
 }
 
 int shx_batch_shrink_backward(shx_batch* b, const float* coeff, const float* th_dev, int th_stride,
