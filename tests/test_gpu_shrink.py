@@ -27,7 +27,7 @@ def thresholds(C, per_image, seed):
 
 @pytest.mark.parametrize("L,nimg,per_image,dct", [(64, 2, False, False), (64, 3, True, True),
                                                   (256, 2, False, True), (256, 2, True, False),
-                                                  (512, 2, False, True)])
+                                                  (512, 2, False, True), (1024, 1, True, True)])
 def test_shrink_matches_unfused_composition(gpu, L, nimg, per_image, dct):
     print("case shrink L=%d nimg=%d per_image=%s dct=%s" % (L, nimg, per_image, dct), flush=True)
     from shearcuda_amd import api, separation as sep
