@@ -438,7 +438,7 @@ def cpu_baseline_3d(n, sigma, nbands_sample=6):
                                                                          nbands_sample, total)}
 
 
-def bench_3d(steps=5, warmup=1, nvol=4, n=128, sigma=25.0, group=8, cpu=True):
+def bench_3d(steps=5, warmup=1, nvol=4, n=128, sigma=25.0, group=8, cpu=True, plane_cols=-1):
     """3D shearlet volume denoise of a batch of 4 x 128^3 FP32 volumes
     (8 random filled ellipsoids + N(0, 25^2) each) as one graph launch:
     Fourier band-pass, 3 levels (Pyr_mode 1), 3 cones with 8x8 / 8x8 / 4x4
@@ -458,6 +458,7 @@ def bench_3d(steps=5, warmup=1, nvol=4, n=128, sigma=25.0, group=8, cpu=True):
     x = torch.tensor(np.stack([p[1] for p in pairs]), dtype=torch.float32, device="cuda")
     th = shearlet3d.thresholds_3d(sh, est, [sigma] * nvol, (3.0, 3.0, 3.0))
     plan = shearlet3d.Plan3D(sh, nvol, group=group)
+    plan.set_schedule(group=group, plane_cols=plane_cols)
     out = torch.empty_like(x)
     g = api.capture(lambda: plan.denoise(x, th, out))
     el = _timed(g.launch, steps, warmup)
@@ -475,7 +476,7 @@ def bench_3d(steps=5, warmup=1, nvol=4, n=128, sigma=25.0, group=8, cpu=True):
             "path_roofline_frac": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "psnr_in": [round(psnr(p[0], p[1]), 3) for p in pairs],
             "psnr_out": [round(psnr(p[0], res[i]), 3) for i, p in enumerate(pairs)],
-            "band_group": group, "dict_bytes": sh.nbytes(), "plan_bytes": plan.nbytes(),
+            "band_group": group, "plane_cols": plane_cols, "dict_bytes": sh.nbytes(), "plan_bytes": plan.nbytes(),
             "dict_prep_s": round(prep_s, 2), "noise_scalars_s": round(scalars_s, 2)}
     if cpu:
         info["cpu_baseline"] = cpu_baseline_3d(n, sigma)
@@ -763,6 +764,8 @@ def main():
                     help="run one secondary workload alone (profiling); prints its result object")
     ap.add_argument("--vol-group", type=int, default=8, help="bands per launch of the 3D path")
     ap.add_argument("--vol-steps", type=int, default=5)
+    ap.add_argument("--vol-plane-cols", type=int, default=-1,
+                    help="3D directional loop: -1 separate row/column passes, 0/32/64 the fused plane kernel")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (HBM counter calibration)")
     ap.add_argument("--no-ledger", action="store_true", help="skip the rocprofv3 traffic ledger of the plain step")
@@ -804,7 +807,7 @@ def main():
         del src, dst
 
     if args.only:
-        r = {"3d": lambda: bench_3d(steps=args.vol_steps, group=args.vol_group),
+        r = {"3d": lambda: bench_3d(steps=args.vol_steps, group=args.vol_group, plane_cols=args.vol_plane_cols),
              "adjoint": bench_adjoint,
              "batched": lambda: bench_batched(max(5, args.steps // 20), 2)}[args.only]()
         print(json.dumps(r), flush=True)
@@ -860,7 +863,8 @@ def main():
         res["adjoint"] = bench_adjoint()
         res["separate"] = bench_separate()
     if not args.no_3d:
-        res["volume"] = bench_3d(steps=args.vol_steps, group=args.vol_group, cpu=args.cpu_budget > 0)
+        res["volume"] = bench_3d(steps=args.vol_steps, group=args.vol_group, cpu=args.cpu_budget > 0,
+                                 plane_cols=args.vol_plane_cols)
     if cpu is not None:
         res["cpu_baseline"] = cpu
     print(json.dumps(res), flush=True)
