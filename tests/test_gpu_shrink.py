@@ -101,3 +101,44 @@ def test_shrink_rejects_bad_threshold_tables(gpu):
         plan.shrink(x, torch.ones((3, shear.nbands), device="cuda"), out)
     plan.close()
     shear.close()
+
+
+@pytest.mark.parametrize("dct", [False, True])
+def test_shrink_identities_at_full_size(gpu, dct):
+    """Size-independent properties at the adjoint benchmark's size (8 x
+    512^2): with thresholds just below zero the shrink is S A = identity (the
+    pyramid and the directional stage reconstruct perfectly) and its backward
+    is (S A)^T = identity; with thresholds above every coefficient the output
+    is exactly zero and the backward passes nothing.  (At a zero threshold the
+    backward's rule 1[|c| > 0] drops the few coefficients that are exactly
+    zero in FP32 -- two in this stack -- so the identity needs t < 0.)"""
+    print("case shrink identities dct=%s" % dct, flush=True)
+    from shearcuda_amd import api
+    L, nimg = 512, 8
+    shear = api.make_dictionary(L)
+    plan = api.Batch(shear, nimg)
+    api.set_plan_options(shear, plan, dct_front=dct, dct_back=dct)
+    rng = np.random.default_rng(91)
+    x = torch.tensor(rng.uniform(0, 255, size=(nimg, L, L)), dtype=torch.float32, device="cuda")
+    g = torch.tensor(rng.standard_normal((nimg, L, L)), dtype=torch.float32, device="cuda")
+    zero = torch.full((shear.nbands,), -1e-30, dtype=torch.float32, device="cuda")
+    out = torch.empty_like(x)
+    C = plan.alloc_coeffs()
+    plan.shrink(x, zero, out, C)
+    gx = torch.empty_like(x)
+    gth = torch.zeros_like(zero)
+    plan.shrink_backward(C, zero, g, gx, gth)
+    big = torch.full((shear.nbands,), 1e30, dtype=torch.float32, device="cuda")
+    out0 = torch.empty_like(x)
+    plan.shrink(x, big, out0, C)
+    gx0 = torch.empty_like(x)
+    gth0 = torch.zeros_like(big)
+    plan.shrink_backward(C, big, g, gx0, gth0)
+    torch.cuda.synchronize()
+    e_fwd, e_bwd = rel(out, x), rel(gx, g)
+    print("S A x vs x %.2e; (S A)^T g vs g %.2e" % (e_fwd, e_bwd), flush=True)
+    assert e_fwd <= 1e-5 and e_bwd <= 1e-5
+    assert torch.count_nonzero(out0) == 0 and torch.count_nonzero(gx0) == 0
+    assert torch.count_nonzero(gth0) == 0
+    plan.close()
+    shear.close()
