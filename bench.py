@@ -732,23 +732,6 @@ def summarize_windows(path, names):
     print("\n".join(out))
 
 
-def measured_traffic(stage, L, sched, algo_bytes):
-    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
-    counter runs (profiles/hbm_traffic.json, FETCH_SIZE and WRITE_SIZE scaled
-    by a 256 MiB copy): the measured launch of the same kernel with the same
-    algorithmic bytes (same direction grouping), or None when not measured."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if not os.path.exists(path):
-        return None
-    names = {"cols_rec": "shx::k_cols_rec<%d, %d>" % (L, sched.get("col_tile_rec", 16)),
-             "cols_dec": "shx::k_cols_dec<%d, %d, true>" % (L, sched.get("col_tile_dec", 16)),
-             "rows_c2r_r2c_bands": "shx::k_rows_c2r_r2c<%d>" % L}
-    for k in json.load(open(path))["kernels"].get(names.get(stage, ""), []):
-        if k.get("algorithmic_bytes") == int(algo_bytes):
-            return round(k["hbm_bytes"])
-    return None
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -817,13 +800,11 @@ def main():
                                                ledger.get("schedule") if ledger else None)
     value = L * L / (ms * 1e-3) / 1e6
     dom = max(stages, key=lambda s: s["ms"])
+    # HBM bytes only from this run's own counter passes (null without them)
     traffic, traffic_src = None, None
     if ledger and "kernels" in ledger and len(ledger["kernels"]) == len(stages):
         traffic = ledger["kernels"][stages.index(dom)]["hbm_bytes"]
         traffic_src = "ledger"
-    else:
-        traffic = measured_traffic(dom["name"], L, sched, dom["bytes"])
-        traffic_src = "profiles/hbm_traffic.json" if traffic is not None else None
     roof = {"kernel": dom["name"], "bound": "hbm", "achieved": round(dom["GBs"], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": traffic_src, "bytes_per_launch": dom["bytes"], "ms_per_launch": round(dom["ms"], 5)}
