@@ -970,24 +970,31 @@ __device__ __forceinline__ void col_block_order(int& tile, int& plane, int& img,
   plane = __builtin_amdgcn_readfirstlane((r / ni) * share + m);
 }
 
-// Block order of the decomposition column passes: plane (band slot) fastest,
-// then column tile, then image, after the same XCD remap.  An XCD then runs
-// every band of a few neighbouring column tiles back to back, so the scale
-// spectrum tile (read by all bands of its scale) and each filter tile (read by
-// the bands sharing it) come from its L2 after the first read: the counted
-// fetch of the 512^2 pass drops from 66 MB to 47 MB against the tile-fastest
-// order, whose XCD ranges re-read the spectrum for every run of bands.  (The
-// reconstruction reads no common input but the shared filter planes and keeps
-// the tile-fastest order, which measured faster there.)
+// Block order of the decomposition column passes, after the same XCD remap:
+// the TG column tiles that share a 128-byte line (TG = 16 / CT for tiles of
+// CT complex columns) fastest, then the plane (band slot), then the tile
+// group, then the image.  An XCD then runs every band of a few neighbouring
+// tiles back to back, so the scale spectrum tile (read by all bands of its
+// scale) and each filter tile (read by the bands sharing it) come from its L2
+// after the first read, and each line is fetched once for the TG tiles that
+// share it.  Measured against the tile-fastest order of the other column
+// passes: 512^2, 8 columns, 28.8 -> 24.8 us (counted fetch 66 -> 47 MB);
+// 1024^2, 8 columns, 91 -> 86 us.  (The reconstruction reads no common input
+// but the shared filter planes and keeps the tile-fastest order, which
+// measured faster there.)
+template <int TG>
 __device__ __forceinline__ void col_block_order_pf(int& tile, int& plane, int& img) {
   const int nt = gridDim.x, np = gridDim.y, ni = gridDim.z;
   const int total = nt * np * ni;
   int lid = blockIdx.x + nt * (blockIdx.y + np * blockIdx.z);
   if (total % 8 == 0) lid = (lid % 8) * (total / 8) + lid / 8;
+  const int tg = nt % TG == 0 ? TG : 1;
+  const int m = lid % tg;
+  lid /= tg;
   plane = __builtin_amdgcn_readfirstlane(lid % np);
-  const int r = lid / np;
-  tile = __builtin_amdgcn_readfirstlane(r % nt);
-  img = __builtin_amdgcn_readfirstlane(r / nt);
+  const int r = lid / np, ng = nt / tg;
+  tile = __builtin_amdgcn_readfirstlane((r % ng) * tg + m);
+  img = __builtin_amdgcn_readfirstlane(r / ng);
 }
 
 // Forward column FFT of every scale spectrum, in place: the column pass of the
@@ -1024,7 +1031,7 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_dec(const float2* _
   constexpr int NH = L / 2;
   __shared__ float2 buf[C::LDS];
   int tile, slot, b;
-  col_block_order_pf(tile, slot, b);
+  col_block_order_pf<(16 / C::CT > 1 ? 16 / C::CT : 1)>(tile, slot, b);
   const int p = map_u8(map.order, slot);
   const int c0 = tile * C::CT;
   const int j = map_u8(map.scale_of, p);
@@ -1130,7 +1137,7 @@ __global__ void __launch_bounds__(CT * 64) k_cols_dec3(const float2* __restrict_
   constexpr int NH = L / 2;
   __shared__ float2 buf[E::LDS];
   int tile, slot, b;
-  col_block_order_pf(tile, slot, b);
+  col_block_order_pf<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
   const int p = map_u8(map.order, slot);
   const int j = map_u8(map.scale_of, p), f = map_u8(map.filt_of, p);
   const int c0 = tile * CT;
@@ -1253,7 +1260,7 @@ __global__ void __launch_bounds__(CT * 128) k_cols_dec3k(const float2* __restric
   using E = Col3m<CT>;
   __shared__ float2 buf[E::LDS];
   int tile, slot, b;
-  col_block_order_pf(tile, slot, b);
+  col_block_order_pf<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
   const int p = map_u8(map.order, slot);
   const int j = map_u8(map.scale_of, p), f = map_u8(map.filt_of, p);
   const int c0 = tile * CT;
@@ -1598,7 +1605,9 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_dec(const float* _
   using C = DctRowCfg<L>;
   constexpr int N = C::N, R1 = C::R1, R2 = C::R2, ROWS = C::ROWS;
   __shared__ float2 buf[C::LDS];
-  __shared__ float vrow[ROWS][L];
+  // the row v' of the inverse in natural order, in buf between the FFTs
+  float (*vrow)[L] = reinterpret_cast<float (*)[L]>(buf);
+  static_assert(ROWS * L <= 2 * C::LDS, "v' fits in the transpose tile");
   const int tid = threadIdx.x;
   const int p = blockIdx.y, b = blockIdx.z;
   const int row0 = blockIdx.x * ROWS;
@@ -1666,6 +1675,7 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_dec(const float* _
   const bool validC = tid < ROWS * R1;
   float2 z[R2];
   row_inv_to_B<L, R2, R1, C::PADSH>(z, true, rB, k2, buf, validC, rC, kc, tw, gather);
+  __syncthreads();  // every pass-B read of buf is complete before v' overwrites it
   // z[k1] = L (v'[2m], v'[2m+1]), m = kc + R1 k1: v' in natural order to LDS
   if (validC) {
 #pragma unroll
@@ -1693,8 +1703,14 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_dec(const float* _
     }
     return;
   }
+  __syncthreads();  // every read of v' is complete before the forward pass writes buf
+  // the forward twiddles again, through an opaque copy of the table pointer so
+  // that the first load's registers are not kept live across the inverse
+  const float2* tw2 = tw;
+  asm volatile("" : "+s"(tw2));
+  const auto ftw2 = row_fwd_tw<L, R1, R2>(n1, k2, tw2);
   float2* dst = spec + (size_t)b * ss + (size_t)p * L * N + (size_t)row0 * N;
-  row_fwd_from_A<L, R1, R2, C::PADSH>(yz, validA, r, n1, buf, dst, true, rB, k2, ftw);
+  row_fwd_from_A<L, R1, R2, C::PADSH>(yz, validA, r, n1, buf, dst, true, rB, k2, ftw2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1926,7 +1942,6 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct_cols_dec(float2* __r
   using C = ColCfg<L, CT_>;
   constexpr int NH = L / 2, R1 = C::R1, R2 = C::R2, CT = C::CT;
   __shared__ float2 buf[C::LDS];
-  __shared__ float2 ybuf[C::LDS];  // the plane's columns in natural order
   int tile, p, b;
   col_block_order(tile, p, b);
   const int c0 = tile * CT;
@@ -1955,11 +1970,13 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct_cols_dec(float2* __r
     col_inv_B_to<L, CT_, false>(buf, [&](int row, int cc, float2 val) { base[(size_t)dct_perm(row, L) * NH + cc] = val; });
     return;
   }
-  // y[perm(n)] = v[n] into the second tile (natural rows), then the forward
-  // FFT from there (its pass A writes buf, whose reads the barrier has ended)
-  col_inv_B_to<L, CT_, false>(buf, [&](int row, int cc, float2 val) { ybuf[C::Lds::ix(dct_perm(row, L), cc)] = val; });
+  // y[perm(n)] = v[n] back into the tile in natural row order (after every
+  // pass-B read), then the forward FFT from there (its pass A writes the tile
+  // after all its reads): one LDS tile, which at L = 1024 doubles the blocks
+  // per CU against a second tile for y
+  col_inv_B_to<L, CT_, true>(buf, [&](int row, int cc, float2 val) { buf[C::Lds::ix(dct_perm(row, L), cc)] = val; });
   __syncthreads();
-  col_fwd_A_src<L, CT_, false>([&](int row, int cc) { return ybuf[C::Lds::ix(row, cc)]; }, buf, tw);
+  col_fwd_A_src<L, CT_, true>([&](int row, int cc) { return buf[C::Lds::ix(row, cc)]; }, buf, tw);
   __syncthreads();
   float2 u[R1];
   col_fwd_B<L, CT_>(u, buf);
@@ -2056,14 +2073,17 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_rec(const float2* 
   using C = DctRowCfg<L>;
   constexpr int N = C::N, R1 = C::R1, R2 = C::R2, ROWS = C::ROWS;
   __shared__ float2 buf[C::LDS];
-  __shared__ float2 yrow[ROWS][N];  // the real row y as pairs (y[2i], y[2i+1])
+  // the real row y as pairs (y[2i], y[2i+1]), in buf between the two FFTs
+  // (one LDS tile per block: at L = 1024 the block count per CU is then set
+  // by registers, not LDS)
+  float2 (*yrow)[N] = reinterpret_cast<float2 (*)[N]>(buf);
+  static_assert(ROWS * N <= C::LDS, "y fits in the transpose tile");
   const int tid = threadIdx.x;
   const int p = blockIdx.y, b = blockIdx.z;
   const int row0 = blockIdx.x * ROWS;
   const int r = tid / R1, n1 = tid % R1;
   const bool validA = r < ROWS;
   const int rB = tid / R2, k2 = tid % R2;
-  const auto ftw = row_fwd_tw<L, R1, R2>(n1, k2, tw);
   const float2 w0 = dtw[k2];
   // z[n] = (y[perm(2n)], y[perm(2n+1)]): (y[4n], y[4n+2]) for n < N/2 and
   // (y[4m+3], y[4m+1]), m = N-1-n, above
@@ -2071,6 +2091,24 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_rec(const float2* 
   if (p < nspec) {
     const int q0 = gf.first[p], q1 = gf.first[p + 1];
     const float2* qb = Qg + (size_t)b * qs;
+    float2 u[R1];
+    if constexpr (L >= 1024) {
+      // one group per round trip and P[N-k] through LDS: the two-group
+      // gather with register partners holds four R2-value arrays (256 VGPRs
+      // at L = 1024).  Same sums in the same order.
+      auto gather = [&](int rr, int nn, float2 (&x)[R2]) {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) x[n2] = make_float2(0.f, 0.f);
+        for (int q = q0; q < q1; ++q) {
+          float2 a[R2];
+          load_row_x<N, R1, R2>(qb + (size_t)q * L * N + (size_t)(row0 + rr) * N, nn, a);
+          __builtin_amdgcn_sched_barrier(0);  // the group's loads in flight together
+#pragma unroll
+          for (int n2 = 0; n2 < R2; ++n2) x[n2] = cadd(x[n2], a[n2]);
+        }
+      };
+      row_inv_to_B_lds<L, R1, R2, C::PADSH>(u, validA, r, n1, buf, true, rB, k2, tw, gather);
+    } else {
     // the scale's partials summed two groups per round trip (an odd last
     // group pairs with itself at weight 0); P[N-k] of the sum from the
     // partner lane (as gather_row)
@@ -2100,11 +2138,12 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_rec(const float2* 
         xm[n2] = nn == 0 ? x[(R2 - n2) & (R2 - 1)] : sh;
       }
     };
-    float2 u[R1];
     row_inv_to_B<L, R1, R2, C::PADSH>(u, validA, r, n1, buf, true, rB, k2, tw, gather);
+    }
+    __syncthreads();  // every pass-B read of buf is complete before y overwrites it
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) yrow[rB][k2 + R2 * k1] = u[k1];
-    __syncthreads();  // also: every pass-B read of buf is complete
+    __syncthreads();
     if (validA) {
 #pragma unroll
       for (int n2 = 0; n2 < R2; ++n2) {
@@ -2132,6 +2171,10 @@ __global__ void __launch_bounds__(DctRowCfg<L>::T) k_dct_rows_rec(const float2* 
       }
     }
   }
+  __syncthreads();  // every read of y is complete before the forward pass writes buf
+  // the forward pass's twiddles only now: loaded at the top they stay live
+  // across the inverse pass
+  const auto ftw = row_fwd_tw<L, R1, R2>(n1, k2, tw);
   float2 P[R1];
   row_fwd_to_P<L, R1, R2, C::PADSH>(v, validA, r, n1, buf, rB, k2, ftw, P);
   dct_row_store<L, R1, R2>(P, k2, w0, D + (size_t)b * ds + (size_t)p * L * L + (size_t)(row0 + rB) * L);
