@@ -505,28 +505,40 @@ __global__ void __launch_bounds__(RowCfg<L>::T) k_rows_c2r_r2c(PlaneSet Tin, Pla
   const int rule = th.mode == 0 ? SHRINK_NONE : (th.soft ? SHRINK_SOFT : SHRINK_HARD);
   const float t = rule ? thresh_of(th, p, b) : 0.0f;
   const int r = tid / R1, n1 = tid % R1, rB = tid / R2, k2 = tid % R2;
-  // twiddles of the forward half (split (R2, R1), pass-A index k2), loaded
-  // before any store of this block
-  const auto ftw = row_fwd_tw<L, R2, R1>(k2, tid % R1, tw);
-  float2 u[R1];
+  // threshold (and the coefficients out), then the forward transform split
+  // (R2, R1): thread k2 is pass-A index n1' = k2
+  auto finish = [&](float2 (&u)[R1], const RowFwdTw<L, R2, R1>& ftw) {
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const float2 c = u[k1];
+      if (store_c) cdst[(size_t)rB * N + k2 + R2 * k1] = c;
+      u[k1] = shrink2(c, t, rule);
+    }
+    __syncthreads();  // pass-B reads of buf are complete
+    row_fwd_from_A<L, R2, R1, C::PADSH>(u, true, rB, k2, buf, udst, tid < C::ROWS * R1, tid / R1, tid % R1, ftw);
+  };
+  // The twiddles of the forward half (pass-A index k2) are loaded before any
+  // store of this block: at the top, or at L >= 1024 after the inverse, so
+  // that they are not live across it (141 -> 121 VGPRs, 3 -> 4 waves per
+  // SIMD, 117 -> 105 us per 1024^2 band stack; at 512 the early load measured
+  // faster).
   if constexpr (L >= 1024) {
+    float2 u[R1];
     auto gather = [&](int rr, int nn, float2 (&x)[R2]) { load_row_x<N, R1, R2>(src + (size_t)rr * N, nn, x); };
     row_inv_to_B_lds<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+    const float2* twf = tw;
+    asm volatile("" : "+s"(twf));  // an opaque copy: the loads stay here
+    const auto ftw = row_fwd_tw<L, R2, R1>(k2, tid % R1, twf);
+    finish(u, ftw);
   } else {
+    const auto ftw = row_fwd_tw<L, R2, R1>(k2, tid % R1, tw);
+    float2 u[R1];
     auto gather = [&](int rr, int nn, float2 (&x)[R2], float2 (&xm)[R2]) {
       gather_row<N, R1, R2>(src + (size_t)rr * N, nn, x, xm);
     };
     row_inv_to_B<L, R1, R2, C::PADSH>(u, r < C::ROWS, r, n1, buf, true, rB, k2, tw, gather);
+    finish(u, ftw);
   }
-#pragma unroll
-  for (int k1 = 0; k1 < R1; ++k1) {
-    const float2 c = u[k1];
-    if (store_c) cdst[(size_t)rB * N + k2 + R2 * k1] = c;
-    u[k1] = shrink2(c, t, rule);
-  }
-  __syncthreads();  // pass-B reads of buf are complete
-  // forward transform split (R2, R1): thread k2 is pass-A index n1' = k2
-  row_fwd_from_A<L, R2, R1, C::PADSH>(u, true, rB, k2, buf, udst, tid < C::ROWS * R1, tid / R1, tid % R1, ftw);
 }
 
 // Backward of a soft-thresholded band read, fused the same way: g = C2R(T)
