@@ -635,19 +635,21 @@ def trace_windows(which):
     """Set a workload up as the bench does (autotuned), then launch its timed
     unit three times between idle gaps of WINDOW_GAP_S, so that a kernel trace
     of this process separates single launches by time (summarize_windows).
-    which: plain (the 512^2 image-only step), batched (the mix in one graph),
-    adjoint (2 unrolled iterations forward, then forward + backward)."""
+    which: plain (the 512^2 image-only step), plain1024 (the same at 1024^2),
+    batched (the mix in one graph), adjoint (2 unrolled iterations forward,
+    then forward + backward)."""
     import torch
 
     from shearcuda_amd import api
     torch.cuda.set_device(0)
     units = []
     keep = []
-    if which == "plain":
-        shear, xg, sc = plain_setup(512, 20.0)
+    if which in ("plain", "plain1024"):
+        L = 512 if which == "plain" else 1024
+        shear, xg, sc = plain_setup(L, 20.0)
         api.autotune(shear)
         g, _, _ = api.denoise_graph(xg, shear, 20.0, sc, coeffs=False)
-        units.append(("plain512", g.launch))
+        units.append(("plain%d" % L, g.launch))
         keep += [shear, g]
     elif which == "batched":
         import numpy as np
@@ -753,7 +755,7 @@ def main():
                     help="run one plain device copy of this many MiB first (HBM counter calibration)")
     ap.add_argument("--no-ledger", action="store_true", help="skip the rocprofv3 traffic ledger of the plain step")
     ap.add_argument("--ledger-child", default=None, help=argparse.SUPPRESS)
-    ap.add_argument("--trace-windows", choices=["plain", "batched", "adjoint"], default=None,
+    ap.add_argument("--trace-windows", choices=["plain", "plain1024", "batched", "adjoint"], default=None,
                     help="profiling aid: launch one workload's timed unit between idle gaps (run under rocprofv3)")
     ap.add_argument("--summarize-windows", default=None, metavar="DIR",
                     help="print per-launch kernel tables from a --trace-windows kernel trace under DIR")
@@ -762,7 +764,7 @@ def main():
         ledger_child(json.loads(args.ledger_child))
         return
     if args.summarize_windows:
-        names = {"plain": ["plain512"], "batched": ["batched_mix"],
+        names = {"plain": ["plain512"], "plain1024": ["plain1024"], "batched": ["batched_mix"],
                  "adjoint": ["adjoint_forward_2iter", "adjoint_fwd_bwd_2iter"]}[args.trace_windows or "plain"]
         summarize_windows(args.summarize_windows, names)
         return

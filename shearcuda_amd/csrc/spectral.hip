@@ -984,16 +984,17 @@ __device__ __forceinline__ void col_block_order(int& tile, int& plane, int& img,
 
 // Block order of the decomposition column passes, after the same XCD remap:
 // the TG column tiles that share a 128-byte line (TG = 16 / CT for tiles of
-// CT complex columns) fastest, then the plane (band slot), then the tile
-// group, then the image.  An XCD then runs every band of a few neighbouring
+// CT complex columns) fastest, then the image, then the plane (band slot),
+// then the tile group.  An XCD then runs every band of a few neighbouring
 // tiles back to back, so the scale spectrum tile (read by all bands of its
-// scale) and each filter tile (read by the bands sharing it) come from its L2
-// after the first read, and each line is fetched once for the TG tiles that
-// share it.  Measured against the tile-fastest order of the other column
-// passes: 512^2, 8 columns, 28.8 -> 24.8 us (counted fetch 66 -> 47 MB);
-// 1024^2, 8 columns, 91 -> 86 us.  (The reconstruction reads no common input
-// but the shared filter planes and keeps the tile-fastest order, which
-// measured faster there.)
+// scale), each filter tile (read by the bands sharing it and by every image
+// of a batch) come from its L2 after the first read, and each line is fetched
+// once for the TG tiles that share it.  Measured against the tile-fastest
+// order of the other column passes: 512^2, 8 columns, 28.8 -> 24.8 us
+// (counted fetch 66 -> 47 MB); 1024^2, 8 columns, 91 -> 86 us; with the image
+// after the tile group instead of before the plane, the batched mix ran 1.5%
+// slower.  (The reconstruction reads no common input but the shared filter
+// planes and keeps the tile-fastest order, which measured faster there.)
 template <int TG>
 __device__ __forceinline__ void col_block_order_pf(int& tile, int& plane, int& img) {
   const int nt = gridDim.x, np = gridDim.y, ni = gridDim.z;
@@ -1003,10 +1004,10 @@ __device__ __forceinline__ void col_block_order_pf(int& tile, int& plane, int& i
   const int tg = nt % TG == 0 ? TG : 1;
   const int m = lid % tg;
   lid /= tg;
+  img = __builtin_amdgcn_readfirstlane(lid % ni);
+  lid /= ni;
   plane = __builtin_amdgcn_readfirstlane(lid % np);
-  const int r = lid / np, ng = nt / tg;
-  tile = __builtin_amdgcn_readfirstlane((r % ng) * tg + m);
-  img = __builtin_amdgcn_readfirstlane(r / ng);
+  tile = __builtin_amdgcn_readfirstlane((lid / np) * tg + m);
 }
 
 // Forward column FFT of every scale spectrum, in place: the column pass of the
