@@ -996,7 +996,7 @@ __device__ __forceinline__ void col_block_order(int& tile, int& plane, int& img,
 // slower.  (The reconstruction reads no common input but the shared filter
 // planes and keeps the tile-fastest order, which measured faster there.)
 template <int TG>
-__device__ __forceinline__ void col_block_order_pf(int& tile, int& plane, int& img) {
+__device__ __forceinline__ void col_block_order_dec(int& tile, int& plane, int& img) {
   const int nt = gridDim.x, np = gridDim.y, ni = gridDim.z;
   const int total = nt * np * ni;
   int lid = blockIdx.x + nt * (blockIdx.y + np * blockIdx.z);
@@ -1044,7 +1044,7 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_dec(const float2* _
   constexpr int NH = L / 2;
   __shared__ float2 buf[C::LDS];
   int tile, slot, b;
-  col_block_order_pf<(16 / C::CT > 1 ? 16 / C::CT : 1)>(tile, slot, b);
+  col_block_order_dec<(16 / C::CT > 1 ? 16 / C::CT : 1)>(tile, slot, b);
   const int p = map_u8(map.order, slot);
   const int c0 = tile * C::CT;
   const int j = map_u8(map.scale_of, p);
@@ -1150,7 +1150,7 @@ __global__ void __launch_bounds__(CT * 64) k_cols_dec3(const float2* __restrict_
   constexpr int NH = L / 2;
   __shared__ float2 buf[E::LDS];
   int tile, slot, b;
-  col_block_order_pf<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
+  col_block_order_dec<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
   const int p = map_u8(map.order, slot);
   const int j = map_u8(map.scale_of, p), f = map_u8(map.filt_of, p);
   const int c0 = tile * CT;
@@ -1273,7 +1273,7 @@ __global__ void __launch_bounds__(CT * 128) k_cols_dec3k(const float2* __restric
   using E = Col3m<CT>;
   __shared__ float2 buf[E::LDS];
   int tile, slot, b;
-  col_block_order_pf<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
+  col_block_order_dec<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
   const int p = map_u8(map.order, slot);
   const int j = map_u8(map.scale_of, p), f = map_u8(map.filt_of, p);
   const int c0 = tile * CT;
