@@ -982,7 +982,8 @@ __device__ __forceinline__ void col_block_order(int& tile, int& plane, int& img,
   plane = __builtin_amdgcn_readfirstlane((r / ni) * share + m);
 }
 
-// Block order of the decomposition column passes, after the same XCD remap:
+// Block order of the decomposition column passes (and of the reconstruction
+// passes up to 512^2), after the same XCD remap:
 // the TG column tiles that share a 128-byte line (TG = 16 / CT for tiles of
 // CT complex columns) fastest, then the image, then the plane (band slot),
 // then the tile group.  An XCD then runs every band of a few neighbouring
@@ -993,10 +994,12 @@ __device__ __forceinline__ void col_block_order(int& tile, int& plane, int& img,
 // order of the other column passes: 512^2, 8 columns, 28.8 -> 24.8 us
 // (counted fetch 66 -> 47 MB); 1024^2, 8 columns, 91 -> 86 us; with the image
 // after the tile group instead of before the plane, the batched mix ran 1.5%
-// slower.  (The reconstruction reads no common input but the shared filter
-// planes and keeps the tile-fastest order, which measured faster there.)
+// slower.  The reconstruction (planes = direction groups) shares only filter
+// tiles; it takes this order for one image up to 512^2 (three-pass 4-column
+// pass 29.3 -> 26.9 us) and the tile-fastest order for batches and at
+// 1024^2, where it measured faster.
 template <int TG>
-__device__ __forceinline__ void col_block_order_dec(int& tile, int& plane, int& img) {
+__device__ __forceinline__ void col_block_order_lines(int& tile, int& plane, int& img) {
   const int nt = gridDim.x, np = gridDim.y, ni = gridDim.z;
   const int total = nt * np * ni;
   int lid = blockIdx.x + nt * (blockIdx.y + np * blockIdx.z);
@@ -1044,7 +1047,7 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_dec(const float2* _
   constexpr int NH = L / 2;
   __shared__ float2 buf[C::LDS];
   int tile, slot, b;
-  col_block_order_dec<(16 / C::CT > 1 ? 16 / C::CT : 1)>(tile, slot, b);
+  col_block_order_lines<(16 / C::CT > 1 ? 16 / C::CT : 1)>(tile, slot, b);
   const int p = map_u8(map.order, slot);
   const int c0 = tile * C::CT;
   const int j = map_u8(map.scale_of, p);
@@ -1090,7 +1093,13 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_cols_rec(const float2* _
   __shared__ float2 buf[C::LDS];
   __shared__ float2 accl[ColRecAcc<L, CT_>::SIZE];
   int tile, slot, b;
-  col_block_order(tile, slot, b, map.gshare);
+  // one image up to 512^2: the line-grouped order (measured 29.6 -> 28.4 us
+  // at 512^2, 4 columns); batches and 1024^2: tile fastest, then image (the
+  // images reading one filter tile together; 112 against 115 us at 1024^2)
+  if (L <= 512 && gridDim.z == 1)
+    col_block_order_lines<(16 / C::CT > 1 ? 16 / C::CT : 1)>(tile, slot, b);
+  else
+    col_block_order(tile, slot, b, map.gshare);
   const int g = map_u16(map.gorder, slot);
   const int c0 = tile * CT;
   const int c = threadIdx.x % CT, k2 = threadIdx.x / CT;
@@ -1150,7 +1159,7 @@ __global__ void __launch_bounds__(CT * 64) k_cols_dec3(const float2* __restrict_
   constexpr int NH = L / 2;
   __shared__ float2 buf[E::LDS];
   int tile, slot, b;
-  col_block_order_dec<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
+  col_block_order_lines<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
   const int p = map_u8(map.order, slot);
   const int j = map_u8(map.scale_of, p), f = map_u8(map.filt_of, p);
   const int c0 = tile * CT;
@@ -1196,7 +1205,10 @@ __global__ void __launch_bounds__(CT * 64) k_cols_rec3(const float2* __restrict_
   constexpr int NH = L / 2;
   __shared__ float2 buf[E::LDS];
   int tile, slot, b;
-  col_block_order(tile, slot, b, map.gshare);
+  if (gridDim.z == 1)  // one image: 29.3 -> 26.9 us at 4 columns
+    col_block_order_lines<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
+  else
+    col_block_order(tile, slot, b, map.gshare);
   const int g = map_u16(map.gorder, slot);
   const int c0 = tile * CT;
   const int c = threadIdx.x % CT, tau = threadIdx.x / CT;
@@ -1273,7 +1285,7 @@ __global__ void __launch_bounds__(CT * 128) k_cols_dec3k(const float2* __restric
   using E = Col3m<CT>;
   __shared__ float2 buf[E::LDS];
   int tile, slot, b;
-  col_block_order_dec<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
+  col_block_order_lines<(16 / CT > 1 ? 16 / CT : 1)>(tile, slot, b);
   const int p = map_u8(map.order, slot);
   const int j = map_u8(map.scale_of, p), f = map_u8(map.filt_of, p);
   const int c0 = tile * CT;
