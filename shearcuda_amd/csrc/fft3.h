@@ -197,4 +197,122 @@ __device__ __forceinline__ void fft1024_inv(float2 (&v)[8], float2* buf, const f
   dft_pair_t<8, +1>(v, hh);
 }
 
+// ---- 1024 points = 16 x 8 x 8 on 64 threads per column (16 values each) ----
+// Thread (c, tau) of a block of CT * 64 threads, c = threadIdx.x % CT, holds
+// v[t] = x[tau + 64 t], t < 16, and gets X[tau + 64 t] back (same layout).
+// With n = a + 64 b (a < 64, b < 16) and k = p + 16 q (p < 16, q = r + 8 s):
+//   stage 1 (thread a = tau):     Z_p[a] = w1024^(a p) DFT16_b x[a + 64 b]
+//   stage 2 (p = tau & 15; a0 = tau >> 4 and a0 + 4):
+//                                 V_{p,a0}[r] = w64^(a0 r) DFT8_a1 Z_p[a0 + 8 a1]
+//   stage 3 (p = tau & 15; r = tau >> 4 and r + 4):
+//                                 X[p + 16 r + 128 s] = DFT8_a0 V_{p,a0}[r]
+// (p + 16 r = tau: the two stage-3 DFTs give rows tau + 64 (2 s) and
+// tau + 64 (2 s + 1)).  LDS slot of row e, column c: e * CT + c + (e / 64) * CT,
+// the layout of the 512-point engine: every access pattern spreads a wave's
+// lanes evenly over the banks for CT = 4 and 8.
+template <int CT>
+struct Col4 {
+  static constexpr int N = 1024, TPC = 64, T = CT * TPC;
+  static constexpr int LDS = N * CT + (N / 64) * CT;
+  __device__ static __forceinline__ int ix(int e, int c) { return e * CT + c + (e >> 6) * CT; }
+};
+
+// w1[p] = w1024^(tau p), w2[r] = w64^(a0 r) (a0 = tau >> 4; the second
+// stage-2 DFT, a0 + 4, takes w2[r] w16^r).  tw[m] = exp(-2 pi i m / 1024).
+struct Fft4Tw {
+  float2 w1[16];
+  float2 w2[8];
+};
+
+__device__ __forceinline__ Fft4Tw fft4_tw(int tau, const float2* __restrict__ tw_table) {
+  const TwView tw(tw_table, 1024);
+  Fft4Tw t;
+  const int a0 = tau >> 4;
+#pragma unroll
+  for (int p = 1; p < 16; ++p) t.w1[p] = tw[tau * p];
+#pragma unroll
+  for (int r = 1; r < 8; ++r) t.w2[r] = tw[16 * a0 * r];
+  return t;
+}
+
+// In place on v (layout above); DIR -1 forward, +1 inverse (unnormalised).
+// buf is written at once (the caller makes sure earlier reads of it are done)
+// and read until return.
+template <int CT, int DIR>
+__device__ __forceinline__ void fft1024_4(float2 (&v)[16], float2* buf, const Fft4Tw& t) {
+  using E = Col4<CT>;
+  const int c = threadIdx.x % CT, tau = threadIdx.x / CT;
+  // stage 1
+  dft<16, DIR>(v);
+#pragma unroll
+  for (int p = 1; p < 16; ++p) v[p] = tw_apply<DIR>(v[p], t.w1[p]);
+#pragma unroll
+  for (int p = 0; p < 16; ++p) buf[E::ix(p * 64 + tau, c)] = v[p];
+  __syncthreads();
+  // stage 2
+  const int p = tau & 15, h = tau >> 4;
+#pragma unroll
+  for (int a1 = 0; a1 < 8; ++a1) {
+    v[a1] = buf[E::ix(p * 64 + h + 8 * a1, c)];
+    v[8 + a1] = buf[E::ix(p * 64 + h + 4 + 8 * a1, c)];
+  }
+  dft8<DIR>(v);
+  dft8<DIR>(v + 8);
+#pragma unroll
+  for (int r = 1; r < 8; ++r) {
+    v[r] = tw_apply<DIR>(v[r], t.w2[r]);
+    v[8 + r] = tw_apply<DIR>(ctw_mul<16, DIR>(v[8 + r], r), t.w2[r]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    buf[E::ix(p * 64 + 8 * r + h, c)] = v[r];
+    buf[E::ix(p * 64 + 8 * r + h + 4, c)] = v[8 + r];
+  }
+  __syncthreads();
+  // stage 3 (r = h and h + 4)
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    v[k] = buf[E::ix(p * 64 + 8 * h + k, c)];
+    v[8 + k] = buf[E::ix(p * 64 + 8 * (h + 4) + k, c)];
+  }
+  dft8<DIR>(v);
+  dft8<DIR>(v + 8);
+  float2 o[16];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    o[2 * s] = v[s];
+    o[2 * s + 1] = v[8 + s];
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = o[k];
+}
+
+// The engines with 64 threads per column (rows tau + 64 s, NV values per
+// thread), by transform length, for kernels written once for both.
+template <int L, int CT>
+struct ColEng;
+template <int CT>
+struct ColEng<512, CT> {
+  static constexpr int NV = 8;
+  using E = Col3<CT>;
+  using Tw = Fft3Tw;
+  __device__ static __forceinline__ Tw twiddles(int tau, const float2* __restrict__ tw) { return fft3_tw(tau, tw); }
+  template <int DIR>
+  __device__ static __forceinline__ void fft(float2 (&v)[NV], float2* buf, const Tw& t) {
+    fft512_3<CT, DIR>(v, buf, t);
+  }
+};
+template <int CT>
+struct ColEng<1024, CT> {
+  static constexpr int NV = 16;
+  using E = Col4<CT>;
+  using Tw = Fft4Tw;
+  __device__ static __forceinline__ Tw twiddles(int tau, const float2* __restrict__ tw) { return fft4_tw(tau, tw); }
+  template <int DIR>
+  __device__ static __forceinline__ void fft(float2 (&v)[NV], float2* buf, const Tw& t) {
+    fft1024_4<CT, DIR>(v, buf, t);
+  }
+};
+
 }  // namespace shx

@@ -2009,16 +2009,19 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct_cols_dec(float2* __r
   for (int k1 = 0; k1 < R1; ++k1) base[(size_t)(k2 + R2 * k1) * NH + c] = u[k1];
 }
 
-// The front's column kernels on the three-pass 512-point engine (fft3.h):
-// thread (c, tau) of CT * 64 threads holds rows tau + 64 s of its complex
-// column.  Same results as k_dct2_cols / k_dct_cols_dec to FP32 rounding.
-template <int CT>
+// The front's column kernels on the engines with 64 threads per column
+// (fft3.h, ColEng: 512 = 8 x 8 x 8, 1024 = 16 x 8 x 8): thread (c, tau) of
+// CT * 64 threads holds rows tau + 64 s of its complex column.  Same results
+// as k_dct2_cols / k_dct_cols_dec to FP32 rounding.
+template <int L, int CT>
 __global__ void __launch_bounds__(CT * 64) k_dct2_cols3(const float* __restrict__ x, long long xs,
                                                     float* __restrict__ out, long long os,
                                                     const float2* __restrict__ tw,
                                                     const float2* __restrict__ dtw) {
-  constexpr int L = 512, NH = L / 2;
-  using E = Col3<CT>;
+  constexpr int NH = L / 2;
+  using G = ColEng<L, CT>;
+  using E = typename G::E;
+  constexpr int NV = G::NV;
   __shared__ float2 buf[E::LDS];
   int tile, plane, b;
   col_block_order(tile, plane, b);
@@ -2026,33 +2029,35 @@ __global__ void __launch_bounds__(CT * 64) k_dct2_cols3(const float* __restrict_
   const float2* src = reinterpret_cast<const float2*>(x + (size_t)b * xs) + c0;
   float2* dst = reinterpret_cast<float2*>(out + (size_t)b * os) + c0;
   const int c = threadIdx.x % CT, tau = threadIdx.x / CT;
-  float2 v[8];
+  float2 v[NV];
 #pragma unroll
-  for (int s = 0; s < 8; ++s) v[s] = src[(size_t)dct_perm(tau + 64 * s, L) * NH + c];
-  const Fft3Tw t = fft3_tw(tau, tw);
+  for (int s = 0; s < NV; ++s) v[s] = src[(size_t)dct_perm(tau + 64 * s, L) * NH + c];
+  const typename G::Tw t = G::twiddles(tau, tw);
   const float2 w0 = dtw[tau];
   __builtin_amdgcn_sched_barrier(0);
-  fft512_3<CT, -1>(v, buf, t);
+  G::template fft<-1>(v, buf, t);
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 8; ++s) buf[E::ix(tau + 64 * s, c)] = v[s];
+  for (int s = 0; s < NV; ++s) buf[E::ix(tau + 64 * s, c)] = v[s];
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
+  for (int s = 0; s < NV; ++s) {
     const int k = tau + 64 * s;
     const float2 vm = buf[E::ix((L - k) & (L - 1), c)];
-    const float2 e = ctw_mul<32, -1>(w0, s);
+    const float2 e = ctw_mul<4 * L / 64, -1>(w0, s);
     dst[(size_t)k * NH + c] = cscale(cadd(cmul(e, v[s]), cmulc(vm, e)), 0.5f);
   }
 }
 
-template <int CT>
+template <int L, int CT>
 __global__ void __launch_bounds__(CT * 64) k_dct_cols_dec3(float2* __restrict__ spec, long long ss, int nspec,
                                                        float* __restrict__ low, long long ls,
                                                        const float2* __restrict__ tw,
                                                        const float2* __restrict__ dtw) {
-  constexpr int L = 512, NH = L / 2;
-  using E = Col3<CT>;
+  constexpr int NH = L / 2;
+  using G = ColEng<L, CT>;
+  using E = typename G::E;
+  constexpr int NV = G::NV;
   __shared__ float2 buf[E::LDS];
   int tile, p, b;
   col_block_order(tile, p, b);
@@ -2061,46 +2066,47 @@ __global__ void __launch_bounds__(CT * 64) k_dct_cols_dec3(float2* __restrict__ 
                              : spec + (size_t)b * ss + (size_t)p * L * NH) + c0;
   const int c = threadIdx.x % CT, tau = threadIdx.x / CT;
   // pre-twiddle V[k] = e^{i t_k} (X[k] - i X[L-k]), X[L] = 0
-  float2 xk[8], xm[8];
+  float2 xk[NV], xm[NV];
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
+  for (int s = 0; s < NV; ++s) {
     const int k = tau + 64 * s;
     xk[s] = base[(size_t)k * NH + c];
     xm[s] = k == 0 ? make_float2(0.f, 0.f) : base[(size_t)(L - k) * NH + c];
   }
-  const Fft3Tw t = fft3_tw(tau, tw);
+  const typename G::Tw t = G::twiddles(tau, tw);
   const float2 w0 = dtw[tau];
   __builtin_amdgcn_sched_barrier(0);
-  float2 v[8];
+  float2 v[NV];
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    const float2 e = ctw_mul<32, -1>(w0, s);
+  for (int s = 0; s < NV; ++s) {
+    const float2 e = ctw_mul<4 * L / 64, -1>(w0, s);
     v[s] = cmulc(make_float2(xk[s].x + xm[s].y, xk[s].y - xm[s].x), e);
   }
-  fft512_3<CT, +1>(v, buf, t);
+  G::template fft<+1>(v, buf, t);
   if (p == nspec) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s) base[(size_t)dct_perm(tau + 64 * s, L) * NH + c] = v[s];
+    for (int s = 0; s < NV; ++s) base[(size_t)dct_perm(tau + 64 * s, L) * NH + c] = v[s];
     return;
   }
   // y[perm(n)] = v[n] into the tile in natural row order, then the forward
   // FFT of y (every read of the tile done before each overwrite)
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 8; ++s) buf[E::ix(dct_perm(tau + 64 * s, L), c)] = v[s];
+  for (int s = 0; s < NV; ++s) buf[E::ix(dct_perm(tau + 64 * s, L), c)] = v[s];
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 8; ++s) v[s] = buf[E::ix(tau + 64 * s, c)];
+  for (int s = 0; s < NV; ++s) v[s] = buf[E::ix(tau + 64 * s, c)];
   __syncthreads();
-  fft512_3<CT, -1>(v, buf, t);
+  G::template fft<-1>(v, buf, t);
 #pragma unroll
-  for (int s = 0; s < 8; ++s) base[(size_t)(tau + 64 * s) * NH + c] = v[s];
+  for (int s = 0; s < NV; ++s) base[(size_t)(tau + 64 * s) * NH + c] = v[s];
 }
 
 // Column tiles: the single-plane first pass takes the narrowest tile (most
-// blocks); the per-plane last pass the narrow tile.  At 512 both run on the
-// three-pass engine (2 and 4 columns; measured 8.6 -> 7.8 and 12.5 -> 10.8 us
-// against the two-pass kernels).
+// blocks); the per-plane last pass the narrow tile.  At 512 and 1024 both run
+// on the engines with 64 threads per column (2 and 4 columns; measured 8.6 ->
+// 7.8 and 12.5 -> 10.8 us at 512^2, 13.9 -> 12.4 and 23.0 -> 22.0 us at
+// 1024^2, against the two-pass kernels).
 template <int L>
 struct LaunchDct {
   static hipError_t dec(const float* x, long long xs, float* xwork, long long ws, float2* spec, long long ss,
@@ -2109,8 +2115,9 @@ struct LaunchDct {
     constexpr int ct = ColTile<L>::narrow ? 4 : 8;
     constexpr int ct1 = 2;
     if (step == 0) {
-      if constexpr (L == 512)
-        hipLaunchKernelGGL((k_dct2_cols3<2>), dim3(L / 2 / 2, 1, nimg), dim3(2 * 64), 0, s, x, xs, xwork, ws, tw, dtw);
+      if constexpr (L == 512 || L == 1024)
+        hipLaunchKernelGGL((k_dct2_cols3<L, 2>), dim3(L / 2 / 2, 1, nimg), dim3(2 * 64), 0, s, x, xs, xwork, ws, tw,
+                           dtw);
       else
         hipLaunchKernelGGL((k_dct2_cols<L, ct1>), dim3(L / 2 / ct1, 1, nimg), dim3(ColCfg<L, ct1>::T), 0, s, x, xs,
                            xwork, ws, tw, dtw);
@@ -2122,9 +2129,9 @@ struct LaunchDct {
         hipLaunchKernelGGL((k_dct_rows_dec<L>), dim3(L / DctRowCfg<L>::ROWS, nspec + 1, nimg),
                            dim3(DctRowCfg<L>::T), 0, s, xwork, ws, spec, ss, nspec, low, ls, mask, tw, dtw);
     } else {
-      if constexpr (L == 512)
-        hipLaunchKernelGGL((k_dct_cols_dec3<4>), dim3(L / 2 / 4, nspec + 1, nimg), dim3(4 * 64), 0, s, spec, ss, nspec,
-                           low, ls, tw, dtw);
+      if constexpr (L == 512 || L == 1024)
+        hipLaunchKernelGGL((k_dct_cols_dec3<L, 4>), dim3(L / 2 / 4, nspec + 1, nimg), dim3(4 * 64), 0, s, spec, ss,
+                           nspec, low, ls, tw, dtw);
       else
         hipLaunchKernelGGL((k_dct_cols_dec<L, ct>), dim3(L / 2 / ct, nspec + 1, nimg), dim3(ColCfg<L, ct>::T), 0, s,
                            spec, ss, nspec, low, ls, tw, dtw);
@@ -2362,18 +2369,21 @@ __global__ void __launch_bounds__(CT_ * Split<L>::R2) k_dct_cols_rec(float* __re
 }
 
 
-// k_dct_cols_rec on the three-pass 512-point engine (fft3.h): thread (c, tau)
+// k_dct_cols_rec on the engines with 64 threads per column (ColEng): thread (c, tau)
 // of CT * 64 threads holds rows tau + 64 s of its complex column; the two
 // spectrum partners (V[L-k], then X[L-k]) are exchanged through the tile.
 // Twice the waves of the two-pass kernel, each with half the values:
-// measured 13.2 -> 11.7 us at 512^2 (4 columns), same results to rounding.
-template <int CT>
+// measured 13.2 -> 11.7 us at 512^2 and 31.4 -> 29.5 us at 1024^2 (4
+// columns), same results to rounding.
+template <int L, int CT>
 __global__ void __launch_bounds__(CT * 64) k_dct_cols_rec3(float* __restrict__ D, long long ds,
                                                        const float* __restrict__ mask,
                                                        const float2* __restrict__ tw,
                                                        const float2* __restrict__ dtw) {
-  constexpr int L = 512, NH = L / 2;
-  using E = Col3<CT>;
+  constexpr int NH = L / 2;
+  using G = ColEng<L, CT>;
+  using E = typename G::E;
+  constexpr int NV = G::NV;
   __shared__ float2 buf[E::LDS];
   int tile, p, b;
   col_block_order(tile, p, b);
@@ -2381,46 +2391,46 @@ __global__ void __launch_bounds__(CT * 64) k_dct_cols_rec3(float* __restrict__ D
   float2* base = reinterpret_cast<float2*>(D + (size_t)b * ds + (size_t)p * L * L) + c0;
   const int c = threadIdx.x % CT, tau = threadIdx.x / CT;
   const float2* mcol = reinterpret_cast<const float2*>(mask + (size_t)p * L * L) + c0 + c;
-  float2 v[8], mk[8];
+  float2 v[NV], mk[NV];
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
+  for (int s = 0; s < NV; ++s) {
     v[s] = base[(size_t)dct_perm(tau + 64 * s, L) * NH + c];
     mk[s] = mcol[(size_t)(tau + 64 * s) * NH];
   }
-  const Fft3Tw t = fft3_tw(tau, tw);
+  const typename G::Tw t = G::twiddles(tau, tw);
   const float2 w0 = dtw[tau];
   __builtin_amdgcn_sched_barrier(0);
-  fft512_3<CT, -1>(v, buf, t);
+  G::template fft<-1>(v, buf, t);
   // X = D2 (two real columns as one complex column) from V[k] and V[L-k]
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < 8; ++s) buf[E::ix(tau + 64 * s, c)] = v[s];
+  for (int s = 0; s < NV; ++s) buf[E::ix(tau + 64 * s, c)] = v[s];
   __syncthreads();
-  float2 x[8];
+  float2 x[NV];
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
+  for (int s = 0; s < NV; ++s) {
     const int k = tau + 64 * s;
     const float2 vm = buf[E::ix((L - k) & (L - 1), c)];
-    const float2 e = ctw_mul<32, -1>(w0, s);
+    const float2 e = ctw_mul<4 * L / 64, -1>(w0, s);
     const float2 X = cscale(cadd(cmul(e, v[s]), cmulc(vm, e)), 0.5f);
     x[s] = make_float2(mk[s].x * X.x, mk[s].y * X.y);
   }
   __syncthreads();  // reads of the exchanged spectrum are complete
 #pragma unroll
-  for (int s = 0; s < 8; ++s) buf[E::ix(tau + 64 * s, c)] = x[s];
+  for (int s = 0; s < NV; ++s) buf[E::ix(tau + 64 * s, c)] = x[s];
   __syncthreads();
   // D2^-1 pre-twiddle V[k] = e^{i t_k} (X[k] - i X[L-k]), X[L] = 0
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
+  for (int s = 0; s < NV; ++s) {
     const int k = tau + 64 * s;
     const float2 xl = k == 0 ? make_float2(0.f, 0.f) : buf[E::ix(L - k, c)];
-    const float2 e = ctw_mul<32, -1>(w0, s);
+    const float2 e = ctw_mul<4 * L / 64, -1>(w0, s);
     v[s] = cmulc(make_float2(x[s].x + xl.y, x[s].y - xl.x), e);
   }
   __syncthreads();
-  fft512_3<CT, +1>(v, buf, t);
+  G::template fft<+1>(v, buf, t);
 #pragma unroll
-  for (int s = 0; s < 8; ++s) base[(size_t)dct_perm(tau + 64 * s, L) * NH + c] = v[s];
+  for (int s = 0; s < NV; ++s) base[(size_t)dct_perm(tau + 64 * s, L) * NH + c] = v[s];
 }
 
 // Sum of the nplanes planes of D, inverse row DCT-II, into x.
@@ -2702,8 +2712,8 @@ struct LaunchDctRec {
         hipLaunchKernelGGL((k_dct_rows_rec<L>), dim3(L / DctRowCfg<L>::ROWS, nspec + 1, nimg),
                            dim3(DctRowCfg<L>::T), 0, s, Qg, qs, gf, nspec, low, ls, lt, D, ds, tw, dtw);
     } else if (step == 1) {
-      if constexpr (L == 512)
-        hipLaunchKernelGGL((k_dct_cols_rec3<4>), dim3(L / 2 / 4, nspec + 1, nimg), dim3(4 * 64), 0, s, D, ds,
+      if constexpr (L == 512 || L == 1024)
+        hipLaunchKernelGGL((k_dct_cols_rec3<L, 4>), dim3(L / 2 / 4, nspec + 1, nimg), dim3(4 * 64), 0, s, D, ds,
                            mask, tw, dtw);
       else
         hipLaunchKernelGGL((k_dct_cols_rec<L, ct>), dim3(L / 2 / ct, nspec + 1, nimg), dim3(ColCfg<L, ct>::T), 0, s, D,
